@@ -1,0 +1,1101 @@
+// CPU oracle for the ePlace mGP/cGP evaluation and iteration.
+//
+// TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg are the only users; the product library never links it.
+//
+// A from-scratch restatement of the reference's CPU path, templated on the
+// arithmetic type T:
+//   T = double : the FP64 truth the GPU kernels are compared against;
+//   T = float  : the reference's arithmetic (FP32 values, FP64 fixed-chunk
+//                reductions), used as the CPU baseline.
+// Every function cites the reference file:line it follows.
+//
+// Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off).  Contraction
+// is off so that the FP32 footprint/overlap arithmetic rounds exactly as the
+// reference's C++ (and as the GPU scatter, which also disables contraction).
+#include "sp_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Fixed-chunk deterministic sums, src/numeric/reduction.h:24-71: 64 chunks
+// [n*c/64, n*(c+1)/64), each summed in order, chunks added in order.
+constexpr int kChunks = 64;
+
+template <typename F>
+double chunkedSum(int64_t n, F&& f) {
+    if (n <= 0) return 0.0;
+    double partial[kChunks];
+#pragma omp parallel for schedule(dynamic, 1) if (n >= (1 << 14))
+    for (int c = 0; c < kChunks; ++c) {
+        const int64_t lo = n * c / kChunks, hi = n * (c + 1) / kChunks;
+        double s = 0.0;
+        for (int64_t i = lo; i < hi; ++i) s += f(i);
+        partial[c] = s;
+    }
+    double total = 0.0;
+    for (int c = 0; c < kChunks; ++c) total += partial[c];
+    return total;
+}
+
+inline bool isMacro(const ora_design* d, int i) { return d->node_flags[i] & ORA_F_MACRO; }
+inline bool isFixed(const ora_design* d, int i) { return d->node_flags[i] & ORA_F_FIXED; }
+inline bool isFiller(const ora_design* d, int i) { return d->node_flags[i] & ORA_F_FILLER; }
+inline bool isNI(const ora_design* d, int i) { return d->node_flags[i] & ORA_F_NI; }
+inline int totalNodes(const ora_design* d) { return d->num_nodes + d->num_fillers; }
+inline double areaOf(const ora_design* d, int i) {
+    return static_cast<double>(d->node_w[i]) * static_cast<double>(d->node_h[i]);
+}
+
+// overlapArea, src/db/geometry.h:35-39.
+template <typename T>
+inline T overlap(T alx, T aly, T ahx, T ahy, T blx, T bly, T bhx, T bhy) {
+    const T dx = std::min(ahx, bhx) - std::max(alx, blx);
+    const T dy = std::min(ahy, bhy) - std::max(aly, bly);
+    return (dx > T(0) && dy > T(0)) ? dx * dy : T(0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Bin grid state.  Values are stored in double; the FP32 path rounds every
+// quantity it computes to float exactly where the reference holds a float.
+struct ora_grid {
+    int precision = 64;
+    int dim = 0;
+    double stepX = 1, stepY = 1, originX = 0, originY = 0, target = 1;
+    double movableAreaScaled = 0;
+    std::vector<double> terminal, base;
+};
+
+namespace {
+
+struct Footprint {
+    int bx0, bx1, by0, by1;
+    double lx, ly, hx, hy, scale;
+};
+
+// BinGrid::footprintOf, bin_grid.cpp:130-162 (local smoothing).
+template <typename T>
+Footprint footprintOf(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+                      int i) {
+    const T w = d->node_w[i], h = d->node_h[i];
+    const T cx = x[i], cy = y[i];
+    const T stepX = T(g->stepX), stepY = T(g->stepY);
+    const T ox = T(g->originX), oy = T(g->originY);
+    T lx = cx - T(0.5) * w, hx = cx + T(0.5) * w;
+    T ly = cy - T(0.5) * h, hy = cy + T(0.5) * h;
+    T sx = 1, sy = 1;
+    if (w < stepX) {
+        sx = w / stepX;
+        lx = cx - T(0.5) * stepX;
+        hx = cx + T(0.5) * stepX;
+    }
+    if (h < stepY) {
+        sy = h / stepY;
+        ly = cy - T(0.5) * stepY;
+        hy = cy + T(0.5) * stepY;
+    }
+    Footprint f;
+    f.lx = lx; f.hx = hx; f.ly = ly; f.hy = hy;
+    f.scale = static_cast<T>(sx * sy);
+    auto cl = [&](T v) { return std::clamp(static_cast<int>(v), 0, g->dim - 1); };
+    f.bx0 = cl((lx - ox) / stepX);
+    f.bx1 = cl((hx - ox) / stepX);
+    f.by0 = cl((ly - oy) / stepY);
+    f.by1 = cl((hy - oy) / stepY);
+    return f;
+}
+
+// BinGrid::chargeWeight, bin_grid.cpp:164-170.
+template <typename T>
+T chargeWeight(const ora_grid* g, const ora_design* d, int i, T scale) {
+    if (isFiller(d, i)) return scale;
+    if (isMacro(d, i)) return scale * T(g->target);
+    return scale;
+}
+
+// Visits every bin overlapped by node i's smoothed footprint with its charge
+// q = overlapArea(bin, rect) * w (bin_grid.cpp:323-339 and :403-421).
+template <typename T, typename V>
+void forEachBinCharge(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+                      int i, V&& visit) {
+    const Footprint f = footprintOf<T>(g, d, x, y, i);
+    const T w = chargeWeight<T>(g, d, i, T(f.scale));
+    const T stepX = T(g->stepX), stepY = T(g->stepY);
+    const T ox = T(g->originX), oy = T(g->originY);
+    for (int bx = f.bx0; bx <= f.bx1; ++bx) {
+        const T blx = ox + T(bx) * stepX;
+        for (int by = f.by0; by <= f.by1; ++by) {
+            const T bly = oy + T(by) * stepY;
+            const T ov = overlap<T>(blx, bly, blx + stepX, bly + stepY, T(f.lx), T(f.ly),
+                                    T(f.hx), T(f.hy)) * w;
+            if (ov <= T(0)) continue;
+            visit(static_cast<size_t>(bx) * g->dim + by, ov);
+        }
+    }
+}
+
+template <typename T>
+void gridInit(ora_grid* g, const ora_design* d, const float* x, const float* y, float target,
+              int binDimOverride) {
+    constexpr int kMaxBinDim = 1024;   // bin_grid.cpp:126
+    g->target = target;
+    g->originX = d->core_lx;
+    g->originY = d->core_ly;
+    // tau denominator and dimension rule, bin_grid.cpp:182-204.
+    double cellArea = 0, macroArea = 0;
+    for (int i = 0; i < d->num_movable; ++i) {
+        if (isFixed(d, i) || isNI(d, i)) continue;
+        if (isMacro(d, i)) macroArea += areaOf(d, i);
+        else cellArea += areaOf(d, i);
+    }
+    g->movableAreaScaled = cellArea + macroArea * static_cast<double>(target);
+    if (binDimOverride > 0) {
+        g->dim = std::min(binDimOverride, kMaxBinDim);
+    } else {
+        const double avgArea =
+            d->num_movable > 0 ? (cellArea + macroArea) / d->num_movable : 1.0;
+        const double idealBinArea = std::max(avgArea / target, 1e-9);
+        const double coreArea = static_cast<double>(d->core_hx - d->core_lx) *
+                                static_cast<double>(d->core_hy - d->core_ly);
+        const double idealCount = coreArea / idealBinArea;
+        g->dim = 4;
+        while (g->dim * 2 <= kMaxBinDim &&
+               static_cast<double>(g->dim * 2) * (g->dim * 2) <= idealCount)
+            g->dim *= 2;
+    }
+    // bin_grid.cpp:206-207: steps are float in the reference.
+    const float coreW = d->core_hx - d->core_lx, coreH = d->core_hy - d->core_ly;
+    if (sizeof(T) == 4) {
+        g->stepX = coreW / static_cast<float>(g->dim);
+        g->stepY = coreH / static_cast<float>(g->dim);
+    } else {
+        g->stepX = static_cast<double>(coreW) / g->dim;
+        g->stepY = static_cast<double>(coreH) / g->dim;
+    }
+    const int dim = g->dim;
+    const size_t nb = static_cast<size_t>(dim) * dim;
+    g->terminal.assign(nb, 0.0);
+    g->base.assign(nb, 0.0);
+    const T stepX = T(g->stepX), stepY = T(g->stepY), ox = T(g->originX), oy = T(g->originY);
+    // terminal density, bin_grid.cpp:228-245.
+    std::vector<T> term(nb, T(0));
+    for (int i = 0; i < d->num_nodes; ++i) {
+        if (!isFixed(d, i) || isNI(d, i)) continue;
+        const T lx = T(x[i]) - T(0.5) * T(d->node_w[i]), hx = T(x[i]) + T(0.5) * T(d->node_w[i]);
+        const T ly = T(y[i]) - T(0.5) * T(d->node_h[i]), hy = T(y[i]) + T(0.5) * T(d->node_h[i]);
+        auto cl = [&](T v) { return std::clamp(static_cast<int>(v), 0, dim - 1); };
+        const int bx0 = cl((lx - ox) / stepX), bx1 = cl((hx - ox) / stepX);
+        const int by0 = cl((ly - oy) / stepY), by1 = cl((hy - oy) / stepY);
+        for (int bx = bx0; bx <= bx1; ++bx) {
+            const T blx = ox + T(bx) * stepX;
+            for (int by = by0; by <= by1; ++by) {
+                const T bly = oy + T(by) * stepY;
+                term[static_cast<size_t>(bx) * dim + by] +=
+                    T(target) * overlap<T>(blx, bly, blx + stepX, bly + stepY, lx, ly, hx, hy);
+            }
+        }
+    }
+    // base density (area not covered by rows), bin_grid.cpp:252-281.
+    std::vector<int> order(static_cast<size_t>(d->num_rows));
+    for (int r = 0; r < d->num_rows; ++r) order[r] = r;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return d->row_ly[a] < d->row_ly[b]; });
+#pragma omp parallel for schedule(static)
+    for (int by = 0; by < dim; ++by) {
+        const T bly = oy + T(by) * stepY, bhy = bly + stepY;
+        size_t lo = 0, hi = order.size();
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            const int r = order[mid];
+            if (T(d->row_ly[r]) + T(d->row_height[r]) <= bly) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int bx = 0; bx < dim; ++bx) {
+            const T blx = ox + T(bx) * stepX;
+            T covered = 0;
+            for (size_t t = lo; t < order.size(); ++t) {
+                const int r = order[t];
+                if (T(d->row_ly[r]) >= bhy) break;
+                covered += overlap<T>(blx, bly, blx + stepX, bhy, T(d->row_lx[r]),
+                                      T(d->row_ly[r]), T(d->row_hx[r]),
+                                      T(d->row_ly[r]) + T(d->row_height[r]));
+            }
+            const T freeArea = std::max(T(0), (blx + stepX - blx) * (bhy - bly) - covered);
+            g->base[static_cast<size_t>(bx) * dim + by] = T(target) * freeArea;
+        }
+    }
+    for (size_t b = 0; b < nb; ++b) g->terminal[b] = term[b];
+}
+
+// BinGrid::accumulate, bin_grid.cpp:299-359: 8 fixed chunks of local grids,
+// then a per-bin sum in chunk order (FP32); FP64 mode sums in double.
+template <typename T>
+void accumulate(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+                double* nodeD, double* fillD) {
+    const size_t nb = static_cast<size_t>(g->dim) * g->dim;
+    const int nc = 8;
+    const int total = totalNodes(d);
+    std::vector<std::vector<T>> ln(nc, std::vector<T>(nb, T(0))), lf(nc, std::vector<T>(nb, T(0)));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < nc; ++c) {
+        const int lo = static_cast<int>(static_cast<int64_t>(total) * c / nc);
+        const int hi = static_cast<int>(static_cast<int64_t>(total) * (c + 1) / nc);
+        T* pn = ln[c].data();
+        T* pf = lf[c].data();
+        for (int i = lo; i < hi; ++i) {
+            if (isFixed(d, i) || isNI(d, i)) continue;
+            const bool filler = isFiller(d, i);
+            forEachBinCharge<T>(g, d, x, y, i, [&](size_t idx, T ov) {
+                if (filler) pf[idx] += ov;
+                else pn[idx] += ov;
+            });
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (size_t b = 0; b < nb; ++b) {
+        T sn = 0, sf = 0;
+        for (int c = 0; c < nc; ++c) {
+            sn += ln[c][b];
+            sf += lf[c][b];
+        }
+        nodeD[b] = sn;
+        fillD[b] = sf;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1-D transforms with FFTW's unnormalised definitions (cpu_poisson.cpp:14-36):
+//   REDFT10: Y_k = 2 sum_j X_j cos(pi (j+1/2) k / n)
+//   REDFT01: Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k+1/2) / n)
+//   RODFT01: Y_k = (-1)^k X_{n-1} + 2 sum_{j<=n-2} X_j sin(pi (j+1)(k+1/2) / n)
+// Returned as a dense n*n matrix M with Y = M X (row k = output index).
+std::vector<double> r2rMatrix(int kind, int n) {
+    std::vector<double> m(static_cast<size_t>(n) * n);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j) {
+            double v;
+            if (kind == 0) {
+                v = 2.0 * std::cos(pi * (j + 0.5) * k / n);
+            } else if (kind == 1) {
+                v = (j == 0) ? 1.0 : 2.0 * std::cos(pi * j * (k + 0.5) / n);
+            } else {
+                if (j == n - 1) v = (k % 2 == 0) ? 1.0 : -1.0;
+                else v = 2.0 * std::sin(pi * (j + 1) * (k + 0.5) / n);
+            }
+            m[static_cast<size_t>(k) * n + j] = v;
+        }
+    return m;
+}
+
+// out[x][k] = sum_y M[k][y] in[x][y]   (transform along the fast axis)
+void applyY(const std::vector<double>& M, int nx, int ny, const double* in, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        const double* row = in + static_cast<size_t>(x) * ny;
+        double* o = out + static_cast<size_t>(x) * ny;
+        for (int k = 0; k < ny; ++k) {
+            const double* m = M.data() + static_cast<size_t>(k) * ny;
+            double s = 0;
+            for (int yy = 0; yy < ny; ++yy) s += m[yy] * row[yy];
+            o[k] = s;
+        }
+    }
+}
+
+// out[j][y] = sum_x M[j][x] in[x][y]   (transform along the slow axis)
+void applyX(const std::vector<double>& M, int nx, int ny, const double* in, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nx; ++j) {
+        double* o = out + static_cast<size_t>(j) * ny;
+        std::fill(o, o + ny, 0.0);
+        const double* m = M.data() + static_cast<size_t>(j) * nx;
+        for (int xx = 0; xx < nx; ++xx) {
+            const double c = m[xx];
+            const double* row = in + static_cast<size_t>(xx) * ny;
+            for (int yy = 0; yy < ny; ++yy) o[yy] += c * row[yy];
+        }
+    }
+}
+
+// CpuPoissonBackend::solve, cpu_poisson.cpp:149-207, evaluated in double.
+void poisson(int nx, int ny, double stepX, double stepY, const double* rho, double* phi,
+             double* ex, double* ey) {
+    const size_t n = static_cast<size_t>(nx) * ny;
+    const double pi = 3.14159265358979323846;
+    std::vector<double> tmp(n), spec(n), work(n);
+    // 1. forward REDFT10 x REDFT10
+    applyY(r2rMatrix(0, ny), nx, ny, rho, tmp.data());
+    applyX(r2rMatrix(0, nx), nx, ny, tmp.data(), spec.data());
+    // 2/3. P = F / (4 nx ny) / (u^2 + v^2), P[0][0] = 0   (cpu_poisson.cpp:163-173)
+    std::vector<double> u(nx), v(ny);
+    for (int j = 0; j < nx; ++j) u[j] = pi * j / (nx * stepX);
+    for (int k = 0; k < ny; ++k) v[k] = pi * k / (ny * stepY);
+    const double invNorm = 1.0 / (4.0 * nx * ny);
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k) {
+            const double den = u[j] * u[j] + v[k] * v[k];
+            double& p = spec[static_cast<size_t>(j) * ny + k];
+            p = den > 0 ? p * invNorm / den : 0.0;
+        }
+    const auto cY = r2rMatrix(1, ny), cX = r2rMatrix(1, nx);
+    // 5. phi = REDFT01 x REDFT01 (P)
+    applyY(cY, nx, ny, spec.data(), tmp.data());
+    applyX(cX, nx, ny, tmp.data(), phi);
+    // 6. xi_x: rows shifted down by one and scaled by u, last row zero (:181-195)
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k)
+            work[static_cast<size_t>(j) * ny + k] =
+                (j == nx - 1) ? 0.0 : spec[static_cast<size_t>(j + 1) * ny + k] * u[j + 1];
+    applyY(cY, nx, ny, work.data(), tmp.data());
+    applyX(r2rMatrix(2, nx), nx, ny, tmp.data(), ex);
+    // 7. xi_y: same on the fast axis (:197-206)
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k)
+            work[static_cast<size_t>(j) * ny + k] =
+                (k == ny - 1) ? 0.0 : spec[static_cast<size_t>(j) * ny + k + 1] * v[k + 1];
+    applyY(r2rMatrix(2, ny), nx, ny, work.data(), tmp.data());
+    applyX(cX, nx, ny, tmp.data(), ey);
+}
+
+// ---------------------------------------------------------------------------
+// WaWirelength::compute, wa_wirelength.cpp:92-226 (two passes, max-shifted
+// exponentials, centred numerators).
+template <typename T>
+double waCompute(const ora_design* d, const float* x, const float* y, float gammaIn,
+                 int ignoreDeg, double* grad) {
+    const int numNets = d->num_nets, numMov = d->num_movable;
+    const T g = std::max(T(gammaIn), T(1e-4f));
+    const T invG = T(1) / g;
+    std::vector<T> ePX(d->num_pins), eNX(d->num_pins), ePY(d->num_pins), eNY(d->num_pins);
+    std::vector<T> bPX(numNets, 0), bNX(numNets), cPX(numNets), cNX(numNets);
+    std::vector<T> bPY(numNets, 0), bNY(numNets), cPY(numNets), cNY(numNets);
+    std::vector<T> mxXv(numNets), mnXv(numNets), mxYv(numNets), mnYv(numNets);
+    auto pinX = [&](int p) { return T(x[d->pin2node[p]]) + T(d->pin_offset_x[p]); };
+    auto pinY = [&](int p) { return T(y[d->pin2node[p]]) + T(d->pin_offset_y[p]); };
+    const double wl = chunkedSum(numNets, [&](int64_t kk) -> double {
+        const int k = static_cast<int>(kk);
+        const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
+        const int deg = e - b;
+        if (deg < 2 || deg > ignoreDeg) {
+            bPX[k] = 0;
+            bPY[k] = 0;
+            return 0.0;
+        }
+        T mxX = -std::numeric_limits<T>::max(), mnX = std::numeric_limits<T>::max();
+        T mxY = -std::numeric_limits<T>::max(), mnY = std::numeric_limits<T>::max();
+        for (int t = b; t < e; ++t) {
+            const int p = d->flat_net2pin[t];
+            mxX = std::max(mxX, pinX(p)); mnX = std::min(mnX, pinX(p));
+            mxY = std::max(mxY, pinY(p)); mnY = std::min(mnY, pinY(p));
+        }
+        mxXv[k] = mxX; mnXv[k] = mnX; mxYv[k] = mxY; mnYv[k] = mnY;
+        T bpx = 0, bnx = 0, cpx = 0, cnx = 0, bpy = 0, bny = 0, cpy = 0, cny = 0;
+        for (int t = b; t < e; ++t) {
+            const int p = d->flat_net2pin[t];
+            const T dxp = pinX(p) - mxX, dxn = pinX(p) - mnX;
+            const T ap = std::exp(dxp * invG), an = std::exp(-dxn * invG);
+            ePX[p] = ap; eNX[p] = an;
+            bpx += ap; cpx += dxp * ap; bnx += an; cnx += dxn * an;
+            const T dyp = pinY(p) - mxY, dyn = pinY(p) - mnY;
+            const T bp = std::exp(dyp * invG), bn = std::exp(-dyn * invG);
+            ePY[p] = bp; eNY[p] = bn;
+            bpy += bp; cpy += dyp * bp; bny += bn; cny += dyn * bn;
+        }
+        bPX[k] = bpx; cPX[k] = cpx; bNX[k] = bnx; cNX[k] = cnx;
+        bPY[k] = bpy; cPY[k] = cpy; bNY[k] = bny; cNY[k] = cny;
+        return static_cast<double>((mxX - mnX) + cpx / bpx - cnx / bnx) +
+               static_cast<double>((mxY - mnY) + cpy / bpy - cny / bny);
+    });
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < numMov; ++i) {
+        T gx = 0, gy = 0;
+        for (int t = d->node2pin_start[i]; t < d->node2pin_start[i + 1]; ++t) {
+            const int p = d->flat_node2pin[t];
+            const int k = d->pin2net[p];
+            if (k < 0) continue;
+            if (bPX[k] <= T(0)) continue;
+            {
+                const T bp = bPX[k], bn = bNX[k], cp = cPX[k], cn = cNX[k];
+                const T dp = pinX(p) - mxXv[k], dn = pinX(p) - mnXv[k];
+                const T tMax = ePX[p] * (bp + (dp * bp - cp) * invG) / (bp * bp);
+                const T tMin = eNX[p] * (bn + (cn - dn * bn) * invG) / (bn * bn);
+                gx += tMax - tMin;
+            }
+            {
+                const T bp = bPY[k], bn = bNY[k], cp = cPY[k], cn = cNY[k];
+                const T dp = pinY(p) - mxYv[k], dn = pinY(p) - mnYv[k];
+                const T tMax = ePY[p] * (bp + (dp * bp - cp) * invG) / (bp * bp);
+                const T tMin = eNY[p] * (bn + (cn - dn * bn) * invG) / (bn * bn);
+                gy += tMax - tMin;
+            }
+        }
+        grad[i] = gx;
+        grad[numMov + i] = gy;
+    }
+    return wl;
+}
+
+// computeHPWL, src/db/hpwl.cpp:13-39 (degree < 2 skipped, large nets kept).
+template <typename T>
+double hpwl(const ora_design* d, const float* x, const float* y) {
+    return chunkedSum(d->num_nets, [&](int64_t kk) -> double {
+        const int k = static_cast<int>(kk);
+        const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
+        if (e - b < 2) return 0.0;
+        T lox = std::numeric_limits<T>::max(), hix = -std::numeric_limits<T>::max();
+        T loy = std::numeric_limits<T>::max(), hiy = -std::numeric_limits<T>::max();
+        for (int t = b; t < e; ++t) {
+            const int p = d->flat_net2pin[t];
+            const T px = T(x[d->pin2node[p]]) + T(d->pin_offset_x[p]);
+            const T py = T(y[d->pin2node[p]]) + T(d->pin_offset_y[p]);
+            lox = std::min(lox, px); hix = std::max(hix, px);
+            loy = std::min(loy, py); hiy = std::max(hiy, py);
+        }
+        return static_cast<double>(hix - lox) + static_cast<double>(hiy - loy);
+    });
+}
+
+double overflowOf(const ora_grid* g, const double* nodeD) {
+    // BinGrid::overflow, bin_grid.cpp:373-393 (fillers excluded, positive part).
+    if (g->movableAreaScaled <= 0.0) return 0.0;
+    const double area = g->stepX * g->stepY;
+    const double target = g->target;
+    const int64_t nb = static_cast<int64_t>(g->dim) * g->dim;
+    const double over = chunkedSum(nb, [&](int64_t b) -> double {
+        const double rho = (nodeD[b] + g->terminal[b] + g->base[b]) / area;
+        const double dd = rho - target;
+        return dd > 0.0 ? dd * area : 0.0;
+    });
+    return over / g->movableAreaScaled;
+}
+
+template <typename T>
+void gather(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+            const double* ex, const double* ey, double* force) {
+    const int total = totalNodes(d);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < total; ++i) {
+        force[i] = 0;
+        force[total + i] = 0;
+        if (isFixed(d, i) || isNI(d, i)) continue;
+        T fx = 0, fy = 0;
+        forEachBinCharge<T>(g, d, x, y, i, [&](size_t idx, T q) {
+            fx += q * T(ex[idx]);
+            fy += q * T(ey[idx]);
+        });
+        force[i] = fx;
+        force[total + i] = fy;
+    }
+}
+
+float computeGamma(float tau, float binStep) {
+    // computeGamma, wa_wirelength.cpp:64-75.
+    float gamma = 8.0f * binStep;
+    if (tau > 1.0f) gamma *= 10.0f;
+    else if (tau < 0.1f) gamma *= 0.1f;
+    else gamma *= std::pow(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
+    return std::max(gamma, 1e-4f);
+}
+
+float lambdaMultiplier(double dHpwl, double ref) {
+    // lambdaMultiplier, src/gp/lambda_schedule.h:22-25.
+    const double mu = std::pow(1.1, -dHpwl / ref + 1.0);
+    return static_cast<float>(std::min(1.05, std::max(0.95, mu)));
+}
+
+// ---------------------------------------------------------------------------
+// NesterovOptimizer, src/numeric/cpu/nesterov.cpp:46-259 (production options:
+// momentum on, step_scale 1, no adaptive restart).
+template <typename T>
+class NesterovOpt {
+public:
+    using ObjFn = std::function<void(const T* pos, T* grad, double* obj)>;
+    using ClampFn = std::function<void(T* pos)>;
+
+    NesterovOpt(ObjFn f, ClampFn c, bool useBB, bool backtrack, int maxBacktrack, float initStep)
+        : f_(std::move(f)), c_(std::move(c)), useBB_(useBB), backtrack_(backtrack),
+          maxBacktrack_(maxBacktrack), initStep_(initStep) {}
+
+    // nesterov.cpp:51-122.
+    void step(T* pos, int n) {
+        if (n <= 0) return;
+        if (!inited_) init(pos, n);
+        f_(u_.data(), g_.data(), &obj_);
+        ++evals_;
+        if (backtrack_ && iter_ > 0) redoIfOvershot(n);
+        gradNorm_ = std::sqrt(sumSq(g_.data(), nullptr, n));
+        alpha_ = (iter_ == 0) ? alpha0_ : estimateStep(n);
+        const T aNext = T(0.5) * (T(1) + std::sqrt(T(4) * a_ * a_ + T(1)));
+        const T momentum = (a_ - T(1)) / aNext;
+        vNew_.resize(n);
+        uNew_.resize(n);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            vNew_[i] = u_[i] - T(alpha_) * g_[i];
+            uNew_[i] = vNew_[i] + momentum * (vNew_[i] - v_[i]);
+        }
+        if (c_) {
+            c_(vNew_.data());
+            c_(uNew_.data());
+        }
+        uPrev_.swap(u_);
+        gPrevTmp_ = g_;
+        v_.swap(vNew_);
+        vPrev_.swap(vNew_);
+        u_.swap(uNew_);
+        momentumPrev_ = momentum;
+        a_ = aNext;
+        ++iter_;
+        std::copy(v_.begin(), v_.end(), pos);
+    }
+    double objective() const { return obj_; }
+    double stepSize() const { return alpha_; }
+    long evaluations() const { return evals_; }
+
+private:
+    static double sumSq(const T* a, const T* b, int n) {
+        return chunkedSum(n, [&](int64_t i) {
+            const double dd = static_cast<double>(a[i]) - (b ? static_cast<double>(b[i]) : 0.0);
+            return dd * dd;
+        });
+    }
+    static double dotDiff(const T* s1, const T* s2, const T* y1, const T* y2, int n) {
+        return chunkedSum(n, [&](int64_t i) {
+            return (static_cast<double>(s1[i]) - static_cast<double>(s2[i])) *
+                   (static_cast<double>(y1[i]) - static_cast<double>(y2[i]));
+        });
+    }
+    // nesterov.cpp:146-169.
+    void redoIfOvershot(int n) {
+        for (int tries = 1; tries < maxBacktrack_; ++tries) {
+            const double dx = std::sqrt(sumSq(u_.data(), uPrev_.data(), n));
+            const double dg = std::sqrt(sumSq(g_.data(), gPrevTmp_.data(), n));
+            if (!(dx > 0.0) || !(dg > 0.0) || !std::isfinite(dg)) return;
+            const double lip = dx / dg;
+            if (lip > 0.95 * alpha_) return;
+            alpha_ = static_cast<T>(lip);
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n; ++i) {
+                v_[i] = uPrev_[i] - T(alpha_) * gPrevTmp_[i];
+                u_[i] = v_[i] + momentumPrev_ * (v_[i] - vPrev_[i]);
+            }
+            if (c_) {
+                c_(v_.data());
+                c_(u_.data());
+            }
+            f_(u_.data(), g_.data(), &obj_);
+            ++evals_;
+        }
+    }
+    // nesterov.cpp:171-183.
+    void init(const T* pos, int n) {
+        u_.assign(pos, pos + n);
+        v_ = u_;
+        uPrev_ = u_;
+        vPrev_ = u_;
+        g_.assign(n, T(0));
+        gPrevTmp_.assign(n, T(0));
+        a_ = 1;
+        alpha0_ = initStep_ > 0.f ? initStep_ : estimateInitialStep(n);
+        inited_ = true;
+    }
+    // nesterov.cpp:190-218.
+    double estimateInitialStep(int n) {
+        std::vector<T> probe(u_), gA(n), gB(n);
+        double dummy = 0;
+        f_(u_.data(), gA.data(), &dummy);
+        const double posNorm = std::sqrt(sumSq(u_.data(), nullptr, n));
+        const double gNorm = std::sqrt(sumSq(gA.data(), nullptr, n));
+        if (!(gNorm > 0.0) || !std::isfinite(gNorm)) return 1.0;
+        const double scale = 0.1 * (posNorm > 0.0 ? posNorm : 1.0) / gNorm;
+        for (int i = 0; i < n; ++i) probe[i] = u_[i] - static_cast<T>(scale) * gA[i];
+        if (c_) c_(probe.data());
+        f_(probe.data(), gB.data(), &dummy);
+        const double dx = std::sqrt(sumSq(probe.data(), u_.data(), n));
+        const double dg = std::sqrt(sumSq(gB.data(), gA.data(), n));
+        if (!(dx > 0.0) || !(dg > 0.0) || !std::isfinite(dg)) return 1.0;
+        const double step = 1.0 / (dg / dx);
+        return (std::isfinite(step) && step > 0.0) ? static_cast<double>(static_cast<T>(step))
+                                                    : 1.0;
+    }
+    // nesterov.cpp:221-240.
+    double estimateStep(int n) {
+        const double dx = std::sqrt(sumSq(u_.data(), uPrev_.data(), n));
+        const double dg = std::sqrt(sumSq(g_.data(), gPrevTmp_.data(), n));
+        if (!(dx > 0.0) || !(dg > 0.0)) return alpha_;
+        const double lipStep = dx / dg;
+        if (useBB_) {
+            const double sy = dotDiff(u_.data(), uPrev_.data(), g_.data(), gPrevTmp_.data(), n);
+            const double yy = dg * dg;
+            if (yy > 0.0) {
+                const double bb = sy / yy;
+                if (bb > 0.0 && std::isfinite(bb)) return static_cast<T>(bb);
+            }
+            return static_cast<T>(std::min(dx / dg, lipStep));
+        }
+        return static_cast<T>(lipStep);
+    }
+
+    ObjFn f_;
+    ClampFn c_;
+    bool useBB_, backtrack_;
+    int maxBacktrack_;
+    float initStep_;
+    std::vector<T> u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrevTmp_;
+    T a_ = 1, momentumPrev_ = 0;
+    double alpha_ = 1, alpha0_ = 1, gradNorm_ = 0, obj_ = 0;
+    long evals_ = 0;
+    int iter_ = 0;
+    bool inited_ = false;
+};
+
+// ---------------------------------------------------------------------------
+// EPlace::run, src/gp/eplace.cpp:223-526.
+template <typename T>
+int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
+              ora_gp_result* res, ora_iter_metrics* rows, int maxRows) {
+    const int stage = cfg->stage;
+    const int total = totalNodes(d);
+    const int nMov = d->num_movable;
+    *res = ora_gp_result{};
+    res->rollback_iter = -1;
+    // buildOptSet, eplace.cpp:82-94.
+    std::vector<int> optIdx;
+    if (stage != 1) {
+        for (int i = 0; i < nMov; ++i) {
+            if (isFixed(d, i) || isNI(d, i)) continue;
+            if (stage == 2 && isMacro(d, i)) continue;
+            optIdx.push_back(i);
+        }
+    }
+    for (int i = d->num_nodes; i < total; ++i) optIdx.push_back(i);
+    const int n = static_cast<int>(optIdx.size());
+    if (n == 0) return 0;
+
+    ora_grid grid;
+    grid.precision = sizeof(T) == 4 ? 32 : 64;
+    gridInit<T>(&grid, d, x, y, cfg->target_density, cfg->gp_bin_dim);
+    const int dim = grid.dim;
+    const size_t nb = static_cast<size_t>(dim) * dim;
+    std::vector<double> nodeD(nb), fillD(nb), rho(nb), phi(nb), ex(nb), ey(nb);
+    std::vector<double> force(2 * static_cast<size_t>(total)), gradWl(2 * static_cast<size_t>(nMov));
+    std::vector<float> cx(x, x + total), cy(y, y + total);   // working db coordinates
+    const float stepX = static_cast<float>(grid.stepX);
+    const double binArea = grid.stepX * grid.stepY;
+
+    // clamp, eplace.cpp:117-133 (kBoundaryEps = 1e-3f, :38).
+    auto clampPos = [&](T* p) {
+        for (int k = 0; k < n; ++k) {
+            const int i = optIdx[k];
+            const T hw = T(0.5) * T(d->node_w[i]), hh = T(0.5) * T(d->node_h[i]);
+            const T xlo = T(d->core_lx) + hw + T(1e-3f);
+            const T xhi = std::max(xlo, T(d->core_hx) - hw - T(1e-3f));
+            const T ylo = T(d->core_ly) + hh + T(1e-3f);
+            const T yhi = std::max(ylo, T(d->core_hy) - hh - T(1e-3f));
+            p[k] = std::min(std::max(p[k], xlo), xhi);
+            p[n + k] = std::min(std::max(p[n + k], ylo), yhi);
+        }
+    };
+    auto scatterPos = [&](const T* p) {
+        for (int k = 0; k < n; ++k) {
+            cx[optIdx[k]] = static_cast<float>(p[k]);
+            cy[optIdx[k]] = static_cast<float>(p[n + k]);
+        }
+    };
+
+    float gamma = 1.f, lambda = 1.f, tau = 0.f;
+    double hp = 0, wlObj = 0, sumWl = 0, sumDen = 0;
+    std::vector<T> lastEval(2 * static_cast<size_t>(n));
+    const int ignoreDeg = cfg->ignore_net_degree;
+
+    // objAndGrad, eplace.cpp:136-203.
+    auto objAndGrad = [&](const T* p, T* grad, double* obj) {
+        scatterPos(p);
+        std::copy(p, p + 2 * n, lastEval.begin());
+        accumulate<T>(&grid, d, cx.data(), cy.data(), nodeD.data(), fillD.data());
+        tau = static_cast<float>(overflowOf(&grid, nodeD.data()));
+        for (size_t b = 0; b < nb; ++b) {
+            if (sizeof(T) == 4) {
+                const float invArea = 1.0f / (static_cast<float>(grid.stepX) *
+                                              static_cast<float>(grid.stepY));
+                rho[b] = (float(nodeD[b]) + float(fillD[b]) + float(grid.terminal[b]) +
+                          float(grid.base[b])) * invArea;
+            } else {
+                rho[b] = (nodeD[b] + fillD[b] + grid.terminal[b] + grid.base[b]) / binArea;
+            }
+        }
+        poisson(dim, dim, grid.stepX, grid.stepY, rho.data(), phi.data(), ex.data(), ey.data());
+        if (sizeof(T) == 4)
+            for (size_t b = 0; b < nb; ++b) {
+                ex[b] = static_cast<float>(ex[b]);
+                ey[b] = static_cast<float>(ey[b]);
+            }
+        gather<T>(&grid, d, cx.data(), cy.data(), ex.data(), ey.data(), force.data());
+        wlObj = waCompute<T>(d, cx.data(), cy.data(), gamma, ignoreDeg, gradWl.data());
+        hp = hpwl<T>(d, cx.data(), cy.data());
+        const T lam = T(lambda);
+        sumWl = chunkedSum(n, [&](int64_t kk) -> double {
+            const int k = static_cast<int>(kk);
+            const int i = optIdx[k];
+            const bool hasWl = i < nMov;
+            const T wlx = hasWl ? T(gradWl[i]) : T(0), wly = hasWl ? T(gradWl[nMov + i]) : T(0);
+            const T fx = T(force[i]), fy = T(force[total + i]);
+            const double degree = d->node2pin_start[i + 1] - d->node2pin_start[i];
+            const T precond = static_cast<T>(
+                1.0 / std::max(1.0, degree + static_cast<double>(lambda) * areaOf(d, i)));
+            grad[k] = precond * (wlx - lam * fx);
+            grad[n + k] = precond * (wly - lam * fy);
+            return hasWl ? std::fabs(double(wlx)) + std::fabs(double(wly)) : 0.0;
+        });
+        sumDen = chunkedSum(n, [&](int64_t kk) -> double {
+            const int i = optIdx[static_cast<size_t>(kk)];
+            return std::fabs(double(T(force[i]))) + std::fabs(double(T(force[total + i])));
+        });
+        *obj = wlObj;
+    };
+
+    std::vector<T> pos(2 * static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        pos[k] = cx[optIdx[k]];
+        pos[n + k] = cy[optIdx[k]];
+    }
+    clampPos(pos.data());
+    scatterPos(pos.data());
+
+    // lambda initialisation, eplace.cpp:247-259.
+    gamma = computeGamma(1.f, stepX);
+    lambda = 1.f;
+    {
+        std::vector<T> probe(2 * static_cast<size_t>(n));
+        double dummy = 0;
+        objAndGrad(pos.data(), probe.data(), &dummy);
+        gamma = computeGamma(tau, stepX);
+        lambda = sumDen > 0.0 ? static_cast<float>(sumWl / sumDen) : 1.f;
+        if (!std::isfinite(lambda) || lambda <= 0.f) lambda = 1.f;
+    }
+    const float targetTau = stage == 2 ? cfg->cgp_target_overflow : cfg->target_overflow;
+    const int maxIter = stage == 1 ? cfg->filler_only_iters : cfg->gp_max_iter;
+    // hpwlReference, lambda_schedule.h:11-20.
+    double effRef = (cfg->lambda_relative && stage != 1) ? hp * cfg->delta_hpwl_relative
+                                                         : cfg->delta_hpwl_ref;
+    if (hp == 0.0 && cfg->lambda_relative && stage != 1) effRef = cfg->delta_hpwl_ref;
+    res->initial_hpwl = hp;
+    res->initial_overflow = tau;
+    res->initial_lambda = lambda;
+
+    auto deliver = [&](const T* p) {
+        std::vector<T> q(p, p + 2 * n);
+        clampPos(q.data());
+        scatterPos(q.data());
+        accumulate<T>(&grid, d, cx.data(), cy.data(), nodeD.data(), fillD.data());
+        res->overflow = static_cast<float>(overflowOf(&grid, nodeD.data()));
+        res->hpwl = hpwl<T>(d, cx.data(), cy.data());
+        for (int k = 0; k < n; ++k) {
+            x[optIdx[k]] = cx[optIdx[k]];
+            y[optIdx[k]] = cy[optIdx[k]];
+        }
+    };
+
+    // already below target: do not move, eplace.cpp:278-296.
+    if (stage != 1 && tau < targetTau) {
+        res->converged = 1;
+        deliver(pos.data());
+        return 0;
+    }
+
+    NesterovOpt<T> opt(objAndGrad, clampPos, cfg->use_bb != 0, cfg->gp_backtrack != 0,
+                       cfg->max_backtrack > 0 ? cfg->max_backtrack : 10, 0.f);
+    struct Snap {
+        float tau;
+        double hpwl;
+        int iter;
+        std::vector<T> pos;
+    };
+    std::vector<Snap> snaps;
+    float bestTau = std::numeric_limits<float>::infinity();
+    float refTau = std::numeric_limits<float>::infinity();
+    int sinceImprove = 0;
+    const float minGain = std::clamp(cfg->gp_stagnation_min_gain, 0.f, 0.5f);
+    double prevHpwl = 0;
+    bool havePrev = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int iter = 0; iter < maxIter; ++iter) {
+        const float evalLambda = lambda, evalGamma = gamma;
+        opt.step(pos.data(), 2 * n);
+        // lambda / gamma schedules, eplace.cpp:371-380.
+        const double dH = havePrev ? hp - prevHpwl : 0.0;
+        const float mu = havePrev ? lambdaMultiplier(dH, effRef) : 1.f;
+        if (havePrev) lambda *= mu;
+        prevHpwl = hp;
+        havePrev = true;
+        gamma = computeGamma(tau, stepX);
+        if (rows && iter < maxRows) {
+            ora_iter_metrics& m = rows[iter];
+            m.iter = iter;
+            m.evaluations = static_cast<int32_t>(opt.evaluations());
+            m.hpwl = hp; m.overflow = tau; m.lambda = lambda; m.gamma = gamma;
+            m.step_size = opt.stepSize();
+            m.grad_norm_wl = static_cast<float>(sumWl);
+            m.grad_norm_den = static_cast<float>(sumDen);
+            m.eval_lambda = evalLambda; m.eval_gamma = evalGamma;
+            m.delta_hpwl = dH; m.lambda_mu = mu;
+            m.elapsed_ms = std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - t0).count();
+        }
+        res->iterations = iter + 1;
+        if (stage != 1 && tau < targetTau) {
+            res->converged = 1;
+            pos = lastEval;   // deliver the look-ahead point u, eplace.cpp:412-419
+            break;
+        }
+        // stagnation guard with tau-ladder snapshots, eplace.cpp:421-473.
+        if (stage != 1 && cfg->gp_stagnation_window > 0) {
+            if (tau < refTau * (1.f - minGain)) {
+                refTau = tau;
+                sinceImprove = 0;
+            } else {
+                ++sinceImprove;
+            }
+            bestTau = std::min(bestTau, tau);
+            const float tol = bestTau * (1.f + minGain);
+            while (!snaps.empty() && snaps.front().tau > tol) snaps.erase(snaps.begin());
+            if (snaps.empty() || tau * (1.f + 0.5f * minGain) <= snaps.back().tau)
+                snaps.push_back(Snap{tau, hp, iter, lastEval});
+            if (sinceImprove >= cfg->gp_stagnation_window) {
+                const Snap* pick = &snaps.front();
+                for (const Snap& s : snaps)
+                    if (s.hpwl < pick->hpwl) pick = &s;
+                pos = pick->pos;
+                res->stagnated = 1;
+                res->rollback_iter = pick->iter;
+                break;
+            }
+        }
+    }
+    res->seconds_loop =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    deliver(pos.data());
+    res->evaluations = opt.evaluations();
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+extern "C" {
+
+ora_grid* ora_grid_create(const ora_design* d, const float* x, const float* y,
+                          float target_density, int bin_dim_override, int precision) {
+    ora_grid* g = new ora_grid;
+    g->precision = precision;
+    if (precision == 32) gridInit<float>(g, d, x, y, target_density, bin_dim_override);
+    else gridInit<double>(g, d, x, y, target_density, bin_dim_override);
+    return g;
+}
+
+void ora_grid_destroy(ora_grid* g) { delete g; }
+
+void ora_grid_info(const ora_grid* g, double* o) {
+    o[0] = g->dim; o[1] = g->stepX; o[2] = g->stepY;
+    o[3] = g->originX; o[4] = g->originY; o[5] = g->movableAreaScaled;
+}
+
+void ora_grid_static(const ora_grid* g, double* terminal, double* base) {
+    std::copy(g->terminal.begin(), g->terminal.end(), terminal);
+    std::copy(g->base.begin(), g->base.end(), base);
+}
+
+void ora_accumulate(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+                    double* nodeD, double* fillD) {
+    if (g->precision == 32) accumulate<float>(g, d, x, y, nodeD, fillD);
+    else accumulate<double>(g, d, x, y, nodeD, fillD);
+}
+
+void ora_accumulate_fixed(const ora_grid* g, const ora_design* d, const float* x,
+                          const float* y, int fracBits, int64_t* nodeQ, int64_t* fillQ) {
+    const size_t nb = static_cast<size_t>(g->dim) * g->dim;
+    std::fill(nodeQ, nodeQ + nb, 0);
+    std::fill(fillQ, fillQ + nb, 0);
+    const float scale = std::ldexp(1.0f, fracBits);
+    for (int i = 0; i < totalNodes(d); ++i) {
+        if (isFixed(d, i) || isNI(d, i)) continue;
+        const bool filler = isFiller(d, i);
+        forEachBinCharge<float>(g, d, x, y, i, [&](size_t idx, float ov) {
+            const int64_t q = std::llrint(ov * scale);
+            if (filler) fillQ[idx] += q;
+            else nodeQ[idx] += q;
+        });
+    }
+}
+
+double ora_overflow(const ora_grid* g, const double* nodeD) { return overflowOf(g, nodeD); }
+
+void ora_density_rho(const ora_grid* g, const double* nodeD, const double* fillD, double* rho) {
+    const size_t nb = static_cast<size_t>(g->dim) * g->dim;
+    const double area = g->stepX * g->stepY;
+    for (size_t b = 0; b < nb; ++b) rho[b] = (nodeD[b] + fillD[b] + g->terminal[b] + g->base[b]) / area;
+}
+
+void ora_gather(const ora_grid* g, const ora_design* d, const float* x, const float* y,
+                const double* ex, const double* ey, double* force) {
+    if (g->precision == 32) gather<float>(g, d, x, y, ex, ey, force);
+    else gather<double>(g, d, x, y, ex, ey, force);
+}
+
+double ora_total_charge(const ora_grid* g, const ora_design* d, const float* x, const float* y) {
+    double s = 0;
+    for (int i = 0; i < totalNodes(d); ++i) {
+        if (isFixed(d, i) || isNI(d, i)) continue;
+        forEachBinCharge<double>(g, d, x, y, i, [&](size_t, double q) { s += q; });
+    }
+    return s;
+}
+
+void ora_poisson(int nx, int ny, double stepX, double stepY, const double* rho, double* phi,
+                 double* ex, double* ey, int precision) {
+    poisson(nx, ny, stepX, stepY, rho, phi, ex, ey);
+    if (precision == 32) {
+        const size_t n = static_cast<size_t>(nx) * ny;
+        for (size_t i = 0; i < n; ++i) {
+            phi[i] = static_cast<float>(phi[i]);
+            ex[i] = static_cast<float>(ex[i]);
+            ey[i] = static_cast<float>(ey[i]);
+        }
+    }
+}
+
+void ora_r2r_1d(int kind, int n, const double* in, double* out) {
+    const auto m = r2rMatrix(kind, n);
+    for (int k = 0; k < n; ++k) {
+        double s = 0;
+        for (int j = 0; j < n; ++j) s += m[static_cast<size_t>(k) * n + j] * in[j];
+        out[k] = s;
+    }
+}
+
+float ora_compute_gamma(float tau, float binStep) { return computeGamma(tau, binStep); }
+
+double ora_wa(const ora_design* d, const float* x, const float* y, float gamma, int ignoreDeg,
+              double* grad, int precision) {
+    if (precision == 32) return waCompute<float>(d, x, y, gamma, ignoreDeg, grad);
+    return waCompute<double>(d, x, y, gamma, ignoreDeg, grad);
+}
+
+double ora_hpwl(const ora_design* d, const float* x, const float* y, int precision) {
+    if (precision == 32) return hpwl<float>(d, x, y);
+    return hpwl<double>(d, x, y);
+}
+
+float ora_lambda_multiplier(double dH, double ref) { return lambdaMultiplier(dH, ref); }
+
+void ora_filler_plan(const ora_design* d, const float* x, const float* y, float target,
+                     float rowHeight, double totalRowArea, double* out) {
+    // initializeFillers, src/gp/filler.cpp:47-139 (count and size only; the
+    // positions are drawn by the caller's own seeded generator).
+    double overlapA = 0;
+    for (int i = d->num_movable; i < d->num_nodes; ++i) {
+        if (isNI(d, i)) continue;
+        const float lx = x[i] - 0.5f * d->node_w[i], hx = x[i] + 0.5f * d->node_w[i];
+        const float ly = y[i] - 0.5f * d->node_h[i], hy = y[i] + 0.5f * d->node_h[i];
+        for (int r = 0; r < d->num_rows; ++r) {
+            const float rly = d->row_ly[r], rhy = d->row_ly[r] + d->row_height[r];
+            if (rly >= hy) break;
+            if (rhy <= ly) continue;
+            overlapA += overlap<float>(lx, ly, hx, hy, d->row_lx[r], rly, d->row_hx[r], rhy);
+        }
+    }
+    const double white = std::max(0.0, totalRowArea - overlapA);
+    double cellArea = 0, macroArea = 0;
+    for (int i = 0; i < d->num_movable; ++i) {
+        if (isMacro(d, i)) macroArea += areaOf(d, i);
+        else cellArea += areaOf(d, i);
+    }
+    const double nodeScaled = cellArea + macroArea * static_cast<double>(target);
+    const double fillerArea = white * static_cast<double>(target) - nodeScaled;
+    out[0] = 0; out[1] = 0; out[2] = 0;
+    out[3] = white; out[4] = nodeScaled; out[5] = fillerArea;
+    if (d->num_movable == 0) return;
+    std::vector<double> areas;
+    for (int i = 0; i < d->num_movable; ++i) areas.push_back(areaOf(d, i));
+    std::sort(areas.begin(), areas.end());
+    const int lo = static_cast<int>(0.10 * d->num_movable);
+    const int hi = static_cast<int>(0.90 * d->num_movable);
+    double avg = 0;
+    if (hi > lo) {
+        for (int i = lo; i < hi; ++i) avg += areas[i];
+        avg /= (hi - lo);
+    } else {
+        avg = areas[d->num_movable / 2];
+    }
+    const float h = rowHeight > 0.f ? rowHeight : 1.f;
+    float w = static_cast<float>(avg) / h;
+    if (!(w > 0.f)) w = 1.f;
+    const double single = static_cast<double>(w) * h;
+    const double raw = fillerArea / std::max(single, 1e-9);
+    out[0] = std::max(0, static_cast<int>(raw + 0.5));
+    out[1] = w;
+    out[2] = h;
+}
+
+double ora_nesterov_quadratic(int n, const float* coef, const float* target, float* pos,
+                              int steps, int useBB, int backtrack, float initStep,
+                              float clampHi, long* evals) {
+    auto f = [&](const float* p, float* g, double* obj) {
+        double o = 0;
+        for (int i = 0; i < n; ++i) {
+            const double dd = static_cast<double>(p[i]) - target[i];
+            o += 0.5 * coef[i] * dd * dd;
+            g[i] = static_cast<float>(coef[i] * dd);
+        }
+        *obj = o;
+    };
+    std::function<void(float*)> c;
+    if (std::isfinite(clampHi))
+        c = [&](float* p) {
+            for (int i = 0; i < n; ++i) p[i] = std::min(p[i], clampHi);
+        };
+    NesterovOpt<float> opt(f, c, useBB != 0, backtrack != 0, 10, initStep);
+    for (int k = 0; k < steps; ++k) opt.step(pos, n);
+    if (evals) *evals = opt.evaluations();
+    return opt.objective();
+}
+
+int ora_eplace_run(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
+                   ora_gp_result* res, ora_iter_metrics* rows, int maxRows) {
+    if (cfg->precision == 32) return eplaceRun<float>(d, x, y, cfg, res, rows, maxRows);
+    return eplaceRun<double>(d, x, y, cfg, res, rows, maxRows);
+}
+
+void ora_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int ora_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

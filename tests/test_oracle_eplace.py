@@ -1,0 +1,115 @@
+"""Pins the oracle's full stage loop (eplace.cpp:223-526 restated) by the
+behavioural checks of test/test_eplace.cpp: density sign (cells spread, tau
+drops, converges), sanity (finite, in core), no move when already converged,
+determinism, quality band, stagnation guard and its rollback point, cGP keeps
+macros fixed; plus FP32-vs-FP64 final HPWL agreement."""
+import numpy as np
+import pytest
+
+import oracle
+from superplace_amd import design as D
+
+
+def run(d, **kw):
+    cfg = oracle.default_config(**kw)
+    return oracle.eplace_run(d, cfg)
+
+
+def test_cells_spread_sign_and_quality_band():
+    d = D.with_fillers(D.fixture_eplace(3000, 12))
+    g = oracle.Grid(d, 1.0, 0)
+    tau0 = g.overflow(g.accumulate(d.node_x, d.node_y)[0])
+    res, rows, x, y = run(d)
+    h0 = oracle.hpwl(d)
+    assert tau0 > 0.5 and res.converged and res.overflow < 0.10
+    assert 1.0 < res.hpwl / h0 < 60.0
+    assert 30 < res.iterations < 500
+    # sanity: finite and inside the core
+    m = ~(d.node_flags & (D.F_FIXED | D.F_NI)).astype(bool)
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
+    lx, ly, hx, hy = d.core
+    assert np.all(x[m] - d.node_w[m] / 2 >= lx - 1e-2) and np.all(x[m] + d.node_w[m] / 2 <= hx + 1e-2)
+    assert np.all(y[m] - d.node_h[m] / 2 >= ly - 1e-2) and np.all(y[m] + d.node_h[m] / 2 <= hy + 1e-2)
+    # schedule alignment (test_eplace.cpp:371-386): eval lambda/gamma are the previous row's
+    for a, b in zip(rows[:-1], rows[1:]):
+        assert b.eval_lambda == a.lambda_ and b.eval_gamma == a.gamma
+        assert b.evaluations > a.evaluations
+        assert b.delta_hpwl == pytest.approx(b.hpwl - a.hpwl, abs=0)
+
+
+def test_fp32_fp64_agree():
+    d = D.with_fillers(D.fixture_eplace(2000, 8))
+    r64, *_ = run(d, precision=64)
+    r32, *_ = run(d, precision=32)
+    assert r64.converged and r32.converged
+    assert abs(r32.hpwl / r64.hpwl - 1) < 0.02
+
+
+def test_already_converged_does_not_move():
+    d = D.fixture_eplace(200, 0)
+    d.node_x[:200] = 100 + (np.arange(200) % 40) * 25.0
+    d.node_y[:200] = 100 + (np.arange(200) // 40) * 100.0
+    h0 = oracle.hpwl(d)
+    res, rows, x, y = run(d)
+    assert res.iterations == 0 and res.converged
+    assert res.hpwl <= h0 * 1.001 + 1.0
+
+
+def test_determinism():
+    d = D.with_fillers(D.fixture_eplace(1000, 4))
+    a = run(d)
+    b = run(d)
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+def overfull():
+    """test_eplace.cpp:115-180: 3000 cells + 40 macros 208x208 (1.4x core)."""
+    base = D.fixture_eplace(3000, 0)
+    rng = D.Rng(11)
+    n = base.num_movable
+    x = np.concatenate([base.node_x[:n], 600 + rng.uniform(40, -20, 20), base.node_x[n:]])
+    y = np.concatenate([base.node_y[:n], 600 + rng.uniform(40, -20, 20), base.node_y[n:]])
+    w = np.concatenate([base.node_w[:n], np.full(40, 208.0), base.node_w[n:]])
+    h = np.concatenate([base.node_h[:n], np.full(40, 208.0), base.node_h[n:]])
+    f = np.concatenate([base.node_flags[:n], np.full(40, D.F_MACRO, np.uint8), base.node_flags[n:]])
+    nm = n + 40
+    nn = nm + 4
+    p2n, p2e, ox, oy = [], [], [], []
+    nets = 0
+    for i in range(nm):
+        a, b, c = i, (i * 7 + 3) % nm, (i * 13 + 11) % nn
+        if a == b or b == c or a == c:
+            continue
+        for nd in (a, b, c):
+            p2n.append(nd); ox.append(0.25 * w[nd]); oy.append(-0.25 * h[nd]); p2e.append(nets)
+        nets += 1
+    rows = (base.row_ly, base.row_height, base.row_lx, base.row_hx)
+    return D.make_design(x, y, w, h, f, nm, p2n, p2e, ox, oy, nets, rows)
+
+
+@pytest.mark.slow
+def test_stagnation_guard_and_rollback():
+    d = D.with_fillers(overfull())
+    assert d.num_fillers == 0
+    g, grows, *_ = run(d, gp_max_iter=500, gp_stagnation_window=40)
+    u, urows, *_ = run(d, gp_max_iter=500, gp_stagnation_window=0)
+    assert not g.converged and g.stagnated and g.overflow > 0.2
+    assert g.iterations < u.iterations == 500
+    assert g.overflow <= u.overflow * 1.05 and g.hpwl < u.hpwl * 0.5
+    # rollback point: delivered metrics equal the recorded row; lowest HPWL on the tau ladder
+    rb = [r for r in grows if r.iter == g.rollback_iter][0]
+    assert g.overflow == pytest.approx(rb.overflow, rel=1e-6, abs=1e-7)
+    assert g.hpwl == pytest.approx(rb.hpwl, rel=1e-10)
+    best = min(grows, key=lambda r: r.overflow)
+    assert rb.overflow <= best.overflow * 1.02 + 1e-7 and rb.hpwl <= best.hpwl
+
+
+def test_cgp_keeps_macros_fixed():
+    d = D.with_fillers(D.fixture_eplace(1000, 6))
+    res, rows, x, y = run(d)
+    d2 = d.copy()
+    d2.node_x, d2.node_y = x, y
+    mac = np.where((d2.node_flags[:d2.num_movable] & D.F_MACRO) != 0)[0]
+    d2.node_flags[mac] |= D.F_FIXED
+    r2, _, x2, y2 = run(d2, stage=2)
+    assert np.array_equal(x2[mac], x[mac]) and np.array_equal(y2[mac], y[mac])
