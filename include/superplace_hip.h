@@ -1,0 +1,211 @@
+/* superPlace-HIP: C ABI of the MI355X (gfx950) backend for the ePlace
+ * mGP/cGP iteration.
+ *
+ * Plain pointers and sizes only; no HIP, torch or C++ types cross the
+ * boundary.  Every function returns an sp_hip_status; on failure
+ * sp_hip_last_error() gives a thread-local message.  Each declaration names
+ * the reference interface it replaces (file:line in the reference tree).
+ *
+ * Threading: an sp_hip_poisson or sp_hip_gp handle is used by one thread at a
+ * time (the reference calls PoissonBackend::solve from one thread per
+ * instance, poisson_backend.h:29-37); separate handles may be used
+ * concurrently, each owns its own HIP stream.
+ */
+#ifndef SUPERPLACE_HIP_H
+#define SUPERPLACE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sp_hip_status {
+    SP_HIP_OK = 0,
+    SP_HIP_ENODEV = 1,      /* no usable GPU: the caller falls back to its CPU backend,
+                               as makePoissonBackend does when the CUDA factory returns
+                               nullptr (src/numeric/poisson_factory.cpp:14-21) */
+    SP_HIP_EINVAL = 2,      /* bad argument (std::runtime_error in the reference,
+                               src/numeric/cpu/cpu_poisson.cpp:78-83, :220-223) */
+    SP_HIP_EOOM = 3,        /* device allocation failed */
+    SP_HIP_EDEVICE = 4,     /* HIP runtime error */
+    SP_HIP_ENONFINITE = 5,  /* non-finite coordinate / gradient / metric
+                               (InvariantError "non_finite", src/gp/eplace.cpp:140-142, :192-197) */
+    SP_HIP_EINTERNAL = 6
+} sp_hip_status;
+
+enum { SP_HIP_FLAG_FP64 = 1 };   /* FP64 arithmetic everywhere (the plain FP64 path) */
+
+/* Thread-local message of the last failing call on this thread. */
+const char* sp_hip_last_error(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int sp_hip_device_count(int* count);
+
+/* ---------------------------------------------------------------- Poisson
+ * Replaces PoissonBackend (src/numeric/poisson_backend.h:20-46) and its
+ * factory makePoissonBackend(BackendConfig) (poisson_backend.h:48-60,
+ * src/numeric/poisson_factory.cpp:14-21).  Solves lap(phi) = -rho with
+ * Neumann boundaries and returns the field xi = -grad(phi) (not a gradient),
+ * with the conventions of src/numeric/cpu/cpu_poisson.cpp:3-40.  Grids are
+ * row-major with y fastest, data[x*ny + y] (src/numeric/bin_grid_span.h:6-19).
+ * All buffers are allocated at create time; solve does not allocate. */
+typedef struct sp_hip_poisson sp_hip_poisson;
+
+/* BackendConfig{nx, ny, step_x, step_y}; flags: SP_HIP_FLAG_FP64.
+ * nx, ny >= 2 (any size: the transforms are padded internally).
+ * Returns SP_HIP_ENODEV when no GPU is present. */
+int sp_hip_poisson_create(int nx, int ny, float step_x, float step_y, int flags,
+                          sp_hip_poisson** out);
+/* PoissonBackend::solve(density, phi, field_x, field_y) on host spans of
+ * nx*ny floats (cpu_poisson.cpp:149-207); synchronous. */
+int sp_hip_poisson_solve(sp_hip_poisson* p, const float* density, float* phi, float* field_x,
+                         float* field_y);
+/* Device-resident variant: density_dev holds nx*ny values and out3_dev
+ * 3*nx*ny values (phi | field_x | field_y), float (or double with
+ * SP_HIP_FLAG_FP64); enqueued on the handle's stream, then synchronised.
+ * No counterpart in the reference (its CUDA backend always crosses to the
+ * host, cuda_poisson.cu:194-210); it is the data path of the engine below. */
+int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* out3_dev);
+/* PoissonBackend::name(): "hip-mfma" or "hip-mfma-fp64". */
+const char* sp_hip_poisson_name(const sp_hip_poisson* p);
+void sp_hip_poisson_destroy(sp_hip_poisson* p);
+
+/* ---------------------------------------------------------------- GP engine
+ * Device-resident evaluation and iteration of one placement stage: the data
+ * of PlaceDB (src/db/place_db.h:27-71) and the loop of EPlace::run
+ * (src/gp/eplace.cpp:223-526) with NesterovOptimizer
+ * (src/numeric/cpu/nesterov.cpp:51-240). */
+
+typedef struct sp_hip_gp sp_hip_gp;
+
+/* Pure SoA + CSR database, node order [movable | fixed | filler], centre
+ * coordinates.  node2pin_start has num_nodes + num_fillers + 1 entries
+ * (fillers own no pins, src/gp/filler.cpp:129-130).  Rows give the core
+ * (their bounding box) and the base density (bin_grid.cpp:252-281). */
+typedef struct sp_hip_design {
+    int32_t num_movable, num_nodes, num_fillers, num_pins, num_nets, num_rows;
+    const float* node_x;
+    const float* node_y;
+    const float* node_w;
+    const float* node_h;
+    const uint8_t* node_flags;   /* 1 macro, 2 fixed, 4 filler, 8 terminal_NI */
+    const float* pin_offset_x;
+    const float* pin_offset_y;
+    const int32_t* pin2node;
+    const int32_t* pin2net;
+    const int32_t* net2pin_start;
+    const int32_t* flat_net2pin;
+    const int32_t* node2pin_start;
+    const int32_t* flat_node2pin;
+    const float* row_ly;
+    const float* row_height;
+    const float* row_lx;
+    const float* row_hx;
+    float core_lx, core_ly, core_hx, core_hy;
+} sp_hip_design;
+
+/* The GP fields of Config (src/util/config.h:77-118, configs/default.json). */
+typedef struct sp_hip_gp_config {
+    float target_density;
+    float target_overflow;
+    float cgp_target_overflow;
+    int32_t gp_max_iter;
+    int32_t ignore_net_degree;
+    int32_t gp_bin_dim;            /* 0 = automatic power of two (bin_grid.cpp:190-204) */
+    float delta_hpwl_ref;
+    int32_t lambda_relative;       /* lambda_schedule == "relative" */
+    double delta_hpwl_relative;
+    int32_t use_bb;
+    int32_t gp_backtrack;
+    int32_t filler_only_iters;
+    int32_t gp_stagnation_window;
+    float gp_stagnation_min_gain;
+    int32_t stage;                 /* 0 mGP, 1 FILLERONLY, 2 cGP (src/gp/eplace.h:24-28) */
+    int32_t precision;             /* 32: FP32 bulk + FP64 reductions; 64: FP64 everywhere */
+    int32_t max_backtrack;         /* OptConfig::max_backtrack (src/numeric/optimizer.h:223) */
+} sp_hip_gp_config;
+
+/* One metrics.csv row (src/util/metrics.h:12-28). */
+typedef struct sp_hip_iter_metrics {
+    int32_t iter;
+    int32_t evaluations;
+    double hpwl, overflow, lambda, gamma, step_size, grad_norm_wl, grad_norm_den;
+    double eval_lambda, eval_gamma, delta_hpwl, lambda_mu, elapsed_ms;
+} sp_hip_iter_metrics;
+
+/* GpResult (src/gp/eplace.h:32-42) plus the stage's initial values. */
+typedef struct sp_hip_gp_result {
+    int32_t iterations, converged, stagnated, rollback_iter;
+    int64_t evaluations;
+    double hpwl, overflow;
+    double initial_hpwl, initial_overflow, initial_lambda;
+    double seconds_loop;
+} sp_hip_gp_result;
+
+/* FP64 scalars of one objAndGrad evaluation (eplace.cpp:136-203). */
+typedef struct sp_hip_eval_scalars {
+    double tau, wl, hpwl, sum_wl, sum_den, dx2, dg2, sy, gg, nonfinite;
+} sp_hip_eval_scalars;
+
+/* Optional host outputs of one evaluation (any pointer may be NULL):
+ * bin grids dim*dim (node_q / filler_q: the raw fixed-point sums, value =
+ * q * 2^-frac_bits); force/grad_wl/grad 2*n_opt in pos order (x half, y half). */
+typedef struct sp_hip_eval_debug {
+    int64_t* node_q;
+    int64_t* filler_q;
+    double* node_density;
+    double* filler_density;
+    double* rho;
+    double* field_x;
+    double* field_y;
+    double* force;
+    double* grad_wl;
+    double* grad;
+} sp_hip_eval_debug;
+
+typedef struct sp_hip_gp_info {
+    int32_t dim;              /* bins per side */
+    int32_t n_opt;            /* optimised nodes (pos has 2*n_opt entries) */
+    int32_t frac_bits;        /* fixed-point fraction bits of the density scatter */
+    int32_t precision;
+    double step_x, step_y, origin_x, origin_y;
+    double movable_area_scaled;   /* tau denominator (bin_grid.cpp:182-188) */
+} sp_hip_gp_info;
+
+/* Per kernel family: total ms and launches measured with HIP events on the
+ * engine's stream (enabled by sp_hip_gp_set_profiling). */
+enum { SP_HIP_K_SCATTER = 0, SP_HIP_K_FINALIZE, SP_HIP_K_POISSON, SP_HIP_K_WA_NETS,
+       SP_HIP_K_NODE_GRAD, SP_HIP_K_UPDATE, SP_HIP_K_COUNT };
+
+/* Builds the device copy of the design and the bin grid for one stage
+ * (EPlace::EPlace + BinGrid::initialize, eplace.cpp:205-221). */
+int sp_hip_gp_create(const sp_hip_design* d, const sp_hip_gp_config* cfg, sp_hip_gp** out);
+int sp_hip_gp_info_get(const sp_hip_gp* gp, sp_hip_gp_info* info);
+/* The optimised node indices, in pos order (buildOptSet, eplace.cpp:82-94). */
+int sp_hip_gp_opt_index(const sp_hip_gp* gp, int32_t* idx);
+/* One objAndGrad at host positions x, y of ALL nodes (written as given,
+ * like EPlace's scatter), with lambda and gamma.  dbg may be NULL. */
+int sp_hip_gp_eval(sp_hip_gp* gp, const float* x, const float* y, float lambda, float gamma,
+                   sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg);
+/* EPlace::run(stage): runs the stage to convergence, stagnation or the
+ * iteration limit.  rows may be NULL; at most max_rows rows are written. */
+int sp_hip_gp_run(sp_hip_gp* gp, sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int max_rows);
+/* Current node positions (all nodes; after run: the delivered placement). */
+int sp_hip_gp_get_positions(const sp_hip_gp* gp, float* x, float* y);
+/* mask: bit f (SP_HIP_K_*) times kernel family f with HIP events; 0 = off. */
+int sp_hip_gp_set_profiling(sp_hip_gp* gp, int mask);
+int sp_hip_gp_kernel_stats(const sp_hip_gp* gp, double* ms, int64_t* launches, double* bytes,
+                           double* flops);
+void sp_hip_gp_destroy(sp_hip_gp* gp);
+
+/* Batched C = A * B of row-major matrices (dims multiples of 64) on the
+ * matrix cores, the primitive of the Poisson transforms.  Device pointers,
+ * float (or double with SP_HIP_FLAG_FP64); synchronous.  No counterpart in
+ * the reference (FFTW / cuFFT there). */
+int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b,
+                void* const* c, int m, int n, int k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

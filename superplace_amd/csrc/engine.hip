@@ -1,0 +1,1175 @@
+// Host side of the HIP backend: the device-resident placement instance, the
+// ePlace stage loop driving the kernels, the Poisson backend and the C ABI.
+//
+// The loop follows EPlace::run (src/gp/eplace.cpp:223-526) and
+// NesterovOptimizer (src/numeric/cpu/nesterov.cpp:51-240).  Each evaluation
+// is a fixed chain of kernels on the instance's stream; the host reads back
+// only the ten FP64 scalars of the evaluation to take the data-dependent
+// decisions (backtracking, step size, schedules, stopping).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/superplace_hip.h"
+#include "gp_kernels.h"
+#include "poisson.h"
+
+namespace sphip {
+namespace {
+
+thread_local std::string g_lastError;
+
+struct InvalidArg : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NonFinite : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return SP_HIP_OK;
+    } catch (const InvalidArg& e) {
+        g_lastError = e.what();
+        return SP_HIP_EINVAL;
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return SP_HIP_EINVAL;
+    } catch (const NonFinite& e) {
+        g_lastError = e.what();
+        return SP_HIP_ENONFINITE;
+    } catch (const NoDevice& e) {
+        g_lastError = e.what();
+        return SP_HIP_ENODEV;
+    } catch (const HipError& e) {
+        g_lastError = e.what();
+        return std::strstr(e.what(), "out of memory") ? SP_HIP_EOOM : SP_HIP_EDEVICE;
+    } catch (const std::bad_alloc& e) {
+        g_lastError = "host allocation failed";
+        return SP_HIP_EOOM;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return SP_HIP_EINTERNAL;
+    }
+}
+
+void requireDevice() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        throw NoDevice("no HIP device available");
+    }
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    SPHIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T, typename S>
+T* dupload(const std::vector<S>& h, hipStream_t s) {
+    std::vector<T> t(h.begin(), h.end());
+    T* d = dalloc<T>(t.size());
+    if (!t.empty())
+        SPHIP_CHECK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    SPHIP_CHECK(hipStreamSynchronize(s));
+    return d;
+}
+
+// computeGamma, src/gp/wa_wirelength.cpp:64-75.
+float computeGamma(float tau, float binStep) {
+    float gamma = 8.0f * binStep;
+    if (tau > 1.0f) gamma *= 10.0f;
+    else if (tau < 0.1f) gamma *= 0.1f;
+    else gamma *= std::pow(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
+    return std::max(gamma, 1e-4f);
+}
+
+// lambdaMultiplier, src/gp/lambda_schedule.h:22-25.
+float lambdaMultiplier(double dH, double ref) {
+    const double mu = std::pow(1.1, -dH / ref + 1.0);
+    return static_cast<float>(std::min(1.05, std::max(0.95, mu)));
+}
+
+template <typename T>
+inline T overlapT(T alx, T aly, T ahx, T ahy, T blx, T bly, T bhx, T bhy) {
+    const T dx = std::min(ahx, bhx) - std::max(alx, blx);
+    const T dy = std::min(ahy, bhy) - std::max(aly, bly);
+    return (dx > T(0) && dy > T(0)) ? dx * dy : T(0);
+}
+
+void validateDesign(const sp_hip_design& d) {
+    if (d.num_movable < 0 || d.num_nodes < d.num_movable || d.num_fillers < 0 ||
+        d.num_pins < 0 || d.num_nets < 0 || d.num_rows < 0)
+        throw InvalidArg("design: inconsistent counts");
+    const int total = d.num_nodes + d.num_fillers;
+    if (total > 0 && (!d.node_x || !d.node_y || !d.node_w || !d.node_h || !d.node_flags))
+        throw InvalidArg("design: missing node arrays");
+    if (!d.net2pin_start || !d.node2pin_start)
+        throw InvalidArg("design: missing CSR starts");
+    if (d.num_pins > 0 && (!d.pin_offset_x || !d.pin_offset_y || !d.pin2node || !d.pin2net ||
+                           !d.flat_net2pin || !d.flat_node2pin))
+        throw InvalidArg("design: missing pin arrays");
+    if (d.net2pin_start[0] != 0 || d.net2pin_start[d.num_nets] != d.num_pins)
+        throw InvalidArg("design: net2pin_start does not span the pins");
+    if (d.node2pin_start[0] != 0 || d.node2pin_start[total] != d.num_pins)
+        throw InvalidArg("design: node2pin_start does not span the pins");
+    for (int k = 0; k < d.num_nets; ++k)
+        if (d.net2pin_start[k + 1] < d.net2pin_start[k]) throw InvalidArg("design: net CSR not monotone");
+    for (int i = 0; i < total; ++i)
+        if (d.node2pin_start[i + 1] < d.node2pin_start[i])
+            throw InvalidArg("design: node CSR not monotone");
+    for (int t = 0; t < d.num_pins; ++t) {
+        if (d.flat_net2pin[t] < 0 || d.flat_net2pin[t] >= d.num_pins ||
+            d.flat_node2pin[t] < 0 || d.flat_node2pin[t] >= d.num_pins)
+            throw InvalidArg("design: CSR pin index out of range");
+        if (d.pin2node[t] < 0 || d.pin2node[t] >= d.num_nodes)
+            throw InvalidArg("design: pin2node out of range");
+        if (d.pin2net[t] >= d.num_nets) throw InvalidArg("design: pin2net out of range");
+    }
+    if (!(d.core_hx > d.core_lx) || !(d.core_hy > d.core_ly)) throw InvalidArg("design: empty core");
+    for (int i = 0; i < total; ++i)
+        if (!std::isfinite(d.node_x[i]) || !std::isfinite(d.node_y[i]))
+            throw NonFinite("design: non-finite node coordinate");
+}
+
+// ===========================================================================
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual void info(sp_hip_gp_info* out) const = 0;
+    virtual void optIndex(int32_t* out) const = 0;
+    virtual void evalHost(const float* x, const float* y, float lambda, float gamma,
+                          sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) = 0;
+    virtual void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) = 0;
+    virtual void positions(float* x, float* y) const = 0;
+    virtual void setProfiling(int mask) = 0;
+    virtual void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const = 0;
+};
+
+template <typename T>
+class Engine final : public EngineBase {
+public:
+    Engine(const sp_hip_design& d, const sp_hip_gp_config& cfg) : cfg_(cfg) {
+        requireDevice();
+        validateDesign(d);
+        SPHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        try {
+            build(d);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~Engine() override { release(); }
+
+    void info(sp_hip_gp_info* o) const override {
+        o->dim = dim_;
+        o->n_opt = n_;
+        o->frac_bits = fracBits_;
+        o->precision = sizeof(T) == 4 ? 32 : 64;
+        o->step_x = stepX_;
+        o->step_y = stepY_;
+        o->origin_x = originX_;
+        o->origin_y = originY_;
+        o->movable_area_scaled = movableAreaScaled_;
+    }
+
+    void optIndex(int32_t* out) const override {
+        std::copy(optIdxHost_.begin(), optIdxHost_.end(), out);
+    }
+
+    void positions(float* x, float* y) const override {
+        std::vector<T> hx(total_), hy(total_);
+        SPHIP_CHECK(hipMemcpyAsync(hx.data(), dd_.x, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipMemcpyAsync(hy.data(), dd_.y, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+        for (int i = 0; i < total_; ++i) {
+            x[i] = static_cast<float>(hx[i]);
+            y[i] = static_cast<float>(hy[i]);
+        }
+    }
+
+    void setProfiling(int mask) override {
+        profMask_ = mask;
+        if (mask && !evStart_[0])
+            for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
+                SPHIP_CHECK(hipEventCreate(&evStart_[f]));
+                SPHIP_CHECK(hipEventCreate(&evStop_[f]));
+            }
+    }
+
+    void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const override {
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
+            if (ms) ms[f] = kms_[f];
+            if (launches) launches[f] = klaunch_[f];
+            if (bytes) bytes[f] = kbytes_[f];
+            if (flops) flops[f] = kflops_[f];
+        }
+    }
+
+    // One evaluation at host positions of all nodes (EPlace scatter + objAndGrad).
+    void evalHost(const float* x, const float* y, float lambda, float gamma,
+                  sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) override {
+        std::vector<T> hx(x, x + total_), hy(y, y + total_);
+        SPHIP_CHECK(hipMemcpyAsync(dd_.x, hx.data(), total_ * sizeof(T), hipMemcpyHostToDevice, s_));
+        SPHIP_CHECK(hipMemcpyAsync(dd_.y, hy.data(), total_ * sizeof(T), hipMemcpyHostToDevice, s_));
+        std::vector<T> pos(2 * static_cast<size_t>(n_));
+        for (int k = 0; k < n_; ++k) {
+            pos[k] = hx[optIdxHost_[k]];
+            pos[n_ + k] = hy[optIdxHost_[k]];
+        }
+        SPHIP_CHECK(hipMemcpyAsync(u_, pos.data(), pos.size() * sizeof(T), hipMemcpyHostToDevice, s_));
+        curPos_ = nullptr;   // x/y now hold the raw positions, not a clamped pos buffer
+        const bool wantDbg = dbg != nullptr;
+        long long *dNode = nullptr, *dFill = nullptr;
+        T *dForce = nullptr, *dWl = nullptr;
+        if (wantDbg) {
+            dNode = dalloc<long long>(nb_);
+            dFill = dalloc<long long>(nb_);
+            dForce = dalloc<T>(2 * static_cast<size_t>(n_));
+            dWl = dalloc<T>(2 * static_cast<size_t>(n_));
+        }
+        EvalScalars sc;
+        try {
+            sc = evaluate(u_, nullptr, nullptr, g_, lambda, gamma, dNode, dFill, dForce, dWl);
+            std::memcpy(out, &sc, sizeof(sc));
+            if (wantDbg) {
+                auto fetchQ = [&](int64_t* q, double* dens, const long long* dev, size_t cnt) {
+                    if (!q && !dens) return;
+                    std::vector<long long> tmp(cnt);
+                    SPHIP_CHECK(hipMemcpyAsync(tmp.data(), dev, cnt * sizeof(long long),
+                                               hipMemcpyDeviceToHost, s_));
+                    SPHIP_CHECK(hipStreamSynchronize(s_));
+                    for (size_t i = 0; i < cnt; ++i) {
+                        if (q) q[i] = tmp[i];
+                        if (dens) dens[i] = std::ldexp(static_cast<double>(tmp[i]), -fracBits_);
+                    }
+                };
+                auto fetchT = [&](double* host, const T* dev, size_t cnt, long ld = -1, int rows = 0,
+                                  int cols = 0) {
+                    if (!host) return;
+                    std::vector<T> tmp(ld > 0 ? static_cast<size_t>(rows) * ld : cnt);
+                    SPHIP_CHECK(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(T),
+                                               hipMemcpyDeviceToHost, s_));
+                    SPHIP_CHECK(hipStreamSynchronize(s_));
+                    if (ld > 0) {
+                        for (int r = 0; r < rows; ++r)
+                            for (int c = 0; c < cols; ++c)
+                                host[static_cast<size_t>(r) * cols + c] = tmp[static_cast<size_t>(r) * ld + c];
+                    } else {
+                        for (size_t i = 0; i < cnt; ++i) host[i] = tmp[i];
+                    }
+                };
+                fetchQ(dbg->node_q, dbg->node_density, dNode, nb_);
+                fetchQ(dbg->filler_q, dbg->filler_density, dFill, nb_);
+                fetchT(dbg->rho, poisson_->rho(), 0, poisson_->ld(), dim_, dim_);
+                if (dbg->rho)
+                    for (size_t b = 0; b < nb_; ++b) dbg->rho[b] += rhoShift_;
+                fetchT(dbg->field_x, poisson_->fieldX(), 0, poisson_->ld(), dim_, dim_);
+                fetchT(dbg->field_y, poisson_->fieldY(), 0, poisson_->ld(), dim_, dim_);
+                fetchT(dbg->force, dForce, 2 * static_cast<size_t>(n_));
+                fetchT(dbg->grad_wl, dWl, 2 * static_cast<size_t>(n_));
+                fetchT(dbg->grad, g_, 2 * static_cast<size_t>(n_));
+                SPHIP_CHECK(hipStreamSynchronize(s_));
+            }
+        } catch (...) {
+            for (void* p : {static_cast<void*>(dNode), static_cast<void*>(dFill),
+                            static_cast<void*>(dForce), static_cast<void*>(dWl)})
+                if (p) (void)hipFree(p);
+            throw;
+        }
+        for (void* p : {static_cast<void*>(dNode), static_cast<void*>(dFill),
+                        static_cast<void*>(dForce), static_cast<void*>(dWl)})
+            if (p) SPHIP_CHECK(hipFree(p));
+    }
+
+    void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) override;
+
+private:
+    void build(const sp_hip_design& d);
+    void release();
+    EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
+                         long long* dbgNode = nullptr, long long* dbgFill = nullptr,
+                         T* dbgForce = nullptr, T* dbgWl = nullptr);
+    void scatterPos(const T* pos);   // clamp + write into node coordinates
+    double deliveredMetrics(double* hpwlOut);
+    void beginK(int f) {
+        if (profMask_ & (1 << f)) SPHIP_CHECK(hipEventRecord(evStart_[f], s_));
+    }
+    void endK(int f) {
+        if (profMask_ & (1 << f)) {
+            SPHIP_CHECK(hipEventRecord(evStop_[f], s_));
+            pendingK_[f] = true;
+        }
+    }
+    void harvestK() {
+        if (!profMask_) return;
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f)
+            if (pendingK_[f]) {
+                float ms = 0;
+                SPHIP_CHECK(hipEventSynchronize(evStop_[f]));
+                SPHIP_CHECK(hipEventElapsedTime(&ms, evStart_[f], evStop_[f]));
+                kms_[f] += ms;
+                klaunch_[f] += 1;
+                kbytes_[f] += bytesPer_[f];
+                kflops_[f] += flopsPer_[f];
+                pendingK_[f] = false;
+            }
+    }
+
+    sp_hip_gp_config cfg_;
+    hipStream_t s_ = nullptr;
+    DevDesign<T> dd_{};
+    std::vector<void*> owned_;
+    int total_ = 0, n_ = 0, dim_ = 0;
+    size_t nb_ = 0;
+    double stepX_ = 1, stepY_ = 1, originX_ = 0, originY_ = 0, binArea_ = 1;
+    double movableAreaScaled_ = 0;
+    double rhoShift_ = 0;   // mean density subtracted before the transforms
+    int fracBits_ = 32;
+    GridGeom<T> geom_{};
+    unsigned long long *nodeQ_ = nullptr, *fillQ_ = nullptr;
+    double *terminal_ = nullptr, *base_ = nullptr;
+    std::unique_ptr<PoissonPlan<T>> poisson_;
+    ReduceWs ws_{};
+    double* hScalars_ = nullptr;   // pinned
+    std::vector<int> optIdxHost_;
+    std::vector<T> wHost_, hHost_;
+    T *u_ = nullptr, *v_ = nullptr, *uPrev_ = nullptr, *vPrev_ = nullptr, *uNew_ = nullptr,
+      *vNew_ = nullptr, *g_ = nullptr, *gPrev_ = nullptr, *gA_ = nullptr, *gB_ = nullptr,
+      *probe_ = nullptr;
+    const T* curPos_ = nullptr;
+    std::vector<T*> snapBufs_;
+    int profMask_ = 0;   // bit f: time kernel family f with HIP events
+    hipEvent_t evStart_[SP_HIP_K_COUNT] = {}, evStop_[SP_HIP_K_COUNT] = {};
+    bool pendingK_[SP_HIP_K_COUNT] = {};
+    double kms_[SP_HIP_K_COUNT] = {}, kbytes_[SP_HIP_K_COUNT] = {}, kflops_[SP_HIP_K_COUNT] = {};
+    int64_t klaunch_[SP_HIP_K_COUNT] = {};
+    double bytesPer_[SP_HIP_K_COUNT] = {}, flopsPer_[SP_HIP_K_COUNT] = {};
+};
+
+template <typename T>
+void Engine<T>::release() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    poisson_.reset();
+    for (void* p : owned_) (void)hipFree(p);
+    owned_.clear();
+    for (T* p : snapBufs_) (void)hipFree(p);
+    snapBufs_.clear();
+    if (hScalars_) (void)hipHostFree(hScalars_);
+    hScalars_ = nullptr;
+    for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
+        if (evStart_[f]) (void)hipEventDestroy(evStart_[f]);
+        if (evStop_[f]) (void)hipEventDestroy(evStop_[f]);
+        evStart_[f] = evStop_[f] = nullptr;
+    }
+    if (s_) (void)hipStreamDestroy(s_);
+    s_ = nullptr;
+}
+
+template <typename T>
+void Engine<T>::build(const sp_hip_design& d) {
+    const int total = d.num_nodes + d.num_fillers;
+    total_ = total;
+    const int stage = cfg_.stage;
+    auto isMacro = [&](int i) { return (d.node_flags[i] & F_MACRO) != 0; };
+    auto isFixed = [&](int i) { return (d.node_flags[i] & F_FIXED) != 0; };
+    auto isNI = [&](int i) { return (d.node_flags[i] & F_NI) != 0; };
+    auto own = [&](void* p) { owned_.push_back(p); return p; };
+
+    // buildOptSet, eplace.cpp:82-94.
+    if (stage != 1)
+        for (int i = 0; i < d.num_movable; ++i) {
+            if (isFixed(i) || isNI(i)) continue;
+            if (stage == 2 && isMacro(i)) continue;
+            optIdxHost_.push_back(i);
+        }
+    for (int i = d.num_nodes; i < total; ++i) optIdxHost_.push_back(i);
+    n_ = static_cast<int>(optIdxHost_.size());
+
+    // BinGrid::initialize, bin_grid.cpp:172-281.
+    constexpr int kMaxBinDim = 1024;
+    const float target = cfg_.target_density;
+    double cellArea = 0, macroArea = 0;
+    for (int i = 0; i < d.num_movable; ++i) {
+        if (isFixed(i) || isNI(i)) continue;
+        const double a = static_cast<double>(d.node_w[i]) * d.node_h[i];
+        if (isMacro(i)) macroArea += a;
+        else cellArea += a;
+    }
+    movableAreaScaled_ = cellArea + macroArea * static_cast<double>(target);
+    if (cfg_.gp_bin_dim > 0) {
+        dim_ = std::min(cfg_.gp_bin_dim, kMaxBinDim);
+    } else {
+        const double avgArea = d.num_movable > 0 ? (cellArea + macroArea) / d.num_movable : 1.0;
+        const double idealBin = std::max(avgArea / target, 1e-9);
+        const double coreArea = static_cast<double>(d.core_hx - d.core_lx) *
+                                static_cast<double>(d.core_hy - d.core_ly);
+        const double ideal = coreArea / idealBin;
+        dim_ = 4;
+        while (dim_ * 2 <= kMaxBinDim && static_cast<double>(dim_ * 2) * (dim_ * 2) <= ideal)
+            dim_ *= 2;
+    }
+    if (dim_ < 2) throw InvalidArg("bin grid must be at least 2x2");
+    const float coreW = d.core_hx - d.core_lx, coreH = d.core_hy - d.core_ly;
+    if (sizeof(T) == 4) {
+        stepX_ = coreW / static_cast<float>(dim_);
+        stepY_ = coreH / static_cast<float>(dim_);
+    } else {
+        stepX_ = static_cast<double>(coreW) / dim_;
+        stepY_ = static_cast<double>(coreH) / dim_;
+    }
+    originX_ = d.core_lx;
+    originY_ = d.core_ly;
+    binArea_ = stepX_ * stepY_;
+    nb_ = static_cast<size_t>(dim_) * dim_;
+
+    // static terminal density (bin_grid.cpp:228-245) and base density (:252-281),
+    // in the working precision
+    const T sx = T(stepX_), sy = T(stepY_), ox = T(originX_), oy = T(originY_);
+    std::vector<T> term(nb_, T(0)), base(nb_, T(0));
+    for (int i = 0; i < d.num_nodes; ++i) {
+        if (!isFixed(i) || isNI(i)) continue;
+        const T lx = T(d.node_x[i]) - T(0.5) * T(d.node_w[i]);
+        const T hx = T(d.node_x[i]) + T(0.5) * T(d.node_w[i]);
+        const T ly = T(d.node_y[i]) - T(0.5) * T(d.node_h[i]);
+        const T hy = T(d.node_y[i]) + T(0.5) * T(d.node_h[i]);
+        auto cl = [&](T v) { return std::clamp(static_cast<int>(v), 0, dim_ - 1); };
+        const int bx0 = cl((lx - ox) / sx), bx1 = cl((hx - ox) / sx);
+        const int by0 = cl((ly - oy) / sy), by1 = cl((hy - oy) / sy);
+        for (int bx = bx0; bx <= bx1; ++bx) {
+            const T blx = ox + T(bx) * sx;
+            for (int by = by0; by <= by1; ++by) {
+                const T bly = oy + T(by) * sy;
+                term[static_cast<size_t>(bx) * dim_ + by] +=
+                    T(target) * overlapT<T>(blx, bly, blx + sx, bly + sy, lx, ly, hx, hy);
+            }
+        }
+    }
+    std::vector<int> order(d.num_rows);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return d.row_ly[a] < d.row_ly[b]; });
+    for (int by = 0; by < dim_; ++by) {
+        const T bly = oy + T(by) * sy, bhy = bly + sy;
+        size_t lo = 0, hi = order.size();
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            const int r = order[mid];
+            if (T(d.row_ly[r]) + T(d.row_height[r]) <= bly) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int bx = 0; bx < dim_; ++bx) {
+            const T blx = ox + T(bx) * sx;
+            T covered = 0;
+            for (size_t t = lo; t < order.size(); ++t) {
+                const int r = order[t];
+                if (T(d.row_ly[r]) >= bhy) break;
+                covered += overlapT<T>(blx, bly, blx + sx, bhy, T(d.row_lx[r]), T(d.row_ly[r]),
+                                       T(d.row_hx[r]), T(d.row_ly[r]) + T(d.row_height[r]));
+            }
+            const T freeA = std::max(T(0), (blx + sx - blx) * (bhy - bly) - covered);
+            base[static_cast<size_t>(bx) * dim_ + by] = T(target) * freeA;
+        }
+    }
+    terminal_ = static_cast<double*>(own(dupload<double>(term, s_)));
+    base_ = static_cast<double*>(own(dupload<double>(base, s_)));
+
+    // fixed-point scale: the largest possible bin total (all density charge in
+    // one bin) stays below 2^61
+    std::vector<int> dens;
+    double chargeBound = 1.0;
+    for (int i = 0; i < total; ++i) {
+        if (isFixed(i) || isNI(i)) continue;
+        dens.push_back(i);
+        chargeBound += static_cast<double>(d.node_w[i]) * d.node_h[i] *
+                       std::max(1.0, static_cast<double>(target));
+    }
+    fracBits_ = std::min(40, 61 - static_cast<int>(std::ceil(std::log2(chargeBound))));
+    {
+        double charge = 0;
+        for (int i : dens) {
+            const double a = static_cast<double>(d.node_w[i]) * d.node_h[i];
+            charge += ((d.node_flags[i] & F_MACRO) && !(d.node_flags[i] & F_FILLER))
+                          ? a * static_cast<double>(target) : a;
+        }
+        for (size_t b = 0; b < nb_; ++b) charge += static_cast<double>(term[b]) + base[b];
+        rhoShift_ = charge / (binArea_ * static_cast<double>(nb_));
+    }
+    if (fracBits_ < 8) throw InvalidArg("design too large for the fixed-point density grid");
+    geom_.dim = dim_;
+    geom_.stepX = T(stepX_);
+    geom_.stepY = T(stepY_);
+    geom_.originX = T(originX_);
+    geom_.originY = T(originY_);
+    geom_.target = T(target);
+
+    poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_);
+    geom_.ld = poisson_->ld();
+
+    // device design
+    dd_.total = total;
+    dd_.numMov = d.num_movable;
+    dd_.numNodes = d.num_nodes;
+    dd_.nOpt = n_;
+    dd_.nDens = static_cast<int>(dens.size());
+    dd_.numNets = d.num_nets;
+    dd_.numPins = d.num_pins;
+    dd_.coreLx = T(d.core_lx);
+    dd_.coreLy = T(d.core_ly);
+    dd_.coreHx = T(d.core_hx);
+    dd_.coreHy = T(d.core_hy);
+    std::vector<float> xs(d.node_x, d.node_x + total), ys(d.node_y, d.node_y + total);
+    dd_.x = static_cast<T*>(own(dupload<T>(xs, s_)));
+    dd_.y = static_cast<T*>(own(dupload<T>(ys, s_)));
+    std::vector<float> ws(d.node_w, d.node_w + total), hs(d.node_h, d.node_h + total);
+    wHost_.assign(ws.begin(), ws.end());
+    hHost_.assign(hs.begin(), hs.end());
+    dd_.w = static_cast<T*>(own(dupload<T>(ws, s_)));
+    dd_.h = static_cast<T*>(own(dupload<T>(hs, s_)));
+    std::vector<uint8_t> fl(d.node_flags, d.node_flags + total);
+    dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_)));
+    dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_)));
+    dd_.densIdx = static_cast<int*>(own(dupload<int>(dens, s_)));
+    std::vector<double> parea(n_);
+    std::vector<float> pdeg(n_);
+    for (int k = 0; k < n_; ++k) {
+        const int i = optIdxHost_[k];
+        parea[k] = static_cast<double>(d.node_w[i]) * static_cast<double>(d.node_h[i]);
+        pdeg[k] = static_cast<float>(d.node2pin_start[i + 1] - d.node2pin_start[i]);
+    }
+    dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_)));
+    dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_)));
+    // net-ordered pin slots
+    std::vector<int> netStart(d.net2pin_start, d.net2pin_start + d.num_nets + 1);
+    std::vector<int> slotNode(d.num_pins);
+    std::vector<float> slotOffX(d.num_pins), slotOffY(d.num_pins);
+    for (int t = 0; t < d.num_pins; ++t) {
+        const int p = d.flat_net2pin[t];
+        slotNode[t] = d.pin2node[p];
+        slotOffX[t] = d.pin_offset_x[p];
+        slotOffY[t] = d.pin_offset_y[p];
+    }
+    std::vector<int> small, large;
+    for (int k = 0; k < d.num_nets; ++k)
+        (netStart[k + 1] - netStart[k] > kLargeNet ? large : small).push_back(k);
+    dd_.nSmallNets = static_cast<int>(small.size());
+    dd_.nLargeNets = static_cast<int>(large.size());
+    dd_.netStart = static_cast<int*>(own(dupload<int>(netStart, s_)));
+    dd_.slotNode = static_cast<int*>(own(dupload<int>(slotNode, s_)));
+    dd_.slotOffX = static_cast<T*>(own(dupload<T>(slotOffX, s_)));
+    dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_)));
+    dd_.smallNets = static_cast<int*>(own(dupload<int>(small, s_)));
+    dd_.largeNets = static_cast<int*>(own(dupload<int>(large, s_)));
+    // node-ordered pin entries
+    std::vector<int> nodeStart(d.node2pin_start, d.node2pin_start + total + 1);
+    std::vector<int> npNet(d.num_pins);
+    std::vector<float> npOffX(d.num_pins), npOffY(d.num_pins);
+    for (int t = 0; t < d.num_pins; ++t) {
+        const int p = d.flat_node2pin[t];
+        npNet[t] = d.pin2net[p];
+        npOffX[t] = d.pin_offset_x[p];
+        npOffY[t] = d.pin_offset_y[p];
+    }
+    dd_.nodeStart = static_cast<int*>(own(dupload<int>(nodeStart, s_)));
+    dd_.npNet = static_cast<int*>(own(dupload<int>(npNet, s_)));
+    dd_.npOffX = static_cast<T*>(own(dupload<T>(npOffX, s_)));
+    dd_.npOffY = static_cast<T*>(own(dupload<T>(npOffY, s_)));
+    dd_.stats = static_cast<NetStat<T>*>(own(dalloc<NetStat<T>>(std::max(1, d.num_nets))));
+
+    nodeQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
+    fillQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
+    SPHIP_CHECK(hipMemsetAsync(nodeQ_, 0, nb_ * 8, s_));
+    SPHIP_CHECK(hipMemsetAsync(fillQ_, 0, nb_ * 8, s_));
+
+    // reduction workspace: 4 regions of maxBlocks x 8 doubles
+    const long maxItems = std::max<long>({static_cast<long>(nb_), n_, d.num_nets,
+                                          dd_.nDens, 1});
+    ws_.maxBlocks = static_cast<int>((maxItems + kBlock - 1) / kBlock) + 1;
+    ws_.partials = static_cast<double*>(own(dalloc<double>(4L * ws_.maxBlocks * 8)));
+    ws_.counters = static_cast<unsigned*>(own(dalloc<unsigned>(8)));
+    ws_.scalars = static_cast<double*>(own(dalloc<double>(kNumScalars)));
+    SPHIP_CHECK(hipMemsetAsync(ws_.counters, 0, 8 * sizeof(unsigned), s_));
+    SPHIP_CHECK(hipMemsetAsync(ws_.scalars, 0, kNumScalars * sizeof(double), s_));
+    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hScalars_), kNumScalars * sizeof(double),
+                              hipHostMallocDefault));
+
+    // optimizer state
+    const size_t n2 = 2 * static_cast<size_t>(std::max(n_, 1));
+    for (T** b : {&u_, &v_, &uPrev_, &vPrev_, &uNew_, &vNew_, &g_, &gPrev_, &gA_, &gB_, &probe_}) {
+        *b = static_cast<T*>(own(dalloc<T>(n2)));
+        SPHIP_CHECK(hipMemsetAsync(*b, 0, n2 * sizeof(T), s_));
+    }
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+
+    // algorithmic bytes / flops per launch (the roofline accounting)
+    const double P = d.num_pins, E = d.num_nets, ts = sizeof(T);
+    bytesPer_[SP_HIP_K_SCATTER] = dd_.nDens * (4 + 2 * ts + 2 * ts + 1) + 8.0 * 2 * nb_;
+    bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + ts + 8 * 2);
+    bytesPer_[SP_HIP_K_POISSON] = 0;
+    flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
+    bytesPer_[SP_HIP_K_WA_NETS] = P * (4 + 2 * ts) + P * 2 * ts + E * (4 + 12 * ts);
+    bytesPer_[SP_HIP_K_NODE_GRAD] =
+        n_ * (4 + 2 * ts + 2 * ts + 1 + 8 + 4 + 4 + 6 * 2 * ts) + P * (4 + 2 * ts + 12 * ts) +
+        n_ * 4.0 * 2 * ts;
+    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 2 * 2 * ts * 3 + 2 * 2 * ts + 2 * ts);
+}
+
+template <typename T>
+EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda,
+                                float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
+                                T* dbgWl) {
+    beginK(SP_HIP_K_SCATTER);
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, s_);
+    endK(SP_HIP_K_SCATTER);
+    beginK(SP_HIP_K_FINALIZE);
+    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, s_);
+    endK(SP_HIP_K_FINALIZE);
+    beginK(SP_HIP_K_POISSON);
+    poisson_->solve(false);
+    endK(SP_HIP_K_POISSON);
+    beginK(SP_HIP_K_WA_NETS);
+    launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, s_);
+    endK(SP_HIP_K_WA_NETS);
+    beginK(SP_HIP_K_NODE_GRAD);
+    launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), gamma, lambda, u, up,
+                      gp, grad, dbgForce, dbgWl, ws_, s_);
+    endK(SP_HIP_K_NODE_GRAD);
+    SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
+                               hipMemcpyDeviceToHost, s_));
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+    harvestK();
+    EvalScalars sc;
+    std::memcpy(&sc, hScalars_, sizeof(sc));
+    return sc;
+}
+
+template <typename T>
+void Engine<T>::scatterPos(const T* pos) {
+    beginK(SP_HIP_K_UPDATE);
+    launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, s_);
+    endK(SP_HIP_K_UPDATE);
+    curPos_ = pos;
+}
+
+// Delivery (eplace.cpp:478-494): positions already scattered; tau and HPWL.
+template <typename T>
+double Engine<T>::deliveredMetrics(double* hpwlOut) {
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, s_);
+    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, s_);
+    launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, s_);
+    SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
+                               hipMemcpyDeviceToHost, s_));
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+    *hpwlOut = hScalars_[2];
+    return hScalars_[0];
+}
+
+template <typename T>
+void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) {
+    *res = sp_hip_gp_result{};
+    res->rollback_iter = -1;
+    const int stage = cfg_.stage;
+    const int n = n_;
+    if (n == 0) return;
+    const size_t bytes2n = 2 * static_cast<size_t>(n) * sizeof(T);
+    const float stepX = static_cast<float>(stepX_);
+
+    // gather -> clamp -> scatter of the initial positions (eplace.cpp:238-245)
+    {
+        std::vector<T> hx(total_), hy(total_);
+        SPHIP_CHECK(hipMemcpyAsync(hx.data(), dd_.x, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipMemcpyAsync(hy.data(), dd_.y, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+        std::vector<T> pos(2 * static_cast<size_t>(n));
+        for (int k = 0; k < n; ++k) {
+            pos[k] = hx[optIdxHost_[k]];
+            pos[n + k] = hy[optIdxHost_[k]];
+        }
+        SPHIP_CHECK(hipMemcpyAsync(probe_, pos.data(), bytes2n, hipMemcpyHostToDevice, s_));
+        // clamp into u (v' = clamp(A) with vOut = u)
+        launchUpdate<T>(dd_, probe_, nullptr, nullptr, 0.0, 0.0, u_, nullptr, s_);
+        curPos_ = u_;
+    }
+
+    auto checkFinite = [&](const EvalScalars& sc) {
+        if (!std::isfinite(sc.tau) || !std::isfinite(sc.hpwl) || !std::isfinite(sc.wl) ||
+            !std::isfinite(sc.sum_wl) || !std::isfinite(sc.sum_den) || sc.nonfinite > 0)
+            throw NonFinite("GP/evaluate: non-finite objective, overflow, force norm or gradient");
+    };
+
+    // lambda initialisation probe, eplace.cpp:247-259
+    float gamma = computeGamma(1.f, stepX);
+    float lambda = 1.f;
+    EvalScalars sc = evaluate(u_, nullptr, nullptr, gA_, lambda, gamma);
+    checkFinite(sc);
+    long evals = 0;
+    gamma = computeGamma(static_cast<float>(sc.tau), stepX);
+    lambda = sc.sum_den > 0.0 ? static_cast<float>(sc.sum_wl / sc.sum_den) : 1.f;
+    if (!std::isfinite(lambda) || lambda <= 0.f) lambda = 1.f;
+    float tau = static_cast<float>(sc.tau);
+    double hpwl = sc.hpwl;
+    res->initial_hpwl = hpwl;
+    res->initial_overflow = tau;
+    res->initial_lambda = lambda;
+    const float targetTau = stage == 2 ? cfg_.cgp_target_overflow : cfg_.target_overflow;
+    const int maxIter = stage == 1 ? cfg_.filler_only_iters : cfg_.gp_max_iter;
+    double effRef = (cfg_.lambda_relative && stage != 1) ? hpwl * cfg_.delta_hpwl_relative
+                                                         : static_cast<double>(cfg_.delta_hpwl_ref);
+    if (hpwl == 0.0 && cfg_.lambda_relative && stage != 1) effRef = cfg_.delta_hpwl_ref;
+    if (!(effRef > 0.0) || !std::isfinite(effRef))
+        throw InvalidArg("invalid effective HPWL reference");
+
+    auto deliver = [&](const T* pos) {
+        scatterPos(pos);
+        double hp = 0;
+        res->overflow = static_cast<float>(deliveredMetrics(&hp));
+        res->hpwl = hp;
+    };
+    if (stage != 1 && tau < targetTau) {   // eplace.cpp:278-296: no step at all
+        res->converged = 1;
+        deliver(u_);
+        return;
+    }
+
+    // Nesterov init (nesterov.cpp:171-183) and first step size (:190-218)
+    const int maxBt = cfg_.max_backtrack > 0 ? cfg_.max_backtrack : 10;
+    SPHIP_CHECK(hipMemcpyAsync(v_, u_, bytes2n, hipMemcpyDeviceToDevice, s_));
+    SPHIP_CHECK(hipMemcpyAsync(uPrev_, u_, bytes2n, hipMemcpyDeviceToDevice, s_));
+    SPHIP_CHECK(hipMemcpyAsync(vPrev_, u_, bytes2n, hipMemcpyDeviceToDevice, s_));
+    SPHIP_CHECK(hipMemsetAsync(g_, 0, bytes2n, s_));
+    SPHIP_CHECK(hipMemsetAsync(gPrev_, 0, bytes2n, s_));
+    double a = 1.0;   // Nesterov a_k, kept in T
+    double alpha0 = 1.0;
+    {
+        if (curPos_ != u_) scatterPos(u_);
+        EvalScalars sA = evaluate(u_, nullptr, nullptr, gA_, lambda, gamma);
+        checkFinite(sA);
+        const double posNorm = std::sqrt(sA.dx2), gNorm = std::sqrt(sA.gg);
+        if (gNorm > 0.0 && std::isfinite(gNorm)) {
+            const double scale = 0.1 * (posNorm > 0.0 ? posNorm : 1.0) / gNorm;
+            launchUpdate<T>(dd_, u_, gA_, nullptr, static_cast<double>(static_cast<T>(scale)), 0.0,
+                            probe_, nullptr, s_);
+            curPos_ = probe_;
+            EvalScalars sB = evaluate(probe_, u_, gA_, gB_, lambda, gamma);
+            checkFinite(sB);
+            const double dx = std::sqrt(sB.dx2), dg = std::sqrt(sB.dg2);
+            if (dx > 0.0 && dg > 0.0 && std::isfinite(dg)) {
+                const double step = 1.0 / (dg / dx);
+                alpha0 = (std::isfinite(step) && step > 0.0) ? static_cast<double>(static_cast<T>(step))
+                                                             : 1.0;
+            }
+        }
+    }
+
+    struct Snap {
+        float tau;
+        double hpwl;
+        int iter;
+        T* buf;
+    };
+    std::vector<Snap> snaps;
+    std::vector<T*> spare(snapBufs_.begin(), snapBufs_.end());
+    float bestTau = std::numeric_limits<float>::infinity();
+    float refTau = std::numeric_limits<float>::infinity();
+    int sinceImprove = 0;
+    const float minGain = std::clamp(cfg_.gp_stagnation_min_gain, 0.f, 0.5f);
+    double prevHpwl = 0;
+    bool havePrev = false;
+    double alpha = 1.0;
+    double momPrev = 0.0;
+    const T* finalPos = nullptr;
+
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int iter = 0; iter < maxIter; ++iter) {
+        const float evalLambda = lambda, evalGamma = gamma;
+        // ---- NesterovOptimizer::step (nesterov.cpp:51-122)
+        if (curPos_ != u_) scatterPos(u_);
+        sc = evaluate(u_, uPrev_, gPrev_, g_, lambda, gamma);
+        checkFinite(sc);
+        ++evals;
+        if (cfg_.gp_backtrack && iter > 0) {
+            for (int tries = 1; tries < maxBt; ++tries) {   // redoIfOvershot, :146-169
+                const double dx = std::sqrt(sc.dx2), dg = std::sqrt(sc.dg2);
+                if (!(dx > 0.0) || !(dg > 0.0) || !std::isfinite(dg)) break;
+                const double lip = dx / dg;
+                if (lip > 0.95 * alpha) break;
+                alpha = static_cast<double>(static_cast<T>(lip));
+                beginK(SP_HIP_K_UPDATE);
+                launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, s_);
+                endK(SP_HIP_K_UPDATE);
+                curPos_ = u_;
+                sc = evaluate(u_, uPrev_, gPrev_, g_, lambda, gamma);
+                checkFinite(sc);
+                ++evals;
+            }
+        }
+        // step size (nesterov.cpp:88, :221-240)
+        if (iter == 0) {
+            alpha = alpha0;
+        } else {
+            const double dx = std::sqrt(sc.dx2), dg = std::sqrt(sc.dg2);
+            if (dx > 0.0 && dg > 0.0) {
+                const double lipStep = dx / dg;
+                double next = lipStep;
+                if (cfg_.use_bb) {
+                    const double yy = dg * dg;
+                    next = std::min(dx / dg, lipStep);
+                    if (yy > 0.0) {
+                        const double bb = sc.sy / yy;
+                        if (bb > 0.0 && std::isfinite(bb)) next = bb;
+                    }
+                }
+                alpha = static_cast<double>(static_cast<T>(next));
+            }
+        }
+        const T aT = static_cast<T>(a);
+        const T aNext = T(0.5) * (T(1) + std::sqrt(T(4) * aT * aT + T(1)));
+        const T mom = (aT - T(1)) / aNext;
+        beginK(SP_HIP_K_UPDATE);
+        launchUpdate<T>(dd_, u_, g_, v_, alpha, static_cast<double>(mom), vNew_, uNew_, s_);
+        endK(SP_HIP_K_UPDATE);
+        // rotate (nesterov.cpp:109-118)
+        std::swap(uPrev_, u_);
+        std::swap(gPrev_, g_);
+        std::swap(v_, vNew_);
+        std::swap(vPrev_, vNew_);
+        std::swap(u_, uNew_);
+        curPos_ = u_;
+        momPrev = static_cast<double>(mom);
+        a = static_cast<double>(aNext);
+        const T* lastEval = uPrev_;   // the point the last evaluation used
+
+        tau = static_cast<float>(sc.tau);
+        hpwl = sc.hpwl;
+        // schedules (eplace.cpp:371-380)
+        const double dH = havePrev ? hpwl - prevHpwl : 0.0;
+        const float mu = havePrev ? lambdaMultiplier(dH, effRef) : 1.f;
+        if (havePrev) lambda *= mu;
+        prevHpwl = hpwl;
+        havePrev = true;
+        gamma = computeGamma(tau, stepX);
+        if (!std::isfinite(lambda) || !std::isfinite(gamma))
+            throw NonFinite("non-finite schedule parameters");
+        if (rows && iter < maxRows) {
+            sp_hip_iter_metrics& m = rows[iter];
+            m.iter = iter;
+            m.evaluations = static_cast<int32_t>(evals);
+            m.hpwl = hpwl;
+            m.overflow = tau;
+            m.lambda = lambda;
+            m.gamma = gamma;
+            m.step_size = alpha;
+            m.grad_norm_wl = static_cast<float>(sc.sum_wl);
+            m.grad_norm_den = static_cast<float>(sc.sum_den);
+            m.eval_lambda = evalLambda;
+            m.eval_gamma = evalGamma;
+            m.delta_hpwl = dH;
+            m.lambda_mu = mu;
+            m.elapsed_ms = std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - t0).count();
+        }
+        res->iterations = iter + 1;
+        if (stage != 1 && tau < targetTau) {   // deliver the look-ahead point, :412-419
+            res->converged = 1;
+            finalPos = lastEval;
+            break;
+        }
+        if (stage != 1 && cfg_.gp_stagnation_window > 0) {   // :421-473
+            if (tau < refTau * (1.f - minGain)) {
+                refTau = tau;
+                sinceImprove = 0;
+            } else {
+                ++sinceImprove;
+            }
+            bestTau = std::min(bestTau, tau);
+            const float tol = bestTau * (1.f + minGain);
+            while (!snaps.empty() && snaps.front().tau > tol) {
+                spare.push_back(snaps.front().buf);
+                snaps.erase(snaps.begin());
+            }
+            if (snaps.empty() || tau * (1.f + 0.5f * minGain) <= snaps.back().tau) {
+                T* buf;
+                if (!spare.empty()) {
+                    buf = spare.back();
+                    spare.pop_back();
+                } else {
+                    buf = dalloc<T>(2 * static_cast<size_t>(n));
+                    snapBufs_.push_back(buf);
+                }
+                SPHIP_CHECK(hipMemcpyAsync(buf, lastEval, bytes2n, hipMemcpyDeviceToDevice, s_));
+                snaps.push_back(Snap{tau, hpwl, iter, buf});
+            }
+            if (sinceImprove >= cfg_.gp_stagnation_window) {
+                const Snap* pick = &snaps.front();
+                for (const Snap& sn : snaps)
+                    if (sn.hpwl < pick->hpwl) pick = &sn;
+                finalPos = pick->buf;
+                res->stagnated = 1;
+                res->rollback_iter = pick->iter;
+                break;
+            }
+        }
+    }
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+    res->seconds_loop =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->evaluations = evals;
+    deliver(finalPos ? finalPos : v_);
+    if (!std::isfinite(res->hpwl) || !std::isfinite(res->overflow))
+        throw NonFinite("non-finite delivered metrics");
+}
+
+// ===========================================================================
+class PoissonHandleBase {
+public:
+    virtual ~PoissonHandleBase() = default;
+    virtual void solveHost(const float* rho, float* phi, float* ex, float* ey) = 0;
+    virtual void solveDev(const void* rho, void* out3) = 0;
+    virtual const char* name() const = 0;
+};
+
+template <typename T>
+class PoissonHandle final : public PoissonHandleBase {
+public:
+    PoissonHandle(int nx, int ny, float sx, float sy) : nx_(nx), ny_(ny) {
+        requireDevice();
+        if (nx < 2 || ny < 2) throw InvalidArg("Poisson: grid must be at least 2x2");
+        if (!(sx > 0.f) || !(sy > 0.f)) throw InvalidArg("Poisson: bin step must be positive");
+        SPHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        try {
+            plan_ = std::make_unique<PoissonPlan<T>>(nx, ny, sx, sy, s_);
+            const size_t n = static_cast<size_t>(nx) * ny;
+            SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hIn_), n * sizeof(T), hipHostMallocDefault));
+            SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hOut_), 3 * n * sizeof(T), hipHostMallocDefault));
+            dIn_ = dalloc<T>(n);
+            dOut_ = dalloc<T>(3 * n);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+    }
+    ~PoissonHandle() override { cleanup(); }
+
+    void solveHost(const float* rho, float* phi, float* ex, float* ey) override {
+        if (!rho || !phi || !ex || !ey) throw InvalidArg("Poisson: null span");
+        const size_t n = static_cast<size_t>(nx_) * ny_;
+        // subtracting the mean changes only the discarded DC term and keeps
+        // the transform inputs small
+        double mean = 0;
+        for (size_t i = 0; i < n; ++i) mean += rho[i];
+        mean /= static_cast<double>(n);
+        for (size_t i = 0; i < n; ++i) hIn_[i] = static_cast<T>(rho[i] - mean);
+        SPHIP_CHECK(hipMemcpyAsync(dIn_, hIn_, n * sizeof(T), hipMemcpyHostToDevice, s_));
+        run(dIn_, dOut_);
+        SPHIP_CHECK(hipMemcpyAsync(hOut_, dOut_, 3 * n * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+        for (size_t i = 0; i < n; ++i) {
+            phi[i] = static_cast<float>(hOut_[i]);
+            ex[i] = static_cast<float>(hOut_[n + i]);
+            ey[i] = static_cast<float>(hOut_[2 * n + i]);
+        }
+    }
+
+    void solveDev(const void* rho, void* out3) override {
+        if (!rho || !out3) throw InvalidArg("Poisson: null device pointer");
+        run(static_cast<const T*>(rho), static_cast<T*>(out3));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+    }
+
+    const char* name() const override { return sizeof(T) == 4 ? "hip-mfma" : "hip-mfma-fp64"; }
+
+private:
+    void run(const T* rho, T* out3) {
+        const size_t n = static_cast<size_t>(nx_) * ny_;
+        const size_t ld = plan_->ld();
+        SPHIP_CHECK(hipMemcpy2DAsync(plan_->rho(), ld * sizeof(T), rho, ny_ * sizeof(T),
+                                     ny_ * sizeof(T), nx_, hipMemcpyDeviceToDevice, s_));
+        plan_->solve(true);
+        T* outs[3] = {out3, out3 + n, out3 + 2 * n};
+        T* srcs[3] = {plan_->phi(), plan_->fieldX(), plan_->fieldY()};
+        for (int i = 0; i < 3; ++i)
+            SPHIP_CHECK(hipMemcpy2DAsync(outs[i], ny_ * sizeof(T), srcs[i], ld * sizeof(T),
+                                         ny_ * sizeof(T), nx_, hipMemcpyDeviceToDevice, s_));
+    }
+    void cleanup() {
+        if (s_) (void)hipStreamSynchronize(s_);
+        plan_.reset();
+        if (hIn_) (void)hipHostFree(hIn_);
+        if (hOut_) (void)hipHostFree(hOut_);
+        if (dIn_) (void)hipFree(dIn_);
+        if (dOut_) (void)hipFree(dOut_);
+        if (s_) (void)hipStreamDestroy(s_);
+        hIn_ = hOut_ = dIn_ = dOut_ = nullptr;
+        s_ = nullptr;
+    }
+    int nx_, ny_;
+    hipStream_t s_ = nullptr;
+    std::unique_ptr<PoissonPlan<T>> plan_;
+    T *hIn_ = nullptr, *hOut_ = nullptr, *dIn_ = nullptr, *dOut_ = nullptr;
+};
+
+}  // namespace
+}  // namespace sphip
+
+struct sp_hip_poisson {
+    std::unique_ptr<sphip::PoissonHandleBase> impl;
+};
+struct sp_hip_gp {
+    std::unique_ptr<sphip::EngineBase> impl;
+};
+
+using sphip::guarded;
+
+extern "C" {
+
+const char* sp_hip_last_error(void) { return sphip::g_lastError.c_str(); }
+
+int sp_hip_device_count(int* count) {
+    return guarded([&] {
+        if (!count) throw sphip::InvalidArg("null count");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        *count = n;
+    });
+}
+
+int sp_hip_poisson_create(int nx, int ny, float step_x, float step_y, int flags,
+                          sp_hip_poisson** out) {
+    return guarded([&] {
+        if (!out) throw sphip::InvalidArg("null output handle");
+        *out = nullptr;
+        auto h = std::make_unique<sp_hip_poisson>();
+        if (flags & SP_HIP_FLAG_FP64)
+            h->impl = std::make_unique<sphip::PoissonHandle<double>>(nx, ny, step_x, step_y);
+        else
+            h->impl = std::make_unique<sphip::PoissonHandle<float>>(nx, ny, step_x, step_y);
+        *out = h.release();
+    });
+}
+
+int sp_hip_poisson_solve(sp_hip_poisson* p, const float* density, float* phi, float* field_x,
+                         float* field_y) {
+    return guarded([&] {
+        if (!p) throw sphip::InvalidArg("null handle");
+        p->impl->solveHost(density, phi, field_x, field_y);
+    });
+}
+
+int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* out3_dev) {
+    return guarded([&] {
+        if (!p) throw sphip::InvalidArg("null handle");
+        p->impl->solveDev(density_dev, out3_dev);
+    });
+}
+
+const char* sp_hip_poisson_name(const sp_hip_poisson* p) { return p ? p->impl->name() : "none"; }
+
+void sp_hip_poisson_destroy(sp_hip_poisson* p) { delete p; }
+
+int sp_hip_gp_create(const sp_hip_design* d, const sp_hip_gp_config* cfg, sp_hip_gp** out) {
+    return guarded([&] {
+        if (!d || !cfg || !out) throw sphip::InvalidArg("null argument");
+        *out = nullptr;
+        if (cfg->precision != 32 && cfg->precision != 64)
+            throw sphip::InvalidArg("precision must be 32 or 64");
+        if (cfg->stage < 0 || cfg->stage > 2) throw sphip::InvalidArg("stage must be 0, 1 or 2");
+        if (!(cfg->target_density > 0.f)) throw sphip::InvalidArg("target_density must be positive");
+        auto h = std::make_unique<sp_hip_gp>();
+        if (cfg->precision == 64) h->impl = std::make_unique<sphip::Engine<double>>(*d, *cfg);
+        else h->impl = std::make_unique<sphip::Engine<float>>(*d, *cfg);
+        *out = h.release();
+    });
+}
+
+int sp_hip_gp_info_get(const sp_hip_gp* gp, sp_hip_gp_info* info) {
+    return guarded([&] {
+        if (!gp || !info) throw sphip::InvalidArg("null argument");
+        gp->impl->info(info);
+    });
+}
+
+int sp_hip_gp_opt_index(const sp_hip_gp* gp, int32_t* idx) {
+    return guarded([&] {
+        if (!gp || !idx) throw sphip::InvalidArg("null argument");
+        gp->impl->optIndex(idx);
+    });
+}
+
+int sp_hip_gp_eval(sp_hip_gp* gp, const float* x, const float* y, float lambda, float gamma,
+                   sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) {
+    return guarded([&] {
+        if (!gp || !x || !y || !out) throw sphip::InvalidArg("null argument");
+        gp->impl->evalHost(x, y, lambda, gamma, out, dbg);
+    });
+}
+
+int sp_hip_gp_run(sp_hip_gp* gp, sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int max_rows) {
+    return guarded([&] {
+        if (!gp || !res) throw sphip::InvalidArg("null argument");
+        gp->impl->run(res, rows, rows ? max_rows : 0);
+    });
+}
+
+int sp_hip_gp_get_positions(const sp_hip_gp* gp, float* x, float* y) {
+    return guarded([&] {
+        if (!gp || !x || !y) throw sphip::InvalidArg("null argument");
+        gp->impl->positions(x, y);
+    });
+}
+
+int sp_hip_gp_set_profiling(sp_hip_gp* gp, int on) {
+    return guarded([&] {
+        if (!gp) throw sphip::InvalidArg("null argument");
+        gp->impl->setProfiling(on);
+    });
+}
+
+int sp_hip_gp_kernel_stats(const sp_hip_gp* gp, double* ms, int64_t* launches, double* bytes,
+                           double* flops) {
+    return guarded([&] {
+        if (!gp) throw sphip::InvalidArg("null argument");
+        gp->impl->kernelStats(ms, launches, bytes, flops);
+    });
+}
+
+void sp_hip_gp_destroy(sp_hip_gp* gp) { delete gp; }
+
+int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b, void* const* c,
+                int m, int n, int k) {
+    return guarded([&] {
+        sphip::requireDevice();
+        if (!a || !b || !c || batch < 1 || batch > 4) throw sphip::InvalidArg("bad gemm batch");
+        if (m <= 0 || n <= 0 || k <= 0 || m % 64 || n % 64 || k % 64)
+            throw sphip::InvalidArg("gemm dims must be positive multiples of 64");
+        if (flags & SP_HIP_FLAG_FP64)
+            sphip::launchGemm<double>(batch, reinterpret_cast<const double* const*>(a),
+                                      reinterpret_cast<const double* const*>(b),
+                                      reinterpret_cast<double* const*>(c), m, n, k, k, n, n, nullptr);
+        else
+            sphip::launchGemm<float>(batch, reinterpret_cast<const float* const*>(a),
+                                     reinterpret_cast<const float* const*>(b),
+                                     reinterpret_cast<float* const*>(c), m, n, k, k, n, n, nullptr);
+        SPHIP_CHECK(hipDeviceSynchronize());
+    });
+}
+
+}  // extern "C"

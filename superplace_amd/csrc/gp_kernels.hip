@@ -1,0 +1,514 @@
+// Per-evaluation kernels of the ePlace iteration for gfx950.
+//
+// Determinism: the density scatter accumulates 64-bit fixed-point integers
+// with atomics (integer addition is associative), and every FP64 reduction is
+// a fixed-shape tree: per-block shuffle trees, per-block partials, and the
+// last block to arrive sums the partials in block order.  Results are
+// therefore bitwise reproducible run to run.
+#include <cmath>
+
+#include "gp_kernels.h"
+
+namespace sphip {
+namespace {
+
+// ------------------------------------------------------------------ reductions
+template <int NV>
+__device__ inline void blockSum(double (&v)[NV]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += __shfl_down(v[j], off, 64);
+    __shared__ double sh[kBlock / 64][NV];
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) sh[wave][j] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double s = 0;
+            for (int w = 0; w < kBlock / 64; ++w) s += sh[w][j];
+            v[j] = s;
+        }
+    __syncthreads();
+}
+
+__device__ inline double loadRelaxed(const double* p) {
+    const unsigned long long bits = __hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double(static_cast<long long>(bits));
+}
+
+// Block partials, then the last block sums them in block order and calls
+// fin(total) on thread 0.  Every thread of every block must call it.
+template <int NV, typename Fin>
+__device__ inline void gridReduce(double (&v)[NV], double* partials, unsigned* counter, Fin fin) {
+    blockSum<NV>(v);
+    __shared__ int amLast;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) partials[static_cast<long>(blockIdx.x) * NV + j] = v[j];
+        const unsigned prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amLast = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!amLast) return;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] += loadRelaxed(&partials[static_cast<long>(b) * NV + j]);
+    blockSum<NV>(acc);
+    if (threadIdx.x == 0) {
+        fin(acc);
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------------------------ footprint
+template <typename T>
+struct Footprint {
+    int bx0, bx1, by0, by1;
+    T lx, ly, hx, hy, wt;
+};
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// BinGrid::footprintOf + chargeWeight (bin_grid.cpp:130-170): cells narrower
+// (shorter) than a bin are widened to one bin, and the charge is scaled by
+// w/stepX (h/stepY); macros are additionally scaled by the target density.
+// Contraction is off so the rounding matches the reference's C++ exactly.
+template <typename T>
+__device__ inline Footprint<T> footprintOf(T cx, T cy, T w, T h, uint8_t f, const GridGeom<T>& g) {
+#pragma clang fp contract(off)
+    Footprint<T> fp;
+    T lx = cx - T(0.5) * w, hx = cx + T(0.5) * w;
+    T ly = cy - T(0.5) * h, hy = cy + T(0.5) * h;
+    T sx = T(1), sy = T(1);
+    if (w < g.stepX) {
+        sx = w / g.stepX;
+        lx = cx - T(0.5) * g.stepX;
+        hx = cx + T(0.5) * g.stepX;
+    }
+    if (h < g.stepY) {
+        sy = h / g.stepY;
+        ly = cy - T(0.5) * g.stepY;
+        hy = cy + T(0.5) * g.stepY;
+    }
+    const T scale = sx * sy;
+    fp.wt = (f & F_FILLER) ? scale : ((f & F_MACRO) ? scale * g.target : scale);
+    fp.lx = lx; fp.hx = hx; fp.ly = ly; fp.hy = hy;
+    fp.bx0 = clampi(static_cast<int>((lx - g.originX) / g.stepX), 0, g.dim - 1);
+    fp.bx1 = clampi(static_cast<int>((hx - g.originX) / g.stepX), 0, g.dim - 1);
+    fp.by0 = clampi(static_cast<int>((ly - g.originY) / g.stepY), 0, g.dim - 1);
+    fp.by1 = clampi(static_cast<int>((hy - g.originY) / g.stepY), 0, g.dim - 1);
+    return fp;
+}
+
+// overlapArea(bin, rect) * w (geometry.h:35-39, bin_grid.cpp:328-334).
+template <typename T>
+__device__ inline T binCharge(int bx, int by, const Footprint<T>& fp, const GridGeom<T>& g) {
+#pragma clang fp contract(off)
+    const T blx = g.originX + static_cast<T>(bx) * g.stepX;
+    const T bly = g.originY + static_cast<T>(by) * g.stepY;
+    const T dx = fmin(blx + g.stepX, fp.hx) - fmax(blx, fp.lx);
+    const T dy = fmin(bly + g.stepY, fp.hy) - fmax(bly, fp.ly);
+    const T ov = (dx > T(0) && dy > T(0)) ? dx * dy : T(0);
+    return ov * fp.wt;
+}
+
+__device__ inline long long toFixed(float v, float s) { return __float2ll_rn(v * s); }
+__device__ inline long long toFixed(double v, double s) { return __double2ll_rn(v * s); }
+
+// ------------------------------------------------------------------ (a) scatter
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
+                                                    unsigned long long* nodeQ,
+                                                    unsigned long long* fillQ) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nDens) return;
+    const int i = d.densIdx[t];
+    const uint8_t f = d.flags[i];
+    const Footprint<T> fp = footprintOf<T>(d.x[i], d.y[i], d.w[i], d.h[i], f, g);
+    unsigned long long* grid = (f & F_FILLER) ? fillQ : nodeQ;
+    for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
+        for (int by = fp.by0; by <= fp.by1; ++by) {
+            const T ov = binCharge<T>(bx, by, fp, g);
+            if (ov <= T(0)) continue;
+            const long long q = toFixed(ov, qscale);
+            atomicAdd(&grid[static_cast<long>(bx) * g.dim + by], static_cast<unsigned long long>(q));
+        }
+}
+
+// ------------------------------------------------------------------ rho + tau
+// rho = (node + filler + terminal + base) / binArea (bin_grid.cpp:361-368);
+// tau = sum_b max(0, (node + terminal + base)/area - target) area / denom
+// (bin_grid.cpp:373-393, fillers excluded).  Clears the integer grids.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned long long* nodeQ,
+                                                     unsigned long long* fillQ,
+                                                     const double* terminal, const double* base,
+                                                     double invQ, double binArea, double denom,
+                                                     double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
+                                                     double* partials, unsigned* counter,
+                                                     double* scalars) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = g.dim * g.dim;
+    double v[1] = {0.0};
+    if (b < nb) {
+        const long long qn = static_cast<long long>(nodeQ[b]);
+        const long long qf = static_cast<long long>(fillQ[b]);
+        const double node = static_cast<double>(qn) * invQ;
+        const double fill = static_cast<double>(qf) * invQ;
+        nodeQ[b] = 0;
+        fillQ[b] = 0;
+        const double tb = terminal[b], bb = base[b];
+        const int bx = b / g.dim, by = b - bx * g.dim;
+        // a constant shift only changes the DC term, which the solve discards;
+        // it keeps the transform inputs small (smaller FP32 rounding)
+        rho[static_cast<long>(bx) * g.ld + by] =
+            static_cast<T>((node + fill + tb + bb) / binArea - rhoShift);
+        const double over = (node + tb + bb) / binArea - static_cast<double>(g.target);
+        v[0] = over > 0.0 ? over * binArea : 0.0;
+        if (dbgNode) dbgNode[b] = qn;
+        if (dbgFill) dbgFill[b] = qf;
+    }
+    gridReduce<1>(v, partials, counter, [&](double (&s)[1]) {
+        scalars[0] = denom > 0.0 ? s[0] / denom : 0.0;
+    });
+}
+
+// ------------------------------------------------------------------ (c) WA pass 1 + HPWL
+template <typename T>
+__device__ inline T texp(T v);
+template <>
+__device__ inline float texp<float>(float v) { return expf(v); }
+template <>
+__device__ inline double texp<double>(double v) { return exp(v); }
+
+template <typename T>
+__device__ inline T tmax() { return sizeof(T) == 4 ? T(3.402823466e+38f) : T(1.7976931348623157e308); }
+
+// One net per thread (degree <= kLargeNet): extrema, max-shifted exponentials
+// and centred sums of WaWirelength::compute pass 1 (wa_wirelength.cpp:119-173);
+// the HPWL of the net (hpwl.cpp:20-37) from the same extrema.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_wa_small(DevDesign<T> d, T invG, int ignoreDeg,
+                                                     double* partials, unsigned* counter,
+                                                     double* scalars) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (s < d.nSmallNets) {
+        const int k = d.smallNets[s];
+        const int b = d.netStart[k], e = d.netStart[k + 1];
+        const int deg = e - b;
+        NetStat<T> st{};
+        if (deg >= 2) {
+            T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+            for (int t = b; t < e; ++t) {
+                const int nd = d.slotNode[t];
+                const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+                mxX = fmax(mxX, px); mnX = fmin(mnX, px);
+                mxY = fmax(mxY, py); mnY = fmin(mnY, py);
+            }
+            v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+            if (deg <= ignoreDeg) {
+                T bpx = 0, bnx = 0, cpx = 0, cnx = 0, bpy = 0, bny = 0, cpy = 0, cny = 0;
+                for (int t = b; t < e; ++t) {
+                    const int nd = d.slotNode[t];
+                    const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+                    const T dxp = px - mxX, dxn = px - mnX;
+                    const T ap = texp<T>(dxp * invG), an = texp<T>(-dxn * invG);
+                    bpx += ap; cpx += dxp * ap; bnx += an; cnx += dxn * an;
+                    const T dyp = py - mxY, dyn = py - mnY;
+                    const T bp = texp<T>(dyp * invG), bn = texp<T>(-dyn * invG);
+                    bpy += bp; cpy += dyp * bp; bny += bn; cny += dyn * bn;
+                }
+                st = NetStat<T>{mxX, mnX, mxY, mnY, bpx, bnx, cpx, cnx, bpy, bny, cpy, cny};
+                v[0] = static_cast<double>((mxX - mnX) + cpx / bpx - cnx / bnx) +
+                       static_cast<double>((mxY - mnY) + cpy / bpy - cny / bny);
+            }
+        }
+        d.stats[k] = st;   // bPX == 0 marks a skipped net (wa_wirelength.cpp:123-128)
+    }
+    gridReduce<2>(v, partials, counter, [&](double (&t)[2]) {
+        scalars[1] = t[0];
+        scalars[2] = t[1];
+    });
+}
+
+template <typename T>
+__device__ inline T waveMax(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <typename T>
+__device__ inline T waveMin(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <typename T>
+__device__ inline T waveSum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One net per wavefront (degree > kLargeNet, the power-law tail): the same
+// quantities with lane-strided pins and shuffle trees.  Adds its totals to
+// those of k_wa_small, which ran before it on the same stream.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int ignoreDeg,
+                                                     double* partials, unsigned* counter,
+                                                     double* scalars) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    double v[2] = {0.0, 0.0};
+    if (s < d.nLargeNets) {
+        const int k = d.largeNets[s];
+        const int b = d.netStart[k], e = d.netStart[k + 1];
+        const int deg = e - b;
+        T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+        for (int t = b + lane; t < e; t += 64) {
+            const int nd = d.slotNode[t];
+            const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+            mxX = fmax(mxX, px); mnX = fmin(mnX, px);
+            mxY = fmax(mxY, py); mnY = fmin(mnY, py);
+        }
+        mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
+        NetStat<T> st{};
+        double wl = 0.0;
+        if (deg <= ignoreDeg) {
+            T bpx = 0, bnx = 0, cpx = 0, cnx = 0, bpy = 0, bny = 0, cpy = 0, cny = 0;
+            for (int t = b + lane; t < e; t += 64) {
+                const int nd = d.slotNode[t];
+                const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+                const T dxp = px - mxX, dxn = px - mnX;
+                const T ap = texp<T>(dxp * invG), an = texp<T>(-dxn * invG);
+                bpx += ap; cpx += dxp * ap; bnx += an; cnx += dxn * an;
+                const T dyp = py - mxY, dyn = py - mnY;
+                const T bp = texp<T>(dyp * invG), bn = texp<T>(-dyn * invG);
+                bpy += bp; cpy += dyp * bp; bny += bn; cny += dyn * bn;
+            }
+            bpx = waveSum(bpx); bnx = waveSum(bnx); cpx = waveSum(cpx); cnx = waveSum(cnx);
+            bpy = waveSum(bpy); bny = waveSum(bny); cpy = waveSum(cpy); cny = waveSum(cny);
+            st = NetStat<T>{mxX, mnX, mxY, mnY, bpx, bnx, cpx, cnx, bpy, bny, cpy, cny};
+            wl = static_cast<double>((mxX - mnX) + cpx / bpx - cnx / bnx) +
+                 static_cast<double>((mxY - mnY) + cpy / bpy - cny / bny);
+        }
+        if (lane == 0) {
+            d.stats[k] = st;
+            v[0] = wl;
+            v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+        }
+    }
+    gridReduce<2>(v, partials, counter, [&](double (&t)[2]) {
+        scalars[1] += t[0];
+        scalars[2] += t[1];
+    });
+}
+
+// ------------------------------------------------------------------ (c)+(d) node pass
+// For each optimised node k (node i = optIdx[k]):
+//  * WA gradient over the node's pins (wa_wirelength.cpp:183-225),
+//  * density force F_i = sum_bins q * xi (bin_grid.cpp:395-426),
+//  * preconditioned gradient grad = (dWL - lambda F) / max(1, deg + lambda area)
+//    (eplace.cpp:162-185),
+//  * FP64 partial sums: sum|dWL|, sum|F| (eplace.cpp:168-190) and the
+//    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
+//    (nesterov.cpp:25-44), plus a count of non-finite gradients.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* ex,
+                                                      const T* ey, T invG, float lambda,
+                                                      const T* u, const T* up, const T* gp, T* grad,
+                                                      T* dbgForce, T* dbgWl, double* partials,
+                                                      unsigned* counter, double* scalars) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = d.nOpt;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (k < n) {
+        const int i = d.optIdx[k];
+        const T xi = d.x[i], yi = d.y[i];
+        T gx = 0, gy = 0;
+        const bool hasWl = i < d.numMov;
+        if (hasWl) {
+            for (int t = d.nodeStart[i]; t < d.nodeStart[i + 1]; ++t) {
+                const int net = d.npNet[t];
+                if (net < 0) continue;
+                const NetStat<T> st = d.stats[net];
+                if (st.bPX <= T(0)) continue;
+                const T px = xi + d.npOffX[t], py = yi + d.npOffY[t];
+                {
+                    const T dp = px - st.mxX, dn = px - st.mnX;
+                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
+                    const T tMax = ap * (st.bPX + (dp * st.bPX - st.cPX) * invG) / (st.bPX * st.bPX);
+                    const T tMin = an * (st.bNX + (st.cNX - dn * st.bNX) * invG) / (st.bNX * st.bNX);
+                    gx += tMax - tMin;
+                }
+                {
+                    const T dp = py - st.mxY, dn = py - st.mnY;
+                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
+                    const T tMax = ap * (st.bPY + (dp * st.bPY - st.cPY) * invG) / (st.bPY * st.bPY);
+                    const T tMin = an * (st.bNY + (st.cNY - dn * st.bNY) * invG) / (st.bNY * st.bNY);
+                    gy += tMax - tMin;
+                }
+            }
+        }
+        T fx = 0, fy = 0;
+        const uint8_t f = d.flags[i];
+        if (!(f & (F_FIXED | F_NI))) {
+            const Footprint<T> fp = footprintOf<T>(xi, yi, d.w[i], d.h[i], f, g);
+            for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
+                for (int by = fp.by0; by <= fp.by1; ++by) {
+                    const T q = binCharge<T>(bx, by, fp, g);
+                    if (q <= T(0)) continue;
+                    const long idx = static_cast<long>(bx) * g.ld + by;
+                    fx += q * ex[idx];
+                    fy += q * ey[idx];
+                }
+        }
+        const T lam = static_cast<T>(lambda);
+        const T precond = static_cast<T>(
+            1.0 / fmax(1.0, static_cast<double>(d.precDeg[k]) +
+                                static_cast<double>(lambda) * d.precArea[k]));
+        const T g0 = precond * (gx - lam * fx);
+        const T g1 = precond * (gy - lam * fy);
+        grad[k] = g0;
+        grad[n + k] = g1;
+        if (dbgForce) { dbgForce[k] = fx; dbgForce[n + k] = fy; }
+        if (dbgWl) { dbgWl[k] = gx; dbgWl[n + k] = gy; }
+        v[0] = hasWl ? fabs(static_cast<double>(gx)) + fabs(static_cast<double>(gy)) : 0.0;
+        v[1] = fabs(static_cast<double>(fx)) + fabs(static_cast<double>(fy));
+        const double u0 = u[k], u1 = u[n + k];
+        const double du0 = u0 - (up ? static_cast<double>(up[k]) : 0.0);
+        const double du1 = u1 - (up ? static_cast<double>(up[n + k]) : 0.0);
+        const double dg0 = static_cast<double>(g0) - (gp ? static_cast<double>(gp[k]) : 0.0);
+        const double dg1 = static_cast<double>(g1) - (gp ? static_cast<double>(gp[n + k]) : 0.0);
+        v[2] = du0 * du0 + du1 * du1;
+        v[3] = dg0 * dg0 + dg1 * dg1;
+        v[4] = du0 * dg0 + du1 * dg1;
+        v[5] = static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1;
+        v[6] = (isfinite(g0) ? 0.0 : 1.0) + (isfinite(g1) ? 0.0 : 1.0);
+    }
+    gridReduce<7>(v, partials, counter, [&](double (&t)[7]) {
+        for (int j = 0; j < 7; ++j) scalars[3 + j] = t[j];
+    });
+}
+
+// ------------------------------------------------------------------ (d) update
+// NesterovOptimizer::step / redoIfOvershot arithmetic (nesterov.cpp:97-107,
+// :156-165) with EPlace's clamp (eplace.cpp:117-133) and scatter (:97-105).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
+                                                   const T* B, T alpha, T m, T* vOut, T* uOut) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = d.nOpt;
+    if (k >= n) return;
+    const int i = d.optIdx[k];
+    const T hw = T(0.5) * d.w[i], hh = T(0.5) * d.h[i];
+    const T eps = static_cast<T>(1e-3f);
+    const T xlo = d.coreLx + hw + eps;
+    const T xhi = fmax(xlo, d.coreHx - hw - eps);
+    const T ylo = d.coreLy + hh + eps;
+    const T yhi = fmax(ylo, d.coreHy - hh - eps);
+    T vv[2], uu[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int j = a * n + k;
+        const T vn = G ? A[j] - alpha * G[j] : A[j];
+        const T un = B ? vn + m * (vn - B[j]) : vn;
+        const T lo = a ? ylo : xlo, hi = a ? yhi : xhi;
+        vv[a] = fmin(fmax(vn, lo), hi);
+        uu[a] = fmin(fmax(un, lo), hi);
+        if (vOut) vOut[j] = vv[a];
+        if (uOut) uOut[j] = uu[a];
+    }
+    d.x[i] = uOut ? uu[0] : vv[0];
+    d.y[i] = uOut ? uu[1] : vv[1];
+}
+
+long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+template <typename T>
+void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
+                   unsigned long long* nodeQ, unsigned long long* fillQ, hipStream_t s) {
+    if (d.nDens == 0) return;
+    k_scatter<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, qscale, nodeQ, fillQ);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
+                    const double* terminal, const double* base, double invQ, double binArea,
+                    double movableAreaScaled, double rhoShift, T* rho, long long* dbgNode,
+                    long long* dbgFill, const ReduceWs& ws, hipStream_t s) {
+    const long nb = static_cast<long>(g.dim) * g.dim;
+    k_finalize<T><<<gridFor(nb), kBlock, 0, s>>>(g, nodeQ, fillQ, terminal, base, invQ, binArea,
+                                                 movableAreaScaled, rhoShift, rho, dbgNode, dbgFill,
+                                                 ws.partials, ws.counters + 0, ws.scalars);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const ReduceWs& ws,
+                  hipStream_t s) {
+    // wa_wirelength.cpp:95-97: gamma floored at 1e-4, then inverted in the working precision
+    const T g = static_cast<T>(gamma) > T(1e-4f) ? static_cast<T>(gamma) : T(1e-4f);
+    const T invG = T(1) / g;
+    const long slot = static_cast<long>(ws.maxBlocks) * 8;
+    k_wa_small<T><<<gridFor(d.nSmallNets > 0 ? d.nSmallNets : 1), kBlock, 0, s>>>(
+        d, invG, ignoreDeg, ws.partials + slot, ws.counters + 1, ws.scalars);
+    SPHIP_CHECK(hipGetLastError());
+    if (d.nLargeNets > 0) {
+        k_wa_large<T><<<(d.nLargeNets + 3) / 4, kBlock, 0, s>>>(d, invG, ignoreDeg,
+                                                                 ws.partials + 2 * slot,
+                                                                 ws.counters + 2, ws.scalars);
+        SPHIP_CHECK(hipGetLastError());
+    }
+}
+
+template <typename T>
+void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
+                    float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
+                    T* dbgForce, T* dbgWl, const ReduceWs& ws, hipStream_t s) {
+    const T gm = static_cast<T>(gamma) > T(1e-4f) ? static_cast<T>(gamma) : T(1e-4f);
+    const T invG = T(1) / gm;
+    const long slot = static_cast<long>(ws.maxBlocks) * 8;
+    k_node_grad<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, g, ex, ey, invG, lambda, u, up, gp, grad,
+                                                      dbgForce, dbgWl, ws.partials + 3 * slot,
+                                                      ws.counters + 3, ws.scalars);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
+                  double m, T* vOut, T* uOut, hipStream_t s) {
+    k_update<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, A, G, B, static_cast<T>(alpha),
+                                                   static_cast<T>(m), vOut, uOut);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+#define SPHIP_INST(T)                                                                            \
+    template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
+                                   unsigned long long*, unsigned long long*, hipStream_t);       \
+    template void launchFinalize<T>(const GridGeom<T>&, unsigned long long*, unsigned long long*, \
+                                    const double*, const double*, double, double, double, double, \
+                                    T*, long long*, long long*, const ReduceWs&, hipStream_t);   \
+    template void launchWaNets<T>(const DevDesign<T>&, float, int, const ReduceWs&, hipStream_t); \
+    template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*, const T*, \
+                                    float, float, const T*, const T*, const T*, T*, T*, T*,      \
+                                    const ReduceWs&, hipStream_t);                               \
+    template void launchUpdate<T>(const DevDesign<T>&, const T*, const T*, const T*, double,     \
+                                  double, T*, T*, hipStream_t);
+SPHIP_INST(float)
+SPHIP_INST(double)
+
+}  // namespace sphip

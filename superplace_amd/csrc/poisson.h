@@ -1,0 +1,62 @@
+// DCT Poisson solve as batched dense transforms on MFMA (host-side plan).
+//
+// Solves lap(phi) = -rho with Neumann boundaries and returns the field
+// xi = -grad(phi) with the conventions of the reference's CPU backend
+// (src/numeric/cpu/cpu_poisson.cpp:3-40, :149-207): forward REDFT10 on both
+// axes, P = F / (4 nx ny) / (u_j^2 + v_k^2) with P[0][0] = 0, then
+//   phi  = REDFT01_x REDFT01_y P
+//   xi_x = RODFT01_x REDFT01_y (P shifted down one row, rows scaled by u)
+//   xi_y = REDFT01_x RODFT01_y (P shifted left one column, columns scaled by v)
+// Each 1-D transform along an axis is a product with a dense nx x nx (or
+// ny x ny) matrix; the index shift and the u/v scaling are folded into the
+// sine matrices, and the spectral divide is the epilogue of the second
+// forward product.  Layout: row-major, x slow, y fast (data[x * ld + y]),
+// padded to a multiple of 64 on both axes with zero rows/columns.
+#pragma once
+
+#include "common.h"
+
+namespace sphip {
+
+template <typename T>
+class PoissonPlan {
+public:
+    PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream);
+    ~PoissonPlan();
+    PoissonPlan(const PoissonPlan&) = delete;
+    PoissonPlan& operator=(const PoissonPlan&) = delete;
+
+    // rho() must hold the padded density (zeros in the padding).  Writes
+    // fieldX()/fieldY() and, if withPhi, phi().  Enqueued on the stream.
+    void solve(bool withPhi);
+
+    int nx() const { return nx_; }
+    int ny() const { return ny_; }
+    int ld() const { return my_; }          // row stride of every grid buffer
+    int rowsPadded() const { return mx_; }
+    T* rho() { return rho_; }
+    T* phi() { return phi_; }
+    T* fieldX() { return ex_; }
+    T* fieldY() { return ey_; }
+    hipStream_t stream() const { return stream_; }
+    // Floating-point operations of one solve (dense products actually executed).
+    double flops(bool withPhi) const;
+
+private:
+    int nx_, ny_, mx_, my_;
+    hipStream_t stream_;
+    T *rho_ = nullptr, *phi_ = nullptr, *ex_ = nullptr, *ey_ = nullptr;
+    T *t0_ = nullptr, *p_ = nullptr, *t1_ = nullptr, *t2_ = nullptr;
+    T *cyT_ = nullptr, *cx_ = nullptr, *ayT_ = nullptr, *bsv_ = nullptr, *sxu_ = nullptr,
+      *ax_ = nullptr;
+    T *u_ = nullptr, *v_ = nullptr;
+    T invNorm_;
+};
+
+// Batched C = A * B (row-major, dims multiples of 64) on MFMA; used by the
+// plan and exposed for the GEMM unit test.
+template <typename T>
+void launchGemm(int batch, const T* const* A, const T* const* B, T* const* C, int M, int N,
+                int K, int lda, int ldb, int ldc, hipStream_t stream);
+
+}  // namespace sphip

@@ -1,0 +1,148 @@
+"""HIP Poisson backend (DCT as dense transforms on MFMA) against the oracle,
+the scipy golden fixtures and the analytic properties of
+test/test_poisson.cpp; field tolerance max|d|/max|ref| < 1e-5 for FP32 (the
+reference's own CPU/GPU criterion, test/test_poisson_parity.cpp:22) and
+< 1e-10 for the FP64 path."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from superplace_amd import hip
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def solve(rho, sx, sy, fp64=False):
+    nx, ny = rho.shape
+    be = hip.make_poisson_backend(hip.BackendConfig(nx, ny, sx, sy, fp64=fp64))
+    out = [np.zeros((nx, ny), np.float32) for _ in range(3)]
+    be.solve(np.ascontiguousarray(rho, np.float32), *out)
+    return be, out
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (17, 29), (128, 256), (512, 512), (1024, 1024)])
+def test_matches_oracle(shape):
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 7 + ny)
+    rho = rng.uniform(0, 1, shape).astype(np.float32)
+    sx, sy = 22.65625, 21.0
+    be, (phi, ex, ey) = solve(rho, sx, sy)
+    assert be.name() == "hip-mfma"
+    rphi, rex, rey = oracle.poisson(rho.astype(np.float64), sx, sy)
+    print(shape, rel(phi, rphi), rel(ex, rex), rel(ey, rey), flush=True)
+    assert rel(phi, rphi) < 1e-5 and rel(ex, rex) < 1e-5 and rel(ey, rey) < 1e-5
+
+
+def test_fp64_path_matches_oracle_tightly():
+    nx = ny = 128
+    rho = np.random.default_rng(5).uniform(0, 1, (nx, ny))
+    be = hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 3.0, 3.0, fp64=True))
+    assert be.name() == "hip-mfma-fp64"
+    from tests.hiprt import DevArray
+    d_in = DevArray(rho.astype(np.float64))
+    d_out = DevArray.zeros(3 * nx * ny, np.float64)
+    be.solve_device(d_in.ptr, d_out.ptr)
+    out = d_out.get().reshape(3, nx, ny)
+    ref = oracle.poisson(rho, 3.0, 3.0)
+    for got, r in zip(out, ref):
+        assert rel(got, r) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["8x8", "16x12", "32x32", "12x20"])
+def test_golden_fixtures(name):
+    g = np.load(os.path.join(GOLD, f"poisson_{name}.npz"))
+    _, out = solve(g["rho"], *g["step"])
+    for got, ref in zip(out, (g["phi"], g["ex"], g["ey"])):
+        assert rel(got, ref) < 1e-5
+
+
+def grid(n, f):
+    xs = np.arange(n)[:, None] + 0.5
+    ys = np.arange(n)[None, :] + 0.5
+    return f(xs, ys)
+
+
+def test_analytic_properties():
+    n, step = 64, 2.5
+    # 1 Gaussian: outward field, Neumann boundary
+    rho = grid(n, lambda X, Y: np.exp(-((X - n / 2) ** 2 + (Y - n / 2) ** 2) / (2 * (n / 10) ** 2)))
+    rho -= rho.mean()
+    _, (phi, ex, ey) = solve(rho, step, step)
+    scale = np.abs(ex).max() + np.abs(ey).max()
+    X = np.arange(n)[:, None] + 0.5 - n / 2
+    Y = np.arange(n)[None, :] + 0.5 - n / 2
+    r = np.sqrt(X ** 2 + Y ** 2)
+    m = (r >= 3) & (r <= 0.4 * n) & (np.sqrt(ex.astype(float) ** 2 + ey ** 2) >= 1e-3 * scale)
+    assert m.sum() > 100 and ((X * ex + Y * ey)[m] > 0).mean() >= 0.99
+    assert max(np.abs(ex[0]).max(), np.abs(ex[-1]).max(), np.abs(ey[:, 0]).max(),
+               np.abs(ey[:, -1]).max()) < 0.05 * scale
+    # 2 uniform -> zero field
+    rho = np.full((n, n), 3.7, np.float32)
+    rho -= rho.mean()
+    _, (phi, ex, ey) = solve(rho, step, step)
+    assert np.abs(ex).max() < 1e-4 and np.abs(ey).max() < 1e-4
+    # 3 mirror antisymmetry
+    rho = grid(n, lambda X, Y: np.exp(-((X - n / 2) ** 2) / 50) * (1 + 0.3 * np.sin(0.2 * (Y - 0.5))))
+    rho -= rho.mean()
+    _, (phi, ex, ey) = solve(rho, step, step)
+    assert np.abs(ex + ex[::-1]).max() < 1e-3 * np.abs(ex).max()
+    # 4 single mode: phi = rho / (u^2 + v^2)
+    j0, k0 = 3, 5
+    u, v = np.pi * j0 / (n * step), np.pi * k0 / (n * step)
+    rho = grid(n, lambda X, Y: np.cos(np.pi * j0 * X / n) * np.cos(np.pi * k0 * Y / n))
+    _, (phi, ex, ey) = solve(rho, step, step)
+    mask = np.abs(rho) >= 0.1
+    assert np.max(np.abs(phi[mask] / rho[mask] * (u * u + v * v) - 1)) < 1e-4
+    # 5 xi = -grad phi by central differences
+    rho = grid(n, lambda X, Y: np.cos(np.pi * 2 * X / n) * np.cos(np.pi * 3 * Y / n)
+               + 0.5 * np.cos(np.pi * X / n))
+    rho -= rho.mean()
+    _, (phi, ex, ey) = solve(rho, step, step)
+    gx = (phi[2:, 1:-1].astype(float) - phi[:-2, 1:-1]) / (2 * step)
+    gy = (phi[1:-1, 2:].astype(float) - phi[1:-1, :-2]) / (2 * step)
+    assert np.abs(ex[1:-1, 1:-1] + gx).max() < 0.01 * np.abs(ex).max()
+    assert np.abs(ey[1:-1, 1:-1] + gy).max() < 0.01 * np.abs(ey).max()
+
+
+def test_determinism_and_reuse():
+    rho = np.random.default_rng(9).uniform(0, 1, (256, 256)).astype(np.float32)
+    be1, a = solve(rho, 2.0, 2.0)
+    _, b = solve(rho, 2.0, 2.0)
+    c = [np.zeros_like(rho) for _ in range(3)]
+    be1.solve(rho, *c)
+    for p, q, r in zip(a, b, c):
+        assert np.array_equal(p, q) and np.array_equal(p, r)
+
+
+def test_bad_span_rejected():
+    be = hip.HipPoissonBackend(hip.BackendConfig(16, 16, 1.0, 1.0))
+    with pytest.raises(ValueError):
+        be.solve(np.zeros((8, 8), np.float32), *[np.zeros((16, 16), np.float32)] * 3)
+    with pytest.raises(hip.HipError):
+        hip.HipPoissonBackend(hip.BackendConfig(1, 16, 1.0, 1.0))
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("mnk", [(192, 128, 320), (1024, 1024, 512)])
+def test_gemm_matches_numpy(fp64, mnk):
+    from tests.hiprt import DevArray
+    dt = np.float64 if fp64 else np.float32
+    rng = np.random.default_rng(3)
+    m, n, k = mnk
+    A = [rng.standard_normal((m, k)).astype(dt) for _ in range(3)]
+    B = [rng.standard_normal((k, n)).astype(dt) for _ in range(3)]
+    dA, dB = [DevArray(a) for a in A], [DevArray(b) for b in B]
+    dC = [DevArray.zeros((m, n), dt) for _ in range(3)]
+    hip.gemm([a.ptr for a in dA], [b.ptr for b in dB], [c.ptr for c in dC], m, n, k, fp64=fp64)
+    for a, b, c in zip(A, B, dC):
+        ref = a.astype(np.float64) @ b.astype(np.float64)
+        err = np.abs(c.get().astype(np.float64) - ref).max() / np.abs(ref).max()
+        # FP32: k products accumulated in FP32, about sqrt(k) * 2^-24 relative
+        assert err < (1e-13 if fp64 else 1e-5), err
