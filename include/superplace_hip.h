@@ -123,6 +123,12 @@ typedef struct sp_hip_gp_config {
     int32_t stage;                 /* 0 mGP, 1 FILLERONLY, 2 cGP (src/gp/eplace.h:24-28) */
     int32_t precision;             /* 32: FP32 bulk + FP64 reductions; 64: FP64 everywhere */
     int32_t max_backtrack;         /* OptConfig::max_backtrack (src/numeric/optimizer.h:223) */
+    /* How the iteration loop is driven (no counterpart in the reference):
+     * 0 = host decisions, the host reads the scalars after every evaluation;
+     * 1 = decisions on the device, kernels launched slot by slot;
+     * 2 = decisions on the device, graph_slots iterations per hipGraph launch. */
+    int32_t loop_mode;
+    int32_t graph_slots;
 } sp_hip_gp_config;
 
 /* One metrics.csv row (src/util/metrics.h:12-28). */

@@ -8,6 +8,7 @@
 // decisions (backtracking, step size, schedules, stopping).
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -300,6 +301,10 @@ public:
     void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) override;
 
 private:
+    void runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows, float lambda,
+                   float gamma, double alpha0, double effRef, float targetTau, int maxIter,
+                   int maxBt, float stepX);
+    void launchSlot(bool events);
     void build(const sp_hip_design& d);
     void release();
     EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
@@ -354,6 +359,16 @@ private:
       *probe_ = nullptr;
     const T* curPos_ = nullptr;
     std::vector<T*> snapBufs_;
+    // device-resident loop: state, buffer table, metrics rows, snapshot pool, graph
+    Ctl* dCtl_ = nullptr;
+    T** dBufs_ = nullptr;
+    DevRow* dRows_ = nullptr;
+    int rowsCap_ = 0;
+    T* snapPool_[kMaxSnaps] = {};
+    int* hFlag_ = nullptr;   // pinned copy of ctl->done
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graphExec_ = nullptr;
+    int graphEvMask_ = 0;    // families timed by event nodes inside the graph
     int profMask_ = 0;   // bit f: time kernel family f with HIP events
     hipEvent_t evStart_[SP_HIP_K_COUNT] = {}, evStop_[SP_HIP_K_COUNT] = {};
     bool pendingK_[SP_HIP_K_COUNT] = {};
@@ -370,6 +385,14 @@ void Engine<T>::release() {
     owned_.clear();
     for (T* p : snapBufs_) (void)hipFree(p);
     snapBufs_.clear();
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graphExec_ = nullptr;
+    graph_ = nullptr;
+    if (dRows_) (void)hipFree(dRows_);
+    dRows_ = nullptr;
+    if (hFlag_) (void)hipHostFree(hFlag_);
+    hFlag_ = nullptr;
     if (hScalars_) (void)hipHostFree(hScalars_);
     hScalars_ = nullptr;
     for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
@@ -609,6 +632,11 @@ void Engine<T>::build(const sp_hip_design& d) {
 
     // optimizer state
     const size_t n2 = 2 * static_cast<size_t>(std::max(n_, 1));
+    dCtl_ = static_cast<Ctl*>(own(dalloc<Ctl>(1)));
+    dBufs_ = static_cast<T**>(own(dalloc<T*>(kNumBufs)));
+    if (cfg_.loop_mode != 0)
+        for (int i = 0; i < kMaxSnaps; ++i) snapPool_[i] = static_cast<T*>(own(dalloc<T>(n2)));
+    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hFlag_), sizeof(int), hipHostMallocDefault));
     for (T** b : {&u_, &v_, &uPrev_, &vPrev_, &uNew_, &vNew_, &g_, &gPrev_, &gA_, &gB_, &probe_}) {
         *b = static_cast<T*>(own(dalloc<T>(n2)));
         SPHIP_CHECK(hipMemsetAsync(*b, 0, n2 * sizeof(T), s_));
@@ -633,21 +661,21 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
                                 float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
                                 T* dbgWl) {
     beginK(SP_HIP_K_SCATTER);
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, s_);
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
     endK(SP_HIP_K_SCATTER);
     beginK(SP_HIP_K_FINALIZE);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, s_);
+                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
     endK(SP_HIP_K_FINALIZE);
     beginK(SP_HIP_K_POISSON);
     poisson_->solve(false);
     endK(SP_HIP_K_POISSON);
     beginK(SP_HIP_K_WA_NETS);
-    launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, s_);
+    launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, nullptr, s_);
     endK(SP_HIP_K_WA_NETS);
     beginK(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), gamma, lambda, u, up,
-                      gp, grad, dbgForce, dbgWl, ws_, s_);
+                      gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, nullptr, s_);
     endK(SP_HIP_K_NODE_GRAD);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
@@ -661,7 +689,7 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
 template <typename T>
 void Engine<T>::scatterPos(const T* pos) {
     beginK(SP_HIP_K_UPDATE);
-    launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, s_);
+    launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr, s_);
     endK(SP_HIP_K_UPDATE);
     curPos_ = pos;
 }
@@ -669,10 +697,10 @@ void Engine<T>::scatterPos(const T* pos) {
 // Delivery (eplace.cpp:478-494): positions already scattered; tau and HPWL.
 template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, s_);
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, s_);
-    launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, s_);
+                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
+    launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
@@ -703,7 +731,7 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         }
         SPHIP_CHECK(hipMemcpyAsync(probe_, pos.data(), bytes2n, hipMemcpyHostToDevice, s_));
         // clamp into u (v' = clamp(A) with vOut = u)
-        launchUpdate<T>(dd_, probe_, nullptr, nullptr, 0.0, 0.0, u_, nullptr, s_);
+        launchUpdate<T>(dd_, probe_, nullptr, nullptr, 0.0, 0.0, u_, nullptr, nullptr, nullptr, s_);
         curPos_ = u_;
     }
 
@@ -764,7 +792,7 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         if (gNorm > 0.0 && std::isfinite(gNorm)) {
             const double scale = 0.1 * (posNorm > 0.0 ? posNorm : 1.0) / gNorm;
             launchUpdate<T>(dd_, u_, gA_, nullptr, static_cast<double>(static_cast<T>(scale)), 0.0,
-                            probe_, nullptr, s_);
+                            probe_, nullptr, nullptr, nullptr, s_);
             curPos_ = probe_;
             EvalScalars sB = evaluate(probe_, u_, gA_, gB_, lambda, gamma);
             checkFinite(sB);
@@ -775,6 +803,12 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
                                                              : 1.0;
             }
         }
+    }
+
+    if (cfg_.loop_mode != 0) {
+        runDevice(res, rows, maxRows, lambda, gamma, alpha0, effRef, targetTau, maxIter, maxBt,
+                  stepX);
+        return;
     }
 
     struct Snap {
@@ -811,7 +845,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
                 if (lip > 0.95 * alpha) break;
                 alpha = static_cast<double>(static_cast<T>(lip));
                 beginK(SP_HIP_K_UPDATE);
-                launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, s_);
+                launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, nullptr, nullptr,
+                                s_);
                 endK(SP_HIP_K_UPDATE);
                 curPos_ = u_;
                 sc = evaluate(u_, uPrev_, gPrev_, g_, lambda, gamma);
@@ -842,7 +877,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         const T aNext = T(0.5) * (T(1) + std::sqrt(T(4) * aT * aT + T(1)));
         const T mom = (aT - T(1)) / aNext;
         beginK(SP_HIP_K_UPDATE);
-        launchUpdate<T>(dd_, u_, g_, v_, alpha, static_cast<double>(mom), vNew_, uNew_, s_);
+        launchUpdate<T>(dd_, u_, g_, v_, alpha, static_cast<double>(mom), vNew_, uNew_, nullptr,
+                        nullptr, s_);
         endK(SP_HIP_K_UPDATE);
         // rotate (nesterov.cpp:109-118)
         std::swap(uPrev_, u_);
@@ -933,6 +969,185 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
     deliver(finalPos ? finalPos : v_);
     if (!std::isfinite(res->hpwl) || !std::isfinite(res->overflow))
         throw NonFinite("non-finite delivered metrics");
+}
+
+template <typename T>
+void Engine<T>::launchSlot(bool events) {
+    const T qscale = static_cast<T>(std::ldexp(1.0, fracBits_));
+    const int* skip = reinterpret_cast<const int*>(reinterpret_cast<const char*>(dCtl_) +
+                                                   offsetof(Ctl, done));
+    auto b = [&](int f) { if (events) beginK(f); };
+    auto e = [&](int f) { if (events) endK(f); };
+    b(SP_HIP_K_SCATTER);
+    launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, dCtl_, s_);
+    e(SP_HIP_K_SCATTER);
+    b(SP_HIP_K_FINALIZE);
+    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr,
+                      ws_, dCtl_, s_);
+    e(SP_HIP_K_FINALIZE);
+    b(SP_HIP_K_POISSON);
+    poisson_->solve(false, skip);
+    e(SP_HIP_K_POISSON);
+    b(SP_HIP_K_WA_NETS);
+    launchWaNets<T>(dd_, 0.f, cfg_.ignore_net_degree, ws_, dCtl_, s_);
+    e(SP_HIP_K_WA_NETS);
+    b(SP_HIP_K_NODE_GRAD);
+    launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), 0.f, 0.f, nullptr,
+                      nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, dRows_, s_);
+    e(SP_HIP_K_NODE_GRAD);
+    b(SP_HIP_K_UPDATE);
+    launchUpdate<T>(dd_, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, dCtl_, dBufs_, s_);
+    e(SP_HIP_K_UPDATE);
+}
+
+// The stage loop with the decisions on the device (loop_mode 1 and 2).
+template <typename T>
+void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows,
+                          float lambda, float gamma, double alpha0, double effRef, float targetTau,
+                          int maxIter, int maxBt, float stepX) {
+    const int n = n_;
+    T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
+    for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
+    Ctl c{};
+    c.iu = 0; c.iv = 1; c.iup = 2; c.ivp = 3; c.iun = 4; c.ivn = 5; c.ig = 6; c.igp = 7;
+    c.mode = kModeNone;
+    c.maxIter = maxIter;
+    c.maxBt = maxBt;
+    c.backtrack = cfg_.gp_backtrack;
+    c.useBB = cfg_.use_bb;
+    c.stage = cfg_.stage;
+    c.window = cfg_.gp_stagnation_window;
+    c.finalBuf = -1;
+    c.rollbackIter = -1;
+    c.snapWrite = -1;
+    c.snapSrc = -1;
+    c.nFree = kMaxSnaps;
+    for (int i = 0; i < kMaxSnaps; ++i) c.freeSnap[i] = 8 + kMaxSnaps - 1 - i;
+    c.a = 1.0;
+    c.alpha = 1.0;
+    c.alpha0 = alpha0;
+    c.lambda = lambda;
+    c.gamma = gamma;
+    c.targetTau = targetTau;
+    c.stepX = stepX;
+    c.minGain = std::clamp(cfg_.gp_stagnation_min_gain, 0.f, 0.5f);
+    c.bestTau = std::numeric_limits<float>::infinity();
+    c.refTau = std::numeric_limits<float>::infinity();
+    c.effRef = effRef;
+    if (maxIter <= 0) {   // no step at all: deliver the clamped start
+        res->iterations = 0;
+        scatterPos(u_);
+        double hp = 0;
+        res->overflow = static_cast<float>(deliveredMetrics(&hp));
+        res->hpwl = hp;
+        return;
+    }
+    if (rowsCap_ < maxIter) {
+        if (dRows_) SPHIP_CHECK(hipFree(dRows_));
+        dRows_ = dalloc<DevRow>(maxIter);
+        rowsCap_ = maxIter;
+        if (graphExec_) {   // the graph holds the old rows pointer
+            SPHIP_CHECK(hipGraphExecDestroy(graphExec_));
+            SPHIP_CHECK(hipGraphDestroy(graph_));
+            graphExec_ = nullptr;
+            graph_ = nullptr;
+        }
+    }
+    SPHIP_CHECK(hipMemcpyAsync(dCtl_, &c, sizeof(Ctl), hipMemcpyHostToDevice, s_));
+    SPHIP_CHECK(hipMemcpyAsync(dBufs_, table, sizeof(table), hipMemcpyHostToDevice, s_));
+    if (curPos_ != u_) scatterPos(u_);
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+
+    const bool graph = cfg_.loop_mode == 2;
+    const int slots = graph ? std::max(1, cfg_.graph_slots) : (profMask_ ? 1 : std::max(1, cfg_.graph_slots));
+    if (graph && !graphExec_) {
+        // capture `slots` iterations; families in profMask_ get event nodes in the first slot
+        graphEvMask_ = profMask_;
+        const int saved = profMask_;
+        SPHIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < slots; ++i) launchSlot(i == 0 && graphEvMask_ != 0);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SPHIP_CHECK(hipStreamEndCapture(s_, &graph_));
+        SPHIP_CHECK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+        profMask_ = saved;
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f) pendingK_[f] = false;
+    }
+    const long maxLaunches = (static_cast<long>(maxIter) * maxBt + 16) / slots + 2;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long launch = 0;; ++launch) {
+        if (launch > maxLaunches) throw std::runtime_error("device loop did not terminate");
+        if (graph) {
+            SPHIP_CHECK(hipGraphLaunch(graphExec_, s_));
+        } else {
+            for (int i = 0; i < slots; ++i) launchSlot(profMask_ != 0);
+        }
+        SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
+                                   sizeof(int), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+        const bool done = *hFlag_ != 0;
+        if (graph && graphEvMask_ && !done) {
+            for (int f = 0; f < SP_HIP_K_COUNT; ++f)
+                if (graphEvMask_ & (1 << f)) pendingK_[f] = true;
+            harvestK();
+        } else if (!graph && profMask_ && !done) {
+            harvestK();
+        }
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f) pendingK_[f] = false;
+        if (done) break;
+    }
+    res->seconds_loop =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    SPHIP_CHECK(hipMemcpyAsync(&c, dCtl_, sizeof(Ctl), hipMemcpyDeviceToHost, s_));
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+    if (c.status == kNonFinite)
+        throw NonFinite("GP/evaluate: non-finite objective, overflow, force norm or gradient");
+    res->iterations = c.iter;
+    res->evaluations = c.evals;
+    res->converged = c.status == kConverged;
+    res->stagnated = c.status == kStagnated;
+    res->rollback_iter = c.rollbackIter;
+    if (rows && maxRows > 0) {
+        const int nr = std::min(c.iter, maxRows);
+        std::vector<DevRow> dr(nr);
+        SPHIP_CHECK(hipMemcpy(dr.data(), dRows_, nr * sizeof(DevRow), hipMemcpyDeviceToHost));
+        int rateKHz = 100000;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+            rateKHz <= 0)
+            rateKHz = 100000;
+        for (int i = 0; i < nr; ++i) {
+            sp_hip_iter_metrics& m = rows[i];
+            const DevRow& r = dr[i];
+            m.iter = r.iter;
+            m.evaluations = r.evaluations;
+            m.hpwl = r.hpwl;
+            m.overflow = r.overflow;
+            m.lambda = r.lambda;
+            m.gamma = r.gamma;
+            m.step_size = r.step;
+            m.grad_norm_wl = r.gradWl;
+            m.grad_norm_den = r.gradDen;
+            m.eval_lambda = r.evalLambda;
+            m.eval_gamma = r.evalGamma;
+            m.delta_hpwl = r.dH;
+            m.lambda_mu = r.mu;
+            m.elapsed_ms = static_cast<double>(r.ticks - dr[0].ticks) / rateKHz;
+        }
+    }
+    if (c.finalBuf < 0 || c.finalBuf >= kNumBufs) throw std::runtime_error("device loop: no result");
+    double hp = 0;
+    scatterPos(table[c.finalBuf]);
+    res->overflow = static_cast<float>(deliveredMetrics(&hp));
+    res->hpwl = hp;
+    (void)n;
 }
 
 // ===========================================================================

@@ -47,6 +47,44 @@ struct DevDesign {
 
 constexpr int kLargeNet = 32;
 
+// ---------------------------------------------------------------------------
+// Device-resident loop state.  In graph mode every iteration is the same
+// fixed slot of kernels (scatter, finalize, transforms, WA, node pass, update);
+// the data-dependent decisions of EPlace::run / NesterovOptimizer::step are
+// taken by one thread in the last block of the node pass and recorded here.
+constexpr int kMaxSnaps = 4;      // tau-ladder snapshots (the ladder holds <= 3)
+constexpr int kNumBufs = 8 + kMaxSnaps;
+
+enum : int { kModeNone = 0, kModeStep = 1, kModeRedo = 2 };
+enum : int { kRunning = 0, kConverged = 1, kStagnated = 2, kIterLimit = 3, kNonFinite = 4 };
+
+struct DevRow {   // one metrics row (src/util/metrics.h:12-28)
+    int iter, evaluations;
+    double hpwl, overflow, lambda, gamma, step, gradWl, gradDen, evalLambda, evalGamma, dH, mu;
+    unsigned long long ticks;   // device real-time clock when the row was written
+};
+
+struct Ctl {
+    // indices into the state-buffer table: u, v, u_prev, v_prev, u_new, v_new, g, g_prev
+    int iu, iv, iup, ivp, iun, ivn, ig, igp;
+    int mode;        // update of this slot: none / step / redo
+    int rotate;      // apply the step rotation before the next node pass
+    int done, status;
+    int iter, tries, maxIter, maxBt, backtrack, useBB, stage, window;
+    long long evals;
+    int finalBuf, rollbackIter;
+    int snapWrite, snapSrc;   // table index to copy the evaluated point into (-1 none)
+    int nSnap, nFree;
+    int snapBuf[kMaxSnaps], snapIter[kMaxSnaps], freeSnap[kMaxSnaps];
+    float snapTau[kMaxSnaps];
+    double snapHpwl[kMaxSnaps];
+    double a, alpha, alpha0, m, momPrev;
+    float lambda, gamma;      // schedule values used by the next evaluation
+    float targetTau, stepX, minGain, bestTau, refTau;
+    int sinceImprove, havePrev;
+    double prevHpwl, effRef;
+};
+
 // Reduction workspace: per-block partials, counters and the final scalars.
 struct ReduceWs {
     double* partials = nullptr;   // [slots]
@@ -55,30 +93,37 @@ struct ReduceWs {
     int maxBlocks = 0;
 };
 
+// Every launcher takes an optional device loop state `ctl` (null in the
+// host-driven mode).  With ctl, kernels exit at once when ctl->done is set,
+// gamma/lambda come from ctl, and the node pass / update resolve their state
+// vectors through the table `bufs` (kNumBufs device pointers).
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
-                   unsigned long long* nodeQ, unsigned long long* fillQ, hipStream_t s);
+                   unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
+                   hipStream_t s);
 
 template <typename T>
 void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
                     const double* terminal, const double* base, double invQ, double binArea,
                     double movableAreaScaled, double rhoShift, T* rho, long long* dbgNode,
-                    long long* dbgFill, const ReduceWs& ws, hipStream_t s);
+                    long long* dbgFill, const ReduceWs& ws, Ctl* ctl, hipStream_t s);
 
 template <typename T>
 void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const ReduceWs& ws,
-                  hipStream_t s);
+                  const Ctl* ctl, hipStream_t s);
 
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, const ReduceWs& ws, hipStream_t s);
+                    T* dbgForce, T* dbgWl, const ReduceWs& ws, Ctl* ctl, T* const* bufs,
+                    DevRow* rows, hipStream_t s);
 
 // v' = A - alpha G; u' = v' + m (v' - B); clamp both to the core; write
 // vOut/uOut (either may be null) and scatter u' (or v' if uOut is null) into
-// the node coordinates.  G null means 0, B null means v'.
+// the node coordinates.  G null means 0, B null means v'.  With ctl, the
+// operands, alpha and m come from ctl->mode (step or redo) and the table.
 template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
-                  double m, T* vOut, T* uOut, hipStream_t s);
+                  double m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs, hipStream_t s);
 
 }  // namespace sphip

@@ -128,7 +128,8 @@ __device__ inline long long toFixed(double v, double s) { return __double2ll_rn(
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
-                                                    unsigned long long* fillQ) {
+                                                    unsigned long long* fillQ, const Ctl* ctl) {
+    if (ctl && ctl->done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.nDens) return;
     const int i = d.densIdx[t];
@@ -153,9 +154,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned lon
                                                      unsigned long long* fillQ,
                                                      const double* terminal, const double* base,
                                                      double invQ, double binArea, double denom,
-                                                     double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
-                                                     double* partials, unsigned* counter,
-                                                     double* scalars) {
+                                                     double rhoShift, T* rho, long long* dbgNode,
+                                                     long long* dbgFill, double* partials,
+                                                     unsigned* counter, double* scalars, Ctl* ctl) {
+    if (ctl && ctl->done) return;   // uniform: no block reaches the reduction
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = g.dim * g.dim;
     double v[1] = {0.0};
@@ -179,6 +181,20 @@ __global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned lon
     }
     gridReduce<1>(v, partials, counter, [&](double (&s)[1]) {
         scalars[0] = denom > 0.0 ? s[0] / denom : 0.0;
+        if (ctl) {
+            // the previous slot's update has finished: apply its step rotation
+            // (nesterov.cpp:109-118) before this slot's node pass
+            Ctl& c = *ctl;
+            if (c.rotate) {
+                const int ou = c.iu, ov = c.iv, oup = c.iup, ovp = c.ivp, oun = c.iun, ovn = c.ivn,
+                          og = c.ig, ogp = c.igp;
+                c.iup = ou; c.igp = og; c.ig = ogp;
+                c.iv = ovn; c.ivp = ov; c.ivn = ovp;
+                c.iu = oun; c.iun = oup;
+                c.rotate = 0;
+            }
+            c.snapWrite = -1;
+        }
     });
 }
 
@@ -190,6 +206,13 @@ __device__ inline float texp<float>(float v) { return expf(v); }
 template <>
 __device__ inline double texp<double>(double v) { return exp(v); }
 
+// wa_wirelength.cpp:95-97: gamma floored at 1e-4, inverted in the working precision.
+template <typename T>
+__device__ __host__ inline T invGammaOf(float gamma) {
+    const T g = static_cast<T>(gamma) > T(1e-4f) ? static_cast<T>(gamma) : T(1e-4f);
+    return T(1) / g;
+}
+
 template <typename T>
 __device__ inline T tmax() { return sizeof(T) == 4 ? T(3.402823466e+38f) : T(1.7976931348623157e308); }
 
@@ -199,7 +222,11 @@ __device__ inline T tmax() { return sizeof(T) == 4 ? T(3.402823466e+38f) : T(1.7
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_wa_small(DevDesign<T> d, T invG, int ignoreDeg,
                                                      double* partials, unsigned* counter,
-                                                     double* scalars) {
+                                                     double* scalars, const Ctl* ctl) {
+    if (ctl) {
+        if (ctl->done) return;
+        invG = invGammaOf<T>(ctl->gamma);
+    }
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (s < d.nSmallNets) {
@@ -266,7 +293,11 @@ __device__ inline T waveSum(T v) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int ignoreDeg,
                                                      double* partials, unsigned* counter,
-                                                     double* scalars) {
+                                                     double* scalars, const Ctl* ctl) {
+    if (ctl) {
+        if (ctl->done) return;
+        invG = invGammaOf<T>(ctl->gamma);
+    }
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     double v[2] = {0.0, 0.0};
@@ -314,6 +345,166 @@ __global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int
     });
 }
 
+// ------------------------------------------------------------------ (d) decisions
+// computeGamma (wa_wirelength.cpp:64-75) and lambdaMultiplier (lambda_schedule.h:22-25).
+__device__ inline float gammaOf(float tau, float binStep) {
+    float gamma = 8.0f * binStep;
+    if (tau > 1.0f) gamma *= 10.0f;
+    else if (tau < 0.1f) gamma *= 0.1f;
+    else gamma *= powf(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
+    return fmaxf(gamma, 1e-4f);
+}
+
+__device__ inline float lambdaMul(double dH, double ref) {
+    const double mu = pow(1.1, -dH / ref + 1.0);
+    return static_cast<float>(fmin(1.05, fmax(0.95, mu)));
+}
+
+// One thread, after the node pass of an evaluation has reduced its scalars
+// (sc = EvalScalars): the backtracking check (nesterov.cpp:146-169), the step
+// size (:88, :221-240), the momentum (:92-93), then - for a completed step -
+// the schedules (eplace.cpp:371-380), the metrics row (:384-399), the stop
+// test (:411-419), the tau-ladder snapshots and stagnation rollback
+// (:421-473) and the iteration limit.  The update kernel executes the chosen
+// mode; the next finalize applies the buffer rotation.
+template <typename T>
+__device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
+    const double tau = sc[0], wl = sc[1], hpwl = sc[2], sumWl = sc[3], sumDen = sc[4];
+    const double dx = sqrt(sc[5]), dg = sqrt(sc[6]), sy = sc[7], nonfin = sc[9];
+    c.snapWrite = -1;
+    if (!isfinite(tau) || !isfinite(hpwl) || !isfinite(wl) || !isfinite(sumWl) ||
+        !isfinite(sumDen) || nonfin > 0.0) {
+        c.done = 1;
+        c.status = kNonFinite;
+        c.mode = kModeNone;
+        return;
+    }
+    ++c.evals;
+    if (c.backtrack && c.iter > 0 && c.tries < c.maxBt - 1 && dx > 0.0 && dg > 0.0 &&
+        isfinite(dg)) {
+        const double lip = dx / dg;
+        if (!(lip > 0.95 * c.alpha)) {
+            c.alpha = static_cast<double>(static_cast<T>(lip));
+            c.m = c.momPrev;
+            c.mode = kModeRedo;
+            ++c.tries;
+            return;
+        }
+    }
+    c.tries = 0;
+    if (c.iter == 0) {
+        c.alpha = c.alpha0;
+    } else if (dx > 0.0 && dg > 0.0) {
+        double next = dx / dg;
+        if (c.useBB) {
+            const double yy = dg * dg;
+            if (yy > 0.0) {
+                const double bb = sy / yy;
+                if (bb > 0.0 && isfinite(bb)) next = bb;
+            }
+        }
+        c.alpha = static_cast<double>(static_cast<T>(next));
+    }
+    const T aT = static_cast<T>(c.a);
+    const T aNext = T(0.5) * (T(1) + sqrt(T(4) * aT * aT + T(1)));
+    const T mom = (aT - T(1)) / aNext;
+    c.m = static_cast<double>(mom);
+    c.momPrev = c.m;
+    c.a = static_cast<double>(aNext);
+    c.mode = kModeStep;
+    c.rotate = 1;
+    const int lastEval = c.iu;   // the point this evaluation used
+
+    const float evalLambda = c.lambda, evalGamma = c.gamma;
+    const float ftau = static_cast<float>(tau);
+    const double dH = c.havePrev ? hpwl - c.prevHpwl : 0.0;
+    const float mu = c.havePrev ? lambdaMul(dH, c.effRef) : 1.f;
+    if (c.havePrev) c.lambda *= mu;
+    c.prevHpwl = hpwl;
+    c.havePrev = 1;
+    c.gamma = gammaOf(ftau, c.stepX);
+    if (!isfinite(c.lambda) || !isfinite(c.gamma)) {
+        c.done = 1;
+        c.status = kNonFinite;
+        c.mode = kModeNone;
+        return;
+    }
+    const int it = c.iter;
+    if (rows) {
+        DevRow& r = rows[it];
+        r.iter = it;
+        r.evaluations = static_cast<int>(c.evals);
+        r.hpwl = hpwl;
+        r.overflow = ftau;
+        r.lambda = c.lambda;
+        r.gamma = c.gamma;
+        r.step = c.alpha;
+        r.gradWl = static_cast<float>(sumWl);
+        r.gradDen = static_cast<float>(sumDen);
+        r.evalLambda = evalLambda;
+        r.evalGamma = evalGamma;
+        r.dH = dH;
+        r.mu = mu;
+        r.ticks = __builtin_amdgcn_s_memrealtime();
+    }
+    c.iter = it + 1;
+    if (c.stage != 1 && ftau < c.targetTau) {   // deliver the look-ahead point
+        c.done = 1;
+        c.status = kConverged;
+        c.finalBuf = lastEval;
+        c.mode = kModeNone;
+        return;
+    }
+    if (c.stage != 1 && c.window > 0) {
+        if (ftau < c.refTau * (1.f - c.minGain)) {
+            c.refTau = ftau;
+            c.sinceImprove = 0;
+        } else {
+            ++c.sinceImprove;
+        }
+        c.bestTau = fminf(c.bestTau, ftau);
+        const float tol = c.bestTau * (1.f + c.minGain);
+        auto dropFront = [&]() {
+            c.freeSnap[c.nFree++] = c.snapBuf[0];
+            for (int s = 1; s < c.nSnap; ++s) {
+                c.snapBuf[s - 1] = c.snapBuf[s];
+                c.snapTau[s - 1] = c.snapTau[s];
+                c.snapHpwl[s - 1] = c.snapHpwl[s];
+                c.snapIter[s - 1] = c.snapIter[s];
+            }
+            --c.nSnap;
+        };
+        while (c.nSnap > 0 && c.snapTau[0] > tol) dropFront();
+        if (c.nSnap == 0 || ftau * (1.f + 0.5f * c.minGain) <= c.snapTau[c.nSnap - 1]) {
+            if (c.nSnap == kMaxSnaps) dropFront();
+            const int b = c.freeSnap[--c.nFree];
+            c.snapBuf[c.nSnap] = b;
+            c.snapTau[c.nSnap] = ftau;
+            c.snapHpwl[c.nSnap] = hpwl;
+            c.snapIter[c.nSnap] = it;
+            ++c.nSnap;
+            c.snapWrite = b;
+            c.snapSrc = lastEval;
+        }
+        if (c.sinceImprove >= c.window) {
+            int pick = 0;
+            for (int s = 1; s < c.nSnap; ++s)
+                if (c.snapHpwl[s] < c.snapHpwl[pick]) pick = s;
+            c.done = 1;
+            c.status = kStagnated;
+            c.finalBuf = c.snapBuf[pick];
+            c.rollbackIter = c.snapIter[pick];
+            // the update still runs this slot so that a snapshot taken now is copied
+            return;
+        }
+    }
+    if (c.iter >= c.maxIter) {
+        c.done = 1;
+        c.status = kIterLimit;
+        c.finalBuf = c.ivn;   // v after this step (pos = v, nesterov.cpp:121)
+    }
+}
+
 // ------------------------------------------------------------------ (c)+(d) node pass
 // For each optimised node k (node i = optIdx[k]):
 //  * WA gradient over the node's pins (wa_wirelength.cpp:183-225),
@@ -328,7 +519,19 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
                                                       const T* ey, T invG, float lambda,
                                                       const T* u, const T* up, const T* gp, T* grad,
                                                       T* dbgForce, T* dbgWl, double* partials,
-                                                      unsigned* counter, double* scalars) {
+                                                      unsigned* counter, double* scalars, Ctl* ctl,
+                                                      T* const* bufs, DevRow* rows) {
+    if (ctl) {
+        // every block reads the state before the last block decides (it is
+        // the last one to arrive at the reduction), so this read is uniform
+        if (ctl->done) return;
+        lambda = ctl->lambda;
+        invG = invGammaOf<T>(ctl->gamma);
+        u = bufs[ctl->iu];
+        up = bufs[ctl->iup];
+        gp = bufs[ctl->igp];
+        grad = bufs[ctl->ig];
+    }
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.nOpt;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -398,6 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
     }
     gridReduce<7>(v, partials, counter, [&](double (&t)[7]) {
         for (int j = 0; j < 7; ++j) scalars[3 + j] = t[j];
+        if (ctl) decideStep<T>(*ctl, scalars, rows);
     });
 }
 
@@ -406,9 +610,29 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
 // :156-165) with EPlace's clamp (eplace.cpp:117-133) and scatter (:97-105).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
-                                                   const T* B, T alpha, T m, T* vOut, T* uOut) {
+                                                   const T* B, T alpha, T m, T* vOut, T* uOut,
+                                                   const Ctl* ctl, T* const* bufs) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.nOpt;
+    if (ctl) {
+        const int mode = ctl->mode;
+        if (mode == kModeNone) return;
+        if (k < n && ctl->snapWrite >= 0) {   // tau-ladder snapshot of the evaluated point
+            const T* src = bufs[ctl->snapSrc];
+            T* dst = bufs[ctl->snapWrite];
+            dst[k] = src[k];
+            dst[n + k] = src[n + k];
+        }
+        alpha = static_cast<T>(ctl->alpha);
+        m = static_cast<T>(ctl->m);
+        if (mode == kModeStep) {
+            A = bufs[ctl->iu]; G = bufs[ctl->ig]; B = bufs[ctl->iv];
+            vOut = bufs[ctl->ivn]; uOut = bufs[ctl->iun];
+        } else {   // redo the previous step from u_prev (nesterov.cpp:156-165)
+            A = bufs[ctl->iup]; G = bufs[ctl->igp]; B = bufs[ctl->ivp];
+            vOut = bufs[ctl->iv]; uOut = bufs[ctl->iu];
+        }
+    }
     if (k >= n) return;
     const int i = d.optIdx[k];
     const T hw = T(0.5) * d.w[i], hh = T(0.5) * d.h[i];
@@ -439,9 +663,10 @@ long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
 
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
-                   unsigned long long* nodeQ, unsigned long long* fillQ, hipStream_t s) {
+                   unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
+                   hipStream_t s) {
     if (d.nDens == 0) return;
-    k_scatter<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, qscale, nodeQ, fillQ);
+    k_scatter<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, qscale, nodeQ, fillQ, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -449,28 +674,26 @@ template <typename T>
 void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
                     const double* terminal, const double* base, double invQ, double binArea,
                     double movableAreaScaled, double rhoShift, T* rho, long long* dbgNode,
-                    long long* dbgFill, const ReduceWs& ws, hipStream_t s) {
+                    long long* dbgFill, const ReduceWs& ws, Ctl* ctl, hipStream_t s) {
     const long nb = static_cast<long>(g.dim) * g.dim;
     k_finalize<T><<<gridFor(nb), kBlock, 0, s>>>(g, nodeQ, fillQ, terminal, base, invQ, binArea,
                                                  movableAreaScaled, rhoShift, rho, dbgNode, dbgFill,
-                                                 ws.partials, ws.counters + 0, ws.scalars);
+                                                 ws.partials, ws.counters + 0, ws.scalars, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const ReduceWs& ws,
-                  hipStream_t s) {
-    // wa_wirelength.cpp:95-97: gamma floored at 1e-4, then inverted in the working precision
-    const T g = static_cast<T>(gamma) > T(1e-4f) ? static_cast<T>(gamma) : T(1e-4f);
-    const T invG = T(1) / g;
+                  const Ctl* ctl, hipStream_t s) {
+    const T invG = invGammaOf<T>(gamma);
     const long slot = static_cast<long>(ws.maxBlocks) * 8;
     k_wa_small<T><<<gridFor(d.nSmallNets > 0 ? d.nSmallNets : 1), kBlock, 0, s>>>(
-        d, invG, ignoreDeg, ws.partials + slot, ws.counters + 1, ws.scalars);
+        d, invG, ignoreDeg, ws.partials + slot, ws.counters + 1, ws.scalars, ctl);
     SPHIP_CHECK(hipGetLastError());
     if (d.nLargeNets > 0) {
         k_wa_large<T><<<(d.nLargeNets + 3) / 4, kBlock, 0, s>>>(d, invG, ignoreDeg,
                                                                  ws.partials + 2 * slot,
-                                                                 ws.counters + 2, ws.scalars);
+                                                                 ws.counters + 2, ws.scalars, ctl);
         SPHIP_CHECK(hipGetLastError());
     }
 }
@@ -478,36 +701,39 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const Reduc
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, const ReduceWs& ws, hipStream_t s) {
-    const T gm = static_cast<T>(gamma) > T(1e-4f) ? static_cast<T>(gamma) : T(1e-4f);
-    const T invG = T(1) / gm;
+                    T* dbgForce, T* dbgWl, const ReduceWs& ws, Ctl* ctl, T* const* bufs,
+                    DevRow* rows, hipStream_t s) {
+    const T invG = invGammaOf<T>(gamma);
     const long slot = static_cast<long>(ws.maxBlocks) * 8;
     k_node_grad<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, g, ex, ey, invG, lambda, u, up, gp, grad,
                                                       dbgForce, dbgWl, ws.partials + 3 * slot,
-                                                      ws.counters + 3, ws.scalars);
+                                                      ws.counters + 3, ws.scalars, ctl, bufs, rows);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
-                  double m, T* vOut, T* uOut, hipStream_t s) {
+                  double m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs, hipStream_t s) {
     k_update<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, A, G, B, static_cast<T>(alpha),
-                                                   static_cast<T>(m), vOut, uOut);
+                                                   static_cast<T>(m), vOut, uOut, ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
 }
 
 #define SPHIP_INST(T)                                                                            \
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
-                                   unsigned long long*, unsigned long long*, hipStream_t);       \
+                                   unsigned long long*, unsigned long long*, const Ctl*,         \
+                                   hipStream_t);                                                 \
     template void launchFinalize<T>(const GridGeom<T>&, unsigned long long*, unsigned long long*, \
                                     const double*, const double*, double, double, double, double, \
-                                    T*, long long*, long long*, const ReduceWs&, hipStream_t);   \
-    template void launchWaNets<T>(const DevDesign<T>&, float, int, const ReduceWs&, hipStream_t); \
+                                    T*, long long*, long long*, const ReduceWs&, Ctl*,           \
+                                    hipStream_t);                                                \
+    template void launchWaNets<T>(const DevDesign<T>&, float, int, const ReduceWs&, const Ctl*,  \
+                                  hipStream_t);                                                  \
     template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*, const T*, \
                                     float, float, const T*, const T*, const T*, T*, T*, T*,      \
-                                    const ReduceWs&, hipStream_t);                               \
+                                    const ReduceWs&, Ctl*, T* const*, DevRow*, hipStream_t);     \
     template void launchUpdate<T>(const DevDesign<T>&, const T*, const T*, const T*, double,     \
-                                  double, T*, T*, hipStream_t);
+                                  double, T*, T*, const Ctl*, T* const*, hipStream_t);
 SPHIP_INST(float)
 SPHIP_INST(double)
 

@@ -27,8 +27,9 @@ public:
     PoissonPlan& operator=(const PoissonPlan&) = delete;
 
     // rho() must hold the padded density (zeros in the padding).  Writes
-    // fieldX()/fieldY() and, if withPhi, phi().  Enqueued on the stream.
-    void solve(bool withPhi);
+    // fieldX()/fieldY() and, if withPhi, phi().  Enqueued on the stream; the
+    // kernels return at once while *skip (a device flag, may be null) is set.
+    void solve(bool withPhi, const int* skip = nullptr);
 
     int nx() const { return nx_; }
     int ny() const { return ny_; }

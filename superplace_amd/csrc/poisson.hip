@@ -54,12 +54,24 @@ struct SpectralEpi {
 // for f64 (both measured on gfx950).
 template <typename T, int TM, int TN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, int K, int lda,
-                                              int ldb, int ldc, SpectralEpi<T> epi) {
+                                              int ldb, int ldc, SpectralEpi<T> epi,
+                                              const int* skip) {
+    if (skip && *skip) return;   // device loop finished: uniform early exit
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int BK = sizeof(T) == 4 ? 32 : 16;
-    constexpr int PAD = sizeof(T) == 4 ? 16 : 8;
-    __shared__ T As[BK][BM + PAD];
-    __shared__ T Bs[BK][BN + PAD];
+    // A is kept row-major in LDS (k fastest).  The MFMA read has 16 lanes on
+    // 16 rows at one k and the next 16 lanes at k+1: a row stride of 2 mod 32
+    // banks (FP32) or an odd number of doubles (FP64) makes it conflict-free.
+    constexpr int LDA_S = sizeof(T) == 4 ? BK + 2 : BK + 1;
+    // B is k-major with rows 16 banks apart: lanes read consecutive n.
+    constexpr int LDB_S = sizeof(T) == 4 ? BN + 16 : BN + 8;
+    constexpr int VEC = 16 / sizeof(T);                 // elements per 16-byte load
+    constexpr int A_LD = BM * BK / (256 * VEC);          // 16-byte loads per thread
+    constexpr int B_LD = BK * BN / (256 * VEC);
+    static_assert(A_LD >= 1 && B_LD >= 1, "tile too small for 256 threads");
+    __shared__ __attribute__((aligned(16))) T As[2][BM][LDA_S];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BK][LDB_S];
+    using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
 
     const T* __restrict__ A = batch.A[blockIdx.z];
     const T* __restrict__ B = batch.B[blockIdx.z];
@@ -82,26 +94,51 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
             acc64[i][j] = vec4<double>{0, 0, 0, 0};
         }
 
+    V ra[A_LD], rb[B_LD];
+    auto loadTiles = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < A_LD; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BK / VEC), c = (e % (BK / VEC)) * VEC;
+            ra[q] = *reinterpret_cast<const V*>(A + static_cast<long>(m0 + r) * lda + k0 + c);
+        }
+#pragma unroll
+        for (int q = 0; q < B_LD; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BN / VEC), c = (e % (BN / VEC)) * VEC;
+            rb[q] = *reinterpret_cast<const V*>(B + static_cast<long>(k0 + r) * ldb + n0 + c);
+        }
+    };
+    auto storeTiles = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < A_LD; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BK / VEC), c = (e % (BK / VEC)) * VEC;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) As[buf][r][c + v] = ra[q][v];
+        }
+#pragma unroll
+        for (int q = 0; q < B_LD; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BN / VEC), c = (e % (BN / VEC)) * VEC;
+            *reinterpret_cast<V*>(&Bs[buf][r][c]) = rb[q];
+        }
+    };
+
+    loadTiles(0);
+    storeTiles(0);
+    __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < K; k0 += BK) {
-        // A tile: consecutive threads walk k inside a row (coalesced), stored k-major.
-#pragma unroll
-        for (int e = tid; e < BM * BK; e += 256) {
-            const int r = e / BK, c = e % BK;
-            As[c][r] = A[static_cast<long>(m0 + r) * lda + k0 + c];
-        }
-#pragma unroll
-        for (int e = tid; e < BK * BN; e += 256) {
-            const int r = e / BN, c = e % BN;
-            Bs[r][c] = B[static_cast<long>(k0 + r) * ldb + n0 + c];
-        }
-        __syncthreads();
+        const bool more = k0 + BK < K;
+        if (more) loadTiles(k0 + BK);   // in flight while the MFMAs run
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = As[kk + lk][wm * 16 * TM + i * 16 + lrow];
+            for (int i = 0; i < TM; ++i) a[i] = As[buf][wm * 16 * TM + i * 16 + lrow][kk + lk];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bs[kk + lk][wn * 16 * TN + j * 16 + lrow];
+            for (int j = 0; j < TN; ++j) b[j] = Bs[buf][kk + lk][wn * 16 * TN + j * 16 + lrow];
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -117,7 +154,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
                     acc[i][j] = vec4<T>{0, 0, 0, 0};
                 }
         }
+        if (more) storeTiles(buf ^ 1);   // the other buffer: no reader is using it
         __syncthreads();
+        buf ^= 1;
     }
 
 #pragma unroll
@@ -151,16 +190,16 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
 
 template <typename T, int EPI>
 void gemmDispatch(int batch, const GemmBatch<T>& gb, int M, int N, int K, int lda, int ldb,
-                  int ldc, const SpectralEpi<T>& epi, hipStream_t s) {
+                  int ldc, const SpectralEpi<T>& epi, hipStream_t s, const int* skip = nullptr) {
     if (M % 64 || N % 64 || K % 64) throw std::runtime_error("gemm: dims must be multiples of 64");
     if (batch < 1 || batch > 4) throw std::runtime_error("gemm: batch must be 1..4");
     const long tiles64 = static_cast<long>(M / 64) * (N / 64) * batch;
     if (tiles64 >= 256) {
         dim3 grid(N / 64, M / 64, batch);
-        k_gemm<T, 2, 2, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi);
+        k_gemm<T, 2, 2, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi, skip);
     } else {
         dim3 grid(N / 32, M / 32, batch);
-        k_gemm<T, 1, 1, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi);
+        k_gemm<T, 1, 1, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi, skip);
     }
     SPHIP_CHECK(hipGetLastError());
 }
@@ -272,32 +311,32 @@ double PoissonPlan<T>::flops(bool withPhi) const {
 }
 
 template <typename T>
-void PoissonPlan<T>::solve(bool withPhi) {
+void PoissonPlan<T>::solve(bool withPhi, const int* skip) {
     const int M = mx_, N = my_;
     SpectralEpi<T> none{};
     {   // T0 = rho * CyT
         GemmBatch<T> gb{};
         gb.A[0] = rho_; gb.B[0] = cyT_; gb.C[0] = t0_;
-        gemmDispatch<T, kEpiNone>(1, gb, M, N, N, N, N, N, none, stream_);
+        gemmDispatch<T, kEpiNone>(1, gb, M, N, N, N, N, N, none, stream_, skip);
     }
     {   // P = spectral(Cx * T0)
         GemmBatch<T> gb{};
         gb.A[0] = cx_; gb.B[0] = t0_; gb.C[0] = p_;
         SpectralEpi<T> epi{u_, v_, nx_, ny_, invNorm_};
-        gemmDispatch<T, kEpiSpectral>(1, gb, M, N, M, M, N, N, epi, stream_);
+        gemmDispatch<T, kEpiSpectral>(1, gb, M, N, M, M, N, N, epi, stream_, skip);
     }
     {   // T1 = P * AyT, T2 = P * Bsv
         GemmBatch<T> gb{};
         gb.A[0] = p_; gb.B[0] = ayT_; gb.C[0] = t1_;
         gb.A[1] = p_; gb.B[1] = bsv_; gb.C[1] = t2_;
-        gemmDispatch<T, kEpiNone>(2, gb, M, N, N, N, N, N, none, stream_);
+        gemmDispatch<T, kEpiNone>(2, gb, M, N, N, N, N, N, none, stream_, skip);
     }
     {   // xi_x = Sxu * T1, xi_y = Ax * T2, phi = Ax * T1
         GemmBatch<T> gb{};
         gb.A[0] = sxu_; gb.B[0] = t1_; gb.C[0] = ex_;
         gb.A[1] = ax_;  gb.B[1] = t2_; gb.C[1] = ey_;
         gb.A[2] = ax_;  gb.B[2] = t1_; gb.C[2] = phi_;
-        gemmDispatch<T, kEpiNone>(withPhi ? 3 : 2, gb, M, N, M, M, N, N, none, stream_);
+        gemmDispatch<T, kEpiNone>(withPhi ? 3 : 2, gb, M, N, M, M, N, N, none, stream_, skip);
     }
 }
 

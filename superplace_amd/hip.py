@@ -67,7 +67,8 @@ class SpHipGpConfig(C.Structure):
                 ("delta_hpwl_relative", C.c_double), ("use_bb", C.c_int32),
                 ("gp_backtrack", C.c_int32), ("filler_only_iters", C.c_int32),
                 ("gp_stagnation_window", C.c_int32), ("gp_stagnation_min_gain", C.c_float),
-                ("stage", C.c_int32), ("precision", C.c_int32), ("max_backtrack", C.c_int32)]
+                ("stage", C.c_int32), ("precision", C.c_int32), ("max_backtrack", C.c_int32),
+                ("loop_mode", C.c_int32), ("graph_slots", C.c_int32)]
 
 
 class SpHipIterMetrics(C.Structure):
@@ -257,6 +258,8 @@ class GpConfig:
     stage: int = 0
     precision: int = 32
     max_backtrack: int = 10
+    loop_mode: int = 2        # 0 host decisions, 1 device decisions, 2 device decisions + hipGraph
+    graph_slots: int = 8      # iterations per graph launch in loop_mode 2
 
     def struct(self) -> SpHipGpConfig:
         return SpHipGpConfig(**dataclasses.asdict(self))

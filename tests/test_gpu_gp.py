@@ -170,3 +170,36 @@ def test_nonfinite_input_rejected():
     d.node_x[3] = np.nan
     with pytest.raises(hip.NonFinite):
         hip.GpEngine(d, hip.GpConfig())
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_loop_modes_agree(mode):
+    """Host decisions (0), device decisions launched slot by slot (1) and the
+    hipGraph-captured loop (2) take the same decisions: same iteration count
+    and evaluation count within 2, final HPWL within 0.5% (the schedules'
+    powf may differ by an ulp between host and device)."""
+    d = D.with_fillers(D.fixture_eplace(3000, 12))
+    ref, rref = hip.GpEngine(d, hip.GpConfig(loop_mode=0)).run()
+    res, rows = hip.GpEngine(d, hip.GpConfig(loop_mode=mode, graph_slots=5)).run()
+    print(mode, res["iterations"], res["evaluations"], res["hpwl"], "| host", ref["iterations"],
+          ref["evaluations"], ref["hpwl"], flush=True)
+    assert res["converged"] and res["overflow"] < 0.10
+    assert abs(res["iterations"] - ref["iterations"]) <= 2
+    assert abs(res["evaluations"] - ref["evaluations"]) <= 3
+    assert abs(res["hpwl"] / ref["hpwl"] - 1) < 0.005
+    # the first rows agree closely (same arithmetic, same decisions)
+    for a, b in zip(rows[:10], rref[:10]):
+        assert a["hpwl"] == pytest.approx(b["hpwl"], rel=1e-4)
+        assert a["overflow"] == pytest.approx(b["overflow"], rel=1e-4, abs=1e-6)
+
+
+def test_graph_loop_backtracks_and_hits_iteration_limit():
+    d = D.with_fillers(D.fixture_eplace(2000, 8))
+    eng = hip.GpEngine(d, hip.GpConfig(loop_mode=2, gp_max_iter=37, graph_slots=8))
+    res, rows = eng.run()
+    assert res["iterations"] == 37 and not res["converged"] and len(rows) == 37
+    assert res["evaluations"] >= 37
+    x, y = eng.positions()
+    assert res["hpwl"] == pytest.approx(oracle.hpwl(d, 64, x, y), rel=1e-6)
+    for a, b in zip(rows[:-1], rows[1:]):
+        assert b["eval_lambda"] == a["lambda_"] and b["eval_gamma"] == a["gamma"]
