@@ -618,14 +618,16 @@ void Engine<T>::build(const sp_hip_design& d) {
     SPHIP_CHECK(hipMemsetAsync(nodeQ_, 0, nb_ * 8, s_));
     SPHIP_CHECK(hipMemsetAsync(fillQ_, 0, nb_ * 8, s_));
 
-    // reduction workspace: 4 regions of maxBlocks x 8 doubles
-    const long maxItems = std::max<long>({static_cast<long>(nb_), n_, d.num_nets,
-                                          dd_.nDens, 1});
-    ws_.maxBlocks = static_cast<int>((maxItems + kBlock - 1) / kBlock) + 1;
+    // reduction workspace: 4 regions of maxBlocks x 8 doubles (one block
+    // partial per region, see ReduceWs)
+    const long maxBlocks = std::max<long>({(static_cast<long>(nb_) + kBlock - 1) / kBlock,
+                                           (dd_.nSmallNets + kBlock - 1L) / kBlock,
+                                           (dd_.nLargeNets + 3L) / 4,
+                                           (n_ + kBlock - 1L) / kBlock, 1L});
+    ws_.maxBlocks = static_cast<int>(maxBlocks) + 1;
     ws_.partials = static_cast<double*>(own(dalloc<double>(4L * ws_.maxBlocks * 8)));
-    ws_.counters = static_cast<unsigned*>(own(dalloc<unsigned>(8)));
     ws_.scalars = static_cast<double*>(own(dalloc<double>(kNumScalars)));
-    SPHIP_CHECK(hipMemsetAsync(ws_.counters, 0, 8 * sizeof(unsigned), s_));
+    SPHIP_CHECK(hipMemsetAsync(ws_.partials, 0, 4L * ws_.maxBlocks * 8 * sizeof(double), s_));
     SPHIP_CHECK(hipMemsetAsync(ws_.scalars, 0, kNumScalars * sizeof(double), s_));
     SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hScalars_), kNumScalars * sizeof(double),
                               hipHostMallocDefault));
@@ -665,7 +667,7 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
     endK(SP_HIP_K_SCATTER);
     beginK(SP_HIP_K_FINALIZE);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
+                      binArea_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
     endK(SP_HIP_K_FINALIZE);
     beginK(SP_HIP_K_POISSON);
     poisson_->solve(false);
@@ -675,7 +677,8 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
     endK(SP_HIP_K_WA_NETS);
     beginK(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), gamma, lambda, u, up,
-                      gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, nullptr, s_);
+                      gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
+    launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
     endK(SP_HIP_K_NODE_GRAD);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
@@ -699,8 +702,9 @@ template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
+                      binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
     launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
+    launchReduce<T>(ws_, movableAreaScaled_, false, nullptr, nullptr, s_);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
@@ -983,8 +987,7 @@ void Engine<T>::launchSlot(bool events) {
     e(SP_HIP_K_SCATTER);
     b(SP_HIP_K_FINALIZE);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, movableAreaScaled_, rhoShift_, poisson_->rho(), nullptr, nullptr,
-                      ws_, dCtl_, s_);
+                      binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, dCtl_, s_);
     e(SP_HIP_K_FINALIZE);
     b(SP_HIP_K_POISSON);
     poisson_->solve(false, skip);
@@ -994,7 +997,8 @@ void Engine<T>::launchSlot(bool events) {
     e(SP_HIP_K_WA_NETS);
     b(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), 0.f, 0.f, nullptr,
-                      nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, dRows_, s_);
+                      nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, s_);
+    launchReduce<T>(ws_, movableAreaScaled_, true, dCtl_, dRows_, s_);
     e(SP_HIP_K_NODE_GRAD);
     b(SP_HIP_K_UPDATE);
     launchUpdate<T>(dd_, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, dCtl_, dBufs_, s_);
@@ -1010,7 +1014,7 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
     for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
     Ctl c{};
-    c.iu = 0; c.iv = 1; c.iup = 2; c.ivp = 3; c.iun = 4; c.ivn = 5; c.ig = 6; c.igp = 7;
+    for (int r = 0; r < B_ROLES; ++r) c.cur[r] = c.nxt[r] = r;   // table order = role order
     c.mode = kModeNone;
     c.maxIter = maxIter;
     c.maxBt = maxBt;

@@ -1,6 +1,6 @@
 // Device-side data of one placement instance and launchers of the
 // per-evaluation kernels (density scatter, rho/tau, WA per net, fused
-// gather + WA node pass + gradient, Nesterov update).
+// gather + WA node pass + gradient, reduction + decisions, Nesterov update).
 #pragma once
 
 #include "common.h"
@@ -47,13 +47,30 @@ struct DevDesign {
 
 constexpr int kLargeNet = 32;
 
+// FP64 reductions: every reducing kernel writes one partial per block (plain
+// stores, no atomics, no fences); k_reduce sums each region in block order
+// after the node pass.  Regions and their values per block:
+//   0 finalize (tau numerator), 1 WA small nets (WL, HPWL),
+//   2 WA large nets (WL, HPWL), 3 node pass (7 sums).
+struct ReduceWs {
+    double* partials = nullptr;   // 4 regions of maxBlocks * 8 doubles
+    double* scalars = nullptr;    // EvalScalars on the device
+    int maxBlocks = 0;
+    int blocks[4] = {0, 0, 0, 0}; // blocks each region's kernel launches
+    __host__ __device__ double* region(int r) const {
+        return partials + static_cast<long>(r) * maxBlocks * 8;
+    }
+};
+
 // ---------------------------------------------------------------------------
-// Device-resident loop state.  In graph mode every iteration is the same
-// fixed slot of kernels (scatter, finalize, transforms, WA, node pass, update);
-// the data-dependent decisions of EPlace::run / NesterovOptimizer::step are
-// taken by one thread in the last block of the node pass and recorded here.
+// Device-resident loop state.  Every iteration is the same fixed slot of
+// kernels (scatter, finalize, transforms, WA, node pass, reduce, update); the
+// data-dependent decisions of EPlace::run / NesterovOptimizer::step are taken
+// by one thread of k_reduce and recorded here.
 constexpr int kMaxSnaps = 4;      // tau-ladder snapshots (the ladder holds <= 3)
 constexpr int kNumBufs = 8 + kMaxSnaps;
+// roles in the state-buffer table (then kMaxSnaps snapshot buffers)
+enum : int { B_U = 0, B_V, B_UP, B_VP, B_UN, B_VN, B_G, B_GP, B_ROLES };
 
 enum : int { kModeNone = 0, kModeStep = 1, kModeRedo = 2 };
 enum : int { kRunning = 0, kConverged = 1, kStagnated = 2, kIterLimit = 3, kNonFinite = 4 };
@@ -65,10 +82,10 @@ struct DevRow {   // one metrics row (src/util/metrics.h:12-28)
 };
 
 struct Ctl {
-    // indices into the state-buffer table: u, v, u_prev, v_prev, u_new, v_new, g, g_prev
-    int iu, iv, iup, ivp, iun, ivn, ig, igp;
+    // cur: table indices the update of this slot uses; nxt: the indices the
+    // next evaluation uses (after this slot's step rotation).
+    int cur[B_ROLES], nxt[B_ROLES];
     int mode;        // update of this slot: none / step / redo
-    int rotate;      // apply the step rotation before the next node pass
     int done, status;
     int iter, tries, maxIter, maxBt, backtrack, useBB, stage, window;
     long long evals;
@@ -85,14 +102,6 @@ struct Ctl {
     double prevHpwl, effRef;
 };
 
-// Reduction workspace: per-block partials, counters and the final scalars.
-struct ReduceWs {
-    double* partials = nullptr;   // [slots]
-    unsigned* counters = nullptr; // [8]
-    double* scalars = nullptr;    // EvalScalars on the device
-    int maxBlocks = 0;
-};
-
 // Every launcher takes an optional device loop state `ctl` (null in the
 // host-driven mode).  With ctl, kernels exit at once when ctl->done is set,
 // gamma/lambda come from ctl, and the node pass / update resolve their state
@@ -105,18 +114,25 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
 template <typename T>
 void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
                     const double* terminal, const double* base, double invQ, double binArea,
-                    double movableAreaScaled, double rhoShift, T* rho, long long* dbgNode,
-                    long long* dbgFill, const ReduceWs& ws, Ctl* ctl, hipStream_t s);
+                    double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
+                    ReduceWs& ws, const Ctl* ctl, hipStream_t s);
 
 template <typename T>
-void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const ReduceWs& ws,
+void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& ws,
                   const Ctl* ctl, hipStream_t s);
 
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, const ReduceWs& ws, Ctl* ctl, T* const* bufs,
-                    DevRow* rows, hipStream_t s);
+                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
+                    hipStream_t s);
+
+// Sums the regions written by this evaluation into EvalScalars (tau is the
+// finalize sum over `denom`); withNode = false for a density + HPWL pass only.
+// With ctl, one thread then takes the step decisions and writes `rows`.
+template <typename T>
+void launchReduce(ReduceWs& ws, double denom, bool withNode, Ctl* ctl, DevRow* rows,
+                  hipStream_t s);
 
 // v' = A - alpha G; u' = v' + m (v' - B); clamp both to the core; write
 // vOut/uOut (either may be null) and scatter u' (or v' if uOut is null) into

@@ -2,9 +2,10 @@
 //
 // Determinism: the density scatter accumulates 64-bit fixed-point integers
 // with atomics (integer addition is associative), and every FP64 reduction is
-// a fixed-shape tree: per-block shuffle trees, per-block partials, and the
-// last block to arrive sums the partials in block order.  Results are
-// therefore bitwise reproducible run to run.
+// a fixed-shape tree: a shuffle tree per block, one partial per block written
+// with a plain store, and k_reduce summing the partials in block order.  No
+// reduction uses atomics or fences, so results are bitwise reproducible and
+// no block has to publish its writes across the chip's L2 caches mid-kernel.
 #include <cmath>
 
 #include "gp_kernels.h"
@@ -35,38 +36,24 @@ __device__ inline void blockSum(double (&v)[NV]) {
     __syncthreads();
 }
 
-__device__ inline double loadRelaxed(const double* p) {
-    const unsigned long long bits = __hip_atomic_load(
-        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double(static_cast<long long>(bits));
-}
-
-// Block partials, then the last block sums them in block order and calls
-// fin(total) on thread 0.  Every thread of every block must call it.
-template <int NV, typename Fin>
-__device__ inline void gridReduce(double (&v)[NV], double* partials, unsigned* counter, Fin fin) {
+// One partial per block: partials[blockIdx.x * NV + j].
+template <int NV>
+__device__ inline void blockPartial(double (&v)[NV], double* partials) {
     blockSum<NV>(v);
-    __shared__ int amLast;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int j = 0; j < NV; ++j) partials[static_cast<long>(blockIdx.x) * NV + j] = v[j];
-        const unsigned prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amLast) return;
-    double acc[NV];
+}
+
+// Sum of `nb` block partials of NV values (single block, all threads).
+template <int NV>
+__device__ inline void sumRegion(const double* partials, int nb, double (&out)[NV]) {
 #pragma unroll
-    for (int j = 0; j < NV; ++j) acc[j] = 0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x)
+    for (int j = 0; j < NV; ++j) out[j] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) acc[j] += loadRelaxed(&partials[static_cast<long>(b) * NV + j]);
-    blockSum<NV>(acc);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+        for (int j = 0; j < NV; ++j) out[j] += partials[static_cast<long>(b) * NV + j];
+    blockSum<NV>(out);
 }
 
 // ------------------------------------------------------------------ footprint
@@ -147,17 +134,17 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
 
 // ------------------------------------------------------------------ rho + tau
 // rho = (node + filler + terminal + base) / binArea (bin_grid.cpp:361-368);
-// tau = sum_b max(0, (node + terminal + base)/area - target) area / denom
+// the tau numerator sum_b max(0, (node + terminal + base)/area - target) area
 // (bin_grid.cpp:373-393, fillers excluded).  Clears the integer grids.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned long long* nodeQ,
                                                      unsigned long long* fillQ,
                                                      const double* terminal, const double* base,
-                                                     double invQ, double binArea, double denom,
-                                                     double rhoShift, T* rho, long long* dbgNode,
+                                                     double invQ, double binArea, double rhoShift,
+                                                     T* rho, long long* dbgNode,
                                                      long long* dbgFill, double* partials,
-                                                     unsigned* counter, double* scalars, Ctl* ctl) {
-    if (ctl && ctl->done) return;   // uniform: no block reaches the reduction
+                                                     const Ctl* ctl) {
+    if (ctl && ctl->done) return;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = g.dim * g.dim;
     double v[1] = {0.0};
@@ -179,23 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned lon
         if (dbgNode) dbgNode[b] = qn;
         if (dbgFill) dbgFill[b] = qf;
     }
-    gridReduce<1>(v, partials, counter, [&](double (&s)[1]) {
-        scalars[0] = denom > 0.0 ? s[0] / denom : 0.0;
-        if (ctl) {
-            // the previous slot's update has finished: apply its step rotation
-            // (nesterov.cpp:109-118) before this slot's node pass
-            Ctl& c = *ctl;
-            if (c.rotate) {
-                const int ou = c.iu, ov = c.iv, oup = c.iup, ovp = c.ivp, oun = c.iun, ovn = c.ivn,
-                          og = c.ig, ogp = c.igp;
-                c.iup = ou; c.igp = og; c.ig = ogp;
-                c.iv = ovn; c.ivp = ov; c.ivn = ovp;
-                c.iu = oun; c.iun = oup;
-                c.rotate = 0;
-            }
-            c.snapWrite = -1;
-        }
-    });
+    blockPartial<1>(v, partials);
 }
 
 // ------------------------------------------------------------------ (c) WA pass 1 + HPWL
@@ -221,8 +192,7 @@ __device__ inline T tmax() { return sizeof(T) == 4 ? T(3.402823466e+38f) : T(1.7
 // the HPWL of the net (hpwl.cpp:20-37) from the same extrema.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_wa_small(DevDesign<T> d, T invG, int ignoreDeg,
-                                                     double* partials, unsigned* counter,
-                                                     double* scalars, const Ctl* ctl) {
+                                                     double* partials, const Ctl* ctl) {
     if (ctl) {
         if (ctl->done) return;
         invG = invGammaOf<T>(ctl->gamma);
@@ -262,10 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_wa_small(DevDesign<T> d, T invG, int
         }
         d.stats[k] = st;   // bPX == 0 marks a skipped net (wa_wirelength.cpp:123-128)
     }
-    gridReduce<2>(v, partials, counter, [&](double (&t)[2]) {
-        scalars[1] = t[0];
-        scalars[2] = t[1];
-    });
+    blockPartial<2>(v, partials);
 }
 
 template <typename T>
@@ -288,12 +255,10 @@ __device__ inline T waveSum(T v) {
 }
 
 // One net per wavefront (degree > kLargeNet, the power-law tail): the same
-// quantities with lane-strided pins and shuffle trees.  Adds its totals to
-// those of k_wa_small, which ran before it on the same stream.
+// quantities with lane-strided pins and shuffle trees.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int ignoreDeg,
-                                                     double* partials, unsigned* counter,
-                                                     double* scalars, const Ctl* ctl) {
+                                                     double* partials, const Ctl* ctl) {
     if (ctl) {
         if (ctl->done) return;
         invG = invGammaOf<T>(ctl->gamma);
@@ -339,10 +304,101 @@ __global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int
             v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
         }
     }
-    gridReduce<2>(v, partials, counter, [&](double (&t)[2]) {
-        scalars[1] += t[0];
-        scalars[2] += t[1];
-    });
+    blockPartial<2>(v, partials);
+}
+
+// ------------------------------------------------------------------ (c)+(d) node pass
+// For each optimised node k (node i = optIdx[k]):
+//  * WA gradient over the node's pins (wa_wirelength.cpp:183-225),
+//  * density force F_i = sum_bins q * xi (bin_grid.cpp:395-426),
+//  * preconditioned gradient grad = (dWL - lambda F) / max(1, deg + lambda area)
+//    (eplace.cpp:162-185),
+//  * FP64 partial sums: sum|dWL|, sum|F| (eplace.cpp:168-190) and the
+//    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
+//    (nesterov.cpp:25-44), plus a count of non-finite gradients.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* ex,
+                                                      const T* ey, T invG, float lambda,
+                                                      const T* u, const T* up, const T* gp, T* grad,
+                                                      T* dbgForce, T* dbgWl, double* partials,
+                                                      const Ctl* ctl, T* const* bufs) {
+    if (ctl) {
+        if (ctl->done) return;
+        lambda = ctl->lambda;
+        invG = invGammaOf<T>(ctl->gamma);
+        u = bufs[ctl->nxt[B_U]];
+        up = bufs[ctl->nxt[B_UP]];
+        gp = bufs[ctl->nxt[B_GP]];
+        grad = bufs[ctl->nxt[B_G]];
+    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = d.nOpt;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (k < n) {
+        const int i = d.optIdx[k];
+        const T xi = d.x[i], yi = d.y[i];
+        T gx = 0, gy = 0;
+        const bool hasWl = i < d.numMov;
+        if (hasWl) {
+            for (int t = d.nodeStart[i]; t < d.nodeStart[i + 1]; ++t) {
+                const int net = d.npNet[t];
+                if (net < 0) continue;
+                const NetStat<T> st = d.stats[net];
+                if (st.bPX <= T(0)) continue;
+                const T px = xi + d.npOffX[t], py = yi + d.npOffY[t];
+                {
+                    const T dp = px - st.mxX, dn = px - st.mnX;
+                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
+                    const T tMax = ap * (st.bPX + (dp * st.bPX - st.cPX) * invG) / (st.bPX * st.bPX);
+                    const T tMin = an * (st.bNX + (st.cNX - dn * st.bNX) * invG) / (st.bNX * st.bNX);
+                    gx += tMax - tMin;
+                }
+                {
+                    const T dp = py - st.mxY, dn = py - st.mnY;
+                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
+                    const T tMax = ap * (st.bPY + (dp * st.bPY - st.cPY) * invG) / (st.bPY * st.bPY);
+                    const T tMin = an * (st.bNY + (st.cNY - dn * st.bNY) * invG) / (st.bNY * st.bNY);
+                    gy += tMax - tMin;
+                }
+            }
+        }
+        T fx = 0, fy = 0;
+        const uint8_t f = d.flags[i];
+        if (!(f & (F_FIXED | F_NI))) {
+            const Footprint<T> fp = footprintOf<T>(xi, yi, d.w[i], d.h[i], f, g);
+            for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
+                for (int by = fp.by0; by <= fp.by1; ++by) {
+                    const T q = binCharge<T>(bx, by, fp, g);
+                    if (q <= T(0)) continue;
+                    const long idx = static_cast<long>(bx) * g.ld + by;
+                    fx += q * ex[idx];
+                    fy += q * ey[idx];
+                }
+        }
+        const T lam = static_cast<T>(lambda);
+        const T precond = static_cast<T>(
+            1.0 / fmax(1.0, static_cast<double>(d.precDeg[k]) +
+                                static_cast<double>(lambda) * d.precArea[k]));
+        const T g0 = precond * (gx - lam * fx);
+        const T g1 = precond * (gy - lam * fy);
+        grad[k] = g0;
+        grad[n + k] = g1;
+        if (dbgForce) { dbgForce[k] = fx; dbgForce[n + k] = fy; }
+        if (dbgWl) { dbgWl[k] = gx; dbgWl[n + k] = gy; }
+        v[0] = hasWl ? fabs(static_cast<double>(gx)) + fabs(static_cast<double>(gy)) : 0.0;
+        v[1] = fabs(static_cast<double>(fx)) + fabs(static_cast<double>(fy));
+        const double u0 = u[k], u1 = u[n + k];
+        const double du0 = u0 - (up ? static_cast<double>(up[k]) : 0.0);
+        const double du1 = u1 - (up ? static_cast<double>(up[n + k]) : 0.0);
+        const double dg0 = static_cast<double>(g0) - (gp ? static_cast<double>(gp[k]) : 0.0);
+        const double dg1 = static_cast<double>(g1) - (gp ? static_cast<double>(gp[n + k]) : 0.0);
+        v[2] = du0 * du0 + du1 * du1;
+        v[3] = dg0 * dg0 + dg1 * dg1;
+        v[4] = du0 * dg0 + du1 * dg1;
+        v[5] = static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1;
+        v[6] = (isfinite(g0) ? 0.0 : 1.0) + (isfinite(g1) ? 0.0 : 1.0);
+    }
+    blockPartial<7>(v, partials);
 }
 
 // ------------------------------------------------------------------ (d) decisions
@@ -360,18 +416,20 @@ __device__ inline float lambdaMul(double dH, double ref) {
     return static_cast<float>(fmin(1.05, fmax(0.95, mu)));
 }
 
-// One thread, after the node pass of an evaluation has reduced its scalars
-// (sc = EvalScalars): the backtracking check (nesterov.cpp:146-169), the step
-// size (:88, :221-240), the momentum (:92-93), then - for a completed step -
-// the schedules (eplace.cpp:371-380), the metrics row (:384-399), the stop
-// test (:411-419), the tau-ladder snapshots and stagnation rollback
-// (:421-473) and the iteration limit.  The update kernel executes the chosen
-// mode; the next finalize applies the buffer rotation.
+// One thread, once the scalars of an evaluation are summed (sc = EvalScalars):
+// the backtracking check (nesterov.cpp:146-169), the step size (:88,
+// :221-240), the momentum (:92-93), then - for a completed step - the
+// schedules (eplace.cpp:371-380), the metrics row (:384-399), the stop test
+// (:411-419), the tau-ladder snapshots and stagnation rollback (:421-473) and
+// the iteration limit.  The update kernel executes the chosen mode with the
+// `cur` indices; `nxt` holds the indices after the step's rotation
+// (nesterov.cpp:109-118).
 template <typename T>
 __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
+    for (int r = 0; r < B_ROLES; ++r) c.cur[r] = c.nxt[r];
+    c.snapWrite = -1;
     const double tau = sc[0], wl = sc[1], hpwl = sc[2], sumWl = sc[3], sumDen = sc[4];
     const double dx = sqrt(sc[5]), dg = sqrt(sc[6]), sy = sc[7], nonfin = sc[9];
-    c.snapWrite = -1;
     if (!isfinite(tau) || !isfinite(hpwl) || !isfinite(wl) || !isfinite(sumWl) ||
         !isfinite(sumDen) || nonfin > 0.0) {
         c.done = 1;
@@ -386,7 +444,7 @@ __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
         if (!(lip > 0.95 * c.alpha)) {
             c.alpha = static_cast<double>(static_cast<T>(lip));
             c.m = c.momPrev;
-            c.mode = kModeRedo;
+            c.mode = kModeRedo;   // v, u rewritten in place: nxt = cur
             ++c.tries;
             return;
         }
@@ -412,8 +470,15 @@ __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
     c.momPrev = c.m;
     c.a = static_cast<double>(aNext);
     c.mode = kModeStep;
-    c.rotate = 1;
-    const int lastEval = c.iu;   // the point this evaluation used
+    c.nxt[B_U] = c.cur[B_UN];
+    c.nxt[B_V] = c.cur[B_VN];
+    c.nxt[B_UP] = c.cur[B_U];
+    c.nxt[B_VP] = c.cur[B_V];
+    c.nxt[B_UN] = c.cur[B_UP];
+    c.nxt[B_VN] = c.cur[B_VP];
+    c.nxt[B_G] = c.cur[B_GP];
+    c.nxt[B_GP] = c.cur[B_G];
+    const int lastEval = c.cur[B_U];   // the point this evaluation used
 
     const float evalLambda = c.lambda, evalGamma = c.gamma;
     const float ftau = static_cast<float>(tau);
@@ -501,108 +566,30 @@ __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
     if (c.iter >= c.maxIter) {
         c.done = 1;
         c.status = kIterLimit;
-        c.finalBuf = c.ivn;   // v after this step (pos = v, nesterov.cpp:121)
+        c.finalBuf = c.cur[B_VN];   // v after this step (pos = v, nesterov.cpp:121)
     }
 }
 
-// ------------------------------------------------------------------ (c)+(d) node pass
-// For each optimised node k (node i = optIdx[k]):
-//  * WA gradient over the node's pins (wa_wirelength.cpp:183-225),
-//  * density force F_i = sum_bins q * xi (bin_grid.cpp:395-426),
-//  * preconditioned gradient grad = (dWL - lambda F) / max(1, deg + lambda area)
-//    (eplace.cpp:162-185),
-//  * FP64 partial sums: sum|dWL|, sum|F| (eplace.cpp:168-190) and the
-//    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
-//    (nesterov.cpp:25-44), plus a count of non-finite gradients.
+// Single block: sums the regions this evaluation wrote into EvalScalars
+// [tau, wl, hpwl, sum_wl, sum_den, dx2, dg2, sy, gg, nonfinite], then (device
+// loop) takes the decisions.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* ex,
-                                                      const T* ey, T invG, float lambda,
-                                                      const T* u, const T* up, const T* gp, T* grad,
-                                                      T* dbgForce, T* dbgWl, double* partials,
-                                                      unsigned* counter, double* scalars, Ctl* ctl,
-                                                      T* const* bufs, DevRow* rows) {
-    if (ctl) {
-        // every block reads the state before the last block decides (it is
-        // the last one to arrive at the reduction), so this read is uniform
-        if (ctl->done) return;
-        lambda = ctl->lambda;
-        invG = invGammaOf<T>(ctl->gamma);
-        u = bufs[ctl->iu];
-        up = bufs[ctl->iup];
-        gp = bufs[ctl->igp];
-        grad = bufs[ctl->ig];
-    }
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = d.nOpt;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (k < n) {
-        const int i = d.optIdx[k];
-        const T xi = d.x[i], yi = d.y[i];
-        T gx = 0, gy = 0;
-        const bool hasWl = i < d.numMov;
-        if (hasWl) {
-            for (int t = d.nodeStart[i]; t < d.nodeStart[i + 1]; ++t) {
-                const int net = d.npNet[t];
-                if (net < 0) continue;
-                const NetStat<T> st = d.stats[net];
-                if (st.bPX <= T(0)) continue;
-                const T px = xi + d.npOffX[t], py = yi + d.npOffY[t];
-                {
-                    const T dp = px - st.mxX, dn = px - st.mnX;
-                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
-                    const T tMax = ap * (st.bPX + (dp * st.bPX - st.cPX) * invG) / (st.bPX * st.bPX);
-                    const T tMin = an * (st.bNX + (st.cNX - dn * st.bNX) * invG) / (st.bNX * st.bNX);
-                    gx += tMax - tMin;
-                }
-                {
-                    const T dp = py - st.mxY, dn = py - st.mnY;
-                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
-                    const T tMax = ap * (st.bPY + (dp * st.bPY - st.cPY) * invG) / (st.bPY * st.bPY);
-                    const T tMin = an * (st.bNY + (st.cNY - dn * st.bNY) * invG) / (st.bNY * st.bNY);
-                    gy += tMax - tMin;
-                }
-            }
-        }
-        T fx = 0, fy = 0;
-        const uint8_t f = d.flags[i];
-        if (!(f & (F_FIXED | F_NI))) {
-            const Footprint<T> fp = footprintOf<T>(xi, yi, d.w[i], d.h[i], f, g);
-            for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
-                for (int by = fp.by0; by <= fp.by1; ++by) {
-                    const T q = binCharge<T>(bx, by, fp, g);
-                    if (q <= T(0)) continue;
-                    const long idx = static_cast<long>(bx) * g.ld + by;
-                    fx += q * ex[idx];
-                    fy += q * ey[idx];
-                }
-        }
-        const T lam = static_cast<T>(lambda);
-        const T precond = static_cast<T>(
-            1.0 / fmax(1.0, static_cast<double>(d.precDeg[k]) +
-                                static_cast<double>(lambda) * d.precArea[k]));
-        const T g0 = precond * (gx - lam * fx);
-        const T g1 = precond * (gy - lam * fy);
-        grad[k] = g0;
-        grad[n + k] = g1;
-        if (dbgForce) { dbgForce[k] = fx; dbgForce[n + k] = fy; }
-        if (dbgWl) { dbgWl[k] = gx; dbgWl[n + k] = gy; }
-        v[0] = hasWl ? fabs(static_cast<double>(gx)) + fabs(static_cast<double>(gy)) : 0.0;
-        v[1] = fabs(static_cast<double>(fx)) + fabs(static_cast<double>(fy));
-        const double u0 = u[k], u1 = u[n + k];
-        const double du0 = u0 - (up ? static_cast<double>(up[k]) : 0.0);
-        const double du1 = u1 - (up ? static_cast<double>(up[n + k]) : 0.0);
-        const double dg0 = static_cast<double>(g0) - (gp ? static_cast<double>(gp[k]) : 0.0);
-        const double dg1 = static_cast<double>(g1) - (gp ? static_cast<double>(gp[n + k]) : 0.0);
-        v[2] = du0 * du0 + du1 * du1;
-        v[3] = dg0 * dg0 + dg1 * dg1;
-        v[4] = du0 * dg0 + du1 * dg1;
-        v[5] = static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1;
-        v[6] = (isfinite(g0) ? 0.0 : 1.0) + (isfinite(g1) ? 0.0 : 1.0);
-    }
-    gridReduce<7>(v, partials, counter, [&](double (&t)[7]) {
-        for (int j = 0; j < 7; ++j) scalars[3 + j] = t[j];
-        if (ctl) decideStep<T>(*ctl, scalars, rows);
-    });
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, int withNode,
+                                                   Ctl* ctl, DevRow* rows) {
+    if (ctl && ctl->done) return;
+    double r0[1], r1[2], r2[2], r3[7];
+    sumRegion<1>(ws.region(0), ws.blocks[0], r0);
+    sumRegion<2>(ws.region(1), ws.blocks[1], r1);
+    sumRegion<2>(ws.region(2), ws.blocks[2], r2);
+    if (withNode) sumRegion<7>(ws.region(3), ws.blocks[3], r3);
+    if (threadIdx.x != 0) return;
+    double* sc = ws.scalars;
+    sc[0] = denom > 0.0 ? r0[0] / denom : 0.0;
+    sc[1] = r1[0] + r2[0];
+    sc[2] = r1[1] + r2[1];
+    if (withNode)
+        for (int j = 0; j < 7; ++j) sc[3 + j] = r3[j];
+    if (ctl && withNode) decideStep<T>(*ctl, sc, rows);
 }
 
 // ------------------------------------------------------------------ (d) update
@@ -615,6 +602,8 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.nOpt;
     if (ctl) {
+        // After `done` the mode and indices stay fixed, so a repeated update
+        // rewrites the same values (reads never alias writes).
         const int mode = ctl->mode;
         if (mode == kModeNone) return;
         if (k < n && ctl->snapWrite >= 0) {   // tau-ladder snapshot of the evaluated point
@@ -625,12 +614,13 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
         }
         alpha = static_cast<T>(ctl->alpha);
         m = static_cast<T>(ctl->m);
+        const int* c = ctl->cur;
         if (mode == kModeStep) {
-            A = bufs[ctl->iu]; G = bufs[ctl->ig]; B = bufs[ctl->iv];
-            vOut = bufs[ctl->ivn]; uOut = bufs[ctl->iun];
+            A = bufs[c[B_U]]; G = bufs[c[B_G]]; B = bufs[c[B_V]];
+            vOut = bufs[c[B_VN]]; uOut = bufs[c[B_UN]];
         } else {   // redo the previous step from u_prev (nesterov.cpp:156-165)
-            A = bufs[ctl->iup]; G = bufs[ctl->igp]; B = bufs[ctl->ivp];
-            vOut = bufs[ctl->iv]; uOut = bufs[ctl->iu];
+            A = bufs[c[B_UP]]; G = bufs[c[B_GP]]; B = bufs[c[B_VP]];
+            vOut = bufs[c[B_V]]; uOut = bufs[c[B_U]];
         }
     }
     if (k >= n) return;
@@ -673,27 +663,26 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
 template <typename T>
 void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
                     const double* terminal, const double* base, double invQ, double binArea,
-                    double movableAreaScaled, double rhoShift, T* rho, long long* dbgNode,
-                    long long* dbgFill, const ReduceWs& ws, Ctl* ctl, hipStream_t s) {
+                    double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
+                    ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
     const long nb = static_cast<long>(g.dim) * g.dim;
-    k_finalize<T><<<gridFor(nb), kBlock, 0, s>>>(g, nodeQ, fillQ, terminal, base, invQ, binArea,
-                                                 movableAreaScaled, rhoShift, rho, dbgNode, dbgFill,
-                                                 ws.partials, ws.counters + 0, ws.scalars, ctl);
+    ws.blocks[0] = static_cast<int>(gridFor(nb));
+    k_finalize<T><<<ws.blocks[0], kBlock, 0, s>>>(g, nodeQ, fillQ, terminal, base, invQ, binArea,
+                                                 rhoShift, rho, dbgNode, dbgFill, ws.region(0),
+                                                 ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const ReduceWs& ws,
+void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& ws,
                   const Ctl* ctl, hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
-    const long slot = static_cast<long>(ws.maxBlocks) * 8;
-    k_wa_small<T><<<gridFor(d.nSmallNets > 0 ? d.nSmallNets : 1), kBlock, 0, s>>>(
-        d, invG, ignoreDeg, ws.partials + slot, ws.counters + 1, ws.scalars, ctl);
+    ws.blocks[1] = static_cast<int>(gridFor(d.nSmallNets > 0 ? d.nSmallNets : 1));
+    k_wa_small<T><<<ws.blocks[1], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(1), ctl);
     SPHIP_CHECK(hipGetLastError());
+    ws.blocks[2] = (d.nLargeNets + 3) / 4;
     if (d.nLargeNets > 0) {
-        k_wa_large<T><<<(d.nLargeNets + 3) / 4, kBlock, 0, s>>>(d, invG, ignoreDeg,
-                                                                 ws.partials + 2 * slot,
-                                                                 ws.counters + 2, ws.scalars, ctl);
+        k_wa_large<T><<<ws.blocks[2], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(2), ctl);
         SPHIP_CHECK(hipGetLastError());
     }
 }
@@ -701,13 +690,19 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, const Reduc
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, const ReduceWs& ws, Ctl* ctl, T* const* bufs,
-                    DevRow* rows, hipStream_t s) {
+                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
+                    hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
-    const long slot = static_cast<long>(ws.maxBlocks) * 8;
-    k_node_grad<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, g, ex, ey, invG, lambda, u, up, gp, grad,
-                                                      dbgForce, dbgWl, ws.partials + 3 * slot,
-                                                      ws.counters + 3, ws.scalars, ctl, bufs, rows);
+    ws.blocks[3] = static_cast<int>(gridFor(d.nOpt));
+    k_node_grad<T><<<ws.blocks[3], kBlock, 0, s>>>(d, g, ex, ey, invG, lambda, u, up, gp, grad,
+                                                  dbgForce, dbgWl, ws.region(3), ctl, bufs);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchReduce(ReduceWs& ws, double denom, bool withNode, Ctl* ctl, DevRow* rows,
+                  hipStream_t s) {
+    k_reduce<T><<<1, kBlock, 0, s>>>(ws, denom, withNode ? 1 : 0, ctl, rows);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -724,14 +719,14 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
                                    unsigned long long*, unsigned long long*, const Ctl*,         \
                                    hipStream_t);                                                 \
     template void launchFinalize<T>(const GridGeom<T>&, unsigned long long*, unsigned long long*, \
-                                    const double*, const double*, double, double, double, double, \
-                                    T*, long long*, long long*, const ReduceWs&, Ctl*,           \
-                                    hipStream_t);                                                \
-    template void launchWaNets<T>(const DevDesign<T>&, float, int, const ReduceWs&, const Ctl*,  \
+                                    const double*, const double*, double, double, double, T*,    \
+                                    long long*, long long*, ReduceWs&, const Ctl*, hipStream_t); \
+    template void launchWaNets<T>(const DevDesign<T>&, float, int, ReduceWs&, const Ctl*,        \
                                   hipStream_t);                                                  \
     template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*, const T*, \
                                     float, float, const T*, const T*, const T*, T*, T*, T*,      \
-                                    const ReduceWs&, Ctl*, T* const*, DevRow*, hipStream_t);     \
+                                    ReduceWs&, const Ctl*, T* const*, hipStream_t);              \
+    template void launchReduce<T>(ReduceWs&, double, bool, Ctl*, DevRow*, hipStream_t);         \
     template void launchUpdate<T>(const DevDesign<T>&, const T*, const T*, const T*, double,     \
                                   double, T*, T*, const Ctl*, T* const*, hipStream_t);
 SPHIP_INST(float)
