@@ -235,6 +235,7 @@ public:
         }
         SPHIP_CHECK(hipMemcpyAsync(u_, pos.data(), pos.size() * sizeof(T), hipMemcpyHostToDevice, s_));
         curPos_ = nullptr;   // x/y now hold the raw positions, not a clamped pos buffer
+        sortNext_ = true;
         const bool wantDbg = dbg != nullptr;
         long long *dNode = nullptr, *dFill = nullptr;
         T *dForce = nullptr, *dWl = nullptr;
@@ -304,7 +305,7 @@ private:
     void runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows, float lambda,
                    float gamma, double alpha0, double effRef, float targetTau, int maxIter,
                    int maxBt, float stepX);
-    void launchSlot(bool events);
+    void launchSlot(bool events, bool sort);
     void build(const sp_hip_design& d);
     void release();
     EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
@@ -369,6 +370,9 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
     int graphEvMask_ = 0;    // families timed by event nodes inside the graph
+    static constexpr int kSortEvery = 1;   // spatial re-sort period (evaluations)
+    long evalCount_ = 0;
+    bool sortNext_ = true;
     int profMask_ = 0;   // bit f: time kernel family f with HIP events
     hipEvent_t evStart_[SP_HIP_K_COUNT] = {}, evStop_[SP_HIP_K_COUNT] = {};
     bool pendingK_[SP_HIP_K_COUNT] = {};
@@ -567,6 +571,11 @@ void Engine<T>::build(const sp_hip_design& d) {
     dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_)));
     dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_)));
     dd_.densIdx = static_cast<int*>(own(dupload<int>(dens, s_)));
+    dd_.densA = static_cast<int*>(own(dupload<int>(dens, s_)));
+    dd_.densB = static_cast<int*>(own(dupload<int>(dens, s_)));
+    dd_.tileCount = static_cast<int*>(own(dalloc<int>(kTileKeys)));
+    dd_.tileCursor = static_cast<int*>(own(dalloc<int>(kTileKeys)));
+    SPHIP_CHECK(hipMemsetAsync(dd_.tileCount, 0, kTileKeys * sizeof(int), s_));
     std::vector<double> parea(n_);
     std::vector<float> pdeg(n_);
     for (int k = 0; k < n_; ++k) {
@@ -576,15 +585,34 @@ void Engine<T>::build(const sp_hip_design& d) {
     }
     dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_)));
     dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_)));
-    // net-ordered pin slots
-    std::vector<int> netStart(d.net2pin_start, d.net2pin_start + d.num_nets + 1);
+    // Nets are renumbered by degree (stable): the thread-per-net pass then
+    // has lanes with the same pin count in each wavefront while its slot reads
+    // and per-net state writes stay contiguous; the tail of large nets is a
+    // contiguous range at the end.  Pin slots are laid out in the new order.
+    std::vector<int> netOrder(d.num_nets);
+    std::iota(netOrder.begin(), netOrder.end(), 0);
+    std::stable_sort(netOrder.begin(), netOrder.end(), [&](int a, int b) {
+        return d.net2pin_start[a + 1] - d.net2pin_start[a] <
+               d.net2pin_start[b + 1] - d.net2pin_start[b];
+    });
+    std::vector<int> newNetId(d.num_nets);
+    for (int k = 0; k < d.num_nets; ++k) newNetId[netOrder[k]] = k;
+    std::vector<int> netStart(d.num_nets + 1, 0);
     std::vector<int> slotNode(d.num_pins);
     std::vector<float> slotOffX(d.num_pins), slotOffY(d.num_pins);
-    for (int t = 0; t < d.num_pins; ++t) {
-        const int p = d.flat_net2pin[t];
-        slotNode[t] = d.pin2node[p];
-        slotOffX[t] = d.pin_offset_x[p];
-        slotOffY[t] = d.pin_offset_y[p];
+    {
+        int t = 0;
+        for (int k = 0; k < d.num_nets; ++k) {
+            const int old = netOrder[k];
+            netStart[k] = t;
+            for (int q = d.net2pin_start[old]; q < d.net2pin_start[old + 1]; ++q, ++t) {
+                const int p = d.flat_net2pin[q];
+                slotNode[t] = d.pin2node[p];
+                slotOffX[t] = d.pin_offset_x[p];
+                slotOffY[t] = d.pin_offset_y[p];
+            }
+        }
+        netStart[d.num_nets] = t;
     }
     std::vector<int> small, large;
     for (int k = 0; k < d.num_nets; ++k)
@@ -597,13 +625,13 @@ void Engine<T>::build(const sp_hip_design& d) {
     dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_)));
     dd_.smallNets = static_cast<int*>(own(dupload<int>(small, s_)));
     dd_.largeNets = static_cast<int*>(own(dupload<int>(large, s_)));
-    // node-ordered pin entries
+    // node-ordered pin entries (net ids in the new numbering)
     std::vector<int> nodeStart(d.node2pin_start, d.node2pin_start + total + 1);
     std::vector<int> npNet(d.num_pins);
     std::vector<float> npOffX(d.num_pins), npOffY(d.num_pins);
     for (int t = 0; t < d.num_pins; ++t) {
         const int p = d.flat_node2pin[t];
-        npNet[t] = d.pin2net[p];
+        npNet[t] = d.pin2net[p] >= 0 ? newNetId[d.pin2net[p]] : -1;
         npOffX[t] = d.pin_offset_x[p];
         npOffY[t] = d.pin_offset_y[p];
     }
@@ -663,6 +691,9 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
                                 float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
                                 T* dbgWl) {
     beginK(SP_HIP_K_SCATTER);
+    if (sortNext_ || evalCount_ % kSortEvery == 0) launchTileSort<T>(dd_, geom_, nullptr, s_);
+    sortNext_ = false;
+    ++evalCount_;
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
     endK(SP_HIP_K_SCATTER);
     beginK(SP_HIP_K_FINALIZE);
@@ -700,6 +731,7 @@ void Engine<T>::scatterPos(const T* pos) {
 // Delivery (eplace.cpp:478-494): positions already scattered; tau and HPWL.
 template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
+    launchTileSort<T>(dd_, geom_, nullptr, s_);
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
     launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
@@ -976,12 +1008,13 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
 }
 
 template <typename T>
-void Engine<T>::launchSlot(bool events) {
+void Engine<T>::launchSlot(bool events, bool sort) {
     const T qscale = static_cast<T>(std::ldexp(1.0, fracBits_));
     const int* skip = reinterpret_cast<const int*>(reinterpret_cast<const char*>(dCtl_) +
                                                    offsetof(Ctl, done));
     auto b = [&](int f) { if (events) beginK(f); };
     auto e = [&](int f) { if (events) endK(f); };
+    if (sort) launchTileSort<T>(dd_, geom_, dCtl_, s_);
     b(SP_HIP_K_SCATTER);
     launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, dCtl_, s_);
     e(SP_HIP_K_SCATTER);
@@ -1071,7 +1104,8 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         const int saved = profMask_;
         SPHIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         try {
-            for (int i = 0; i < slots; ++i) launchSlot(i == 0 && graphEvMask_ != 0);
+            for (int i = 0; i < slots; ++i)
+                launchSlot(i == 0 && graphEvMask_ != 0, i % kSortEvery == 0);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(s_, &g);
@@ -1090,7 +1124,8 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         if (graph) {
             SPHIP_CHECK(hipGraphLaunch(graphExec_, s_));
         } else {
-            for (int i = 0; i < slots; ++i) launchSlot(profMask_ != 0);
+            for (int i = 0; i < slots; ++i)
+                launchSlot(profMask_ != 0, (launch * slots + i) % kSortEvery == 0);
         }
         SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
                                    sizeof(int), hipMemcpyDeviceToHost, s_));

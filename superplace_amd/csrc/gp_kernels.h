@@ -28,6 +28,13 @@ struct DevDesign {
     const uint8_t* flags = nullptr;
     const int* optIdx = nullptr;     // pos index k -> node i
     const int* densIdx = nullptr;    // nodes with a density contribution
+    // the density nodes in spatial order (Morton order of 16x16-bin tiles of
+    // their centre bin), refreshed periodically by launchTileSort: the
+    // scatter reads densB and keeps densA = densB for the next sort
+    int* densA = nullptr;
+    int* densB = nullptr;
+    int* tileCount = nullptr;     // kTileKeys counters (zero between sorts)
+    int* tileCursor = nullptr;    // kTileKeys insertion cursors
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
     // net -> pin slots (net order)
@@ -46,6 +53,10 @@ struct DevDesign {
 };
 
 constexpr int kLargeNet = 32;
+constexpr int kTileBins = 16;        // bins per tile side for the spatial sort
+constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
+constexpr int kScatterChunk = 256;   // density nodes per scatter block (1 per thread)
+constexpr int kWindowBins = 1024;    // LDS window of the scatter block (per grid)
 
 // FP64 reductions: every reducing kernel writes one partial per block (plain
 // stores, no atomics, no fences); k_reduce sums each region in block order
@@ -110,6 +121,11 @@ template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
                    hipStream_t s);
+
+// Re-sorts the density nodes by tile (counting sort; the order inside a tile
+// is arbitrary, which the integer density sums do not see).
+template <typename T>
+void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s);
 
 template <typename T>
 void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,

@@ -111,25 +111,199 @@ __device__ inline T binCharge(int bx, int by, const Footprint<T>& fp, const Grid
 __device__ inline long long toFixed(float v, float s) { return __float2ll_rn(v * s); }
 __device__ inline long long toFixed(double v, double s) { return __double2ll_rn(v * s); }
 
+// ------------------------------------------------------------------ spatial order
+// Morton interleave of 6-bit tile coordinates (dim <= 1024 -> 64 x 64 tiles).
+__device__ inline unsigned mortonKey(unsigned tx, unsigned ty) {
+    unsigned k = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) k |= ((tx >> b) & 1u) << (2 * b) | ((ty >> b) & 1u) << (2 * b + 1);
+    return k;
+}
+
+template <typename T>
+__device__ inline unsigned tileKeyOf(T x, T y, const GridGeom<T>& g) {
+    const int bx = clampi(static_cast<int>((x - g.originX) / g.stepX), 0, g.dim - 1);
+    const int by = clampi(static_cast<int>((y - g.originY) / g.stepY), 0, g.dim - 1);
+    return mortonKey(static_cast<unsigned>(bx / kTileBins), static_cast<unsigned>(by / kTileBins));
+}
+
+// Counting sort by tile key.  Each block takes kBlock nodes of the current
+// order, counts its keys in an LDS histogram (remembering the distinct keys),
+// then touches each distinct key once in global memory: pass 1 adds the
+// block's counts, pass 3 reserves the block's range at the key's cursor and
+// hands out ranks from LDS.  The order within a tile is arbitrary, which the
+// integer density sums do not see.
+struct TileHist {
+    int count[kTileKeys];
+    int keys[kBlock];
+    int nKeys;
+};
+
+template <typename T>
+__device__ inline unsigned blockTileHist(const DevDesign<T>& d, const GridGeom<T>& g, TileHist& h,
+                                         int* outNode) {
+    for (int k = threadIdx.x; k < kTileKeys; k += kBlock) h.count[k] = 0;
+    if (threadIdx.x == 0) h.nKeys = 0;
+    __syncthreads();
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    unsigned key = 0xFFFFFFFFu;
+    if (t < d.nDens) {
+        const int i = d.densA[t];
+        *outNode = i;
+        key = tileKeyOf<T>(d.x[i], d.y[i], g);
+        if (atomicAdd(&h.count[key], 1) == 0) h.keys[atomicAdd(&h.nKeys, 1)] = key;
+    }
+    __syncthreads();
+    return key;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<T> g,
+                                                       const Ctl* ctl) {
+    if (ctl && ctl->done) return;
+    __shared__ TileHist h;
+    int node = 0;
+    blockTileHist<T>(d, g, h, &node);
+    for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
+        const int key = h.keys[q];
+        atomicAdd(&d.tileCount[key], h.count[key]);
+    }
+}
+
+// Pass 2 (one block): exclusive scan of the counts into cursors; clears counts.
+__global__ __launch_bounds__(kBlock) void k_tile_scan(int* count, int* cursor, const Ctl* ctl) {
+    if (ctl && ctl->done) return;
+    constexpr int per = kTileKeys / kBlock;
+    const int t0 = threadIdx.x * per;
+    int local[per];
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        local[j] = count[t0 + j];
+        sum += local[j];
+    }
+    // inclusive scan of per-thread sums: wave shuffles, then the wave totals
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += v;
+    }
+    __shared__ int waveTot[kBlock / 64];
+    if (lane == 63) waveTot[wave] = inc;
+    __syncthreads();
+    int offset = 0;
+    for (int w = 0; w < wave; ++w) offset += waveTot[w];
+    int run = offset + inc - sum;   // exclusive prefix of this thread's first key
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        cursor[t0 + j] = run;
+        run += local[j];
+        count[t0 + j] = 0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<T> g,
+                                                       const Ctl* ctl) {
+    if (ctl && ctl->done) return;
+    __shared__ TileHist h;
+    int node = 0;
+    const unsigned key = blockTileHist<T>(d, g, h, &node);
+    // reserve this block's range per key; reuse count[] as the range base
+    for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
+        const int k = h.keys[q];
+        h.count[k] = atomicAdd(&d.tileCursor[k], h.count[k]);
+    }
+    __syncthreads();
+    if (key != 0xFFFFFFFFu) d.densB[atomicAdd(&h.count[key], 1)] = node;
+}
+
 // ------------------------------------------------------------------ (a) scatter
+// A block takes kScatterChunk consecutive nodes of the spatial order and
+// accumulates their fixed-point charges into an LDS window (the bin bounding
+// box of the chunk's small footprints, clipped to kWindowBins), then adds the
+// window's non-zero bins to the grids with one global atomic each.  Charges of
+// footprints outside the window go straight to global atomics.  The scatter
+// also copies its part of the order back (densA = densB) for the next sort.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
                                                     unsigned long long* fillQ, const Ctl* ctl) {
     if (ctl && ctl->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.nDens) return;
-    const int i = d.densIdx[t];
-    const uint8_t f = d.flags[i];
-    const Footprint<T> fp = footprintOf<T>(d.x[i], d.y[i], d.w[i], d.h[i], f, g);
-    unsigned long long* grid = (f & F_FILLER) ? fillQ : nodeQ;
-    for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
-        for (int by = fp.by0; by <= fp.by1; ++by) {
-            const T ov = binCharge<T>(bx, by, fp, g);
-            if (ov <= T(0)) continue;
-            const long long q = toFixed(ov, qscale);
-            atomicAdd(&grid[static_cast<long>(bx) * g.dim + by], static_cast<unsigned long long>(q));
+    constexpr int PER = kScatterChunk / kBlock;
+    __shared__ unsigned long long wn[kWindowBins], wf[kWindowBins];
+    __shared__ int box[4];   // min bx, min by, -max bx, -max by of small footprints
+    if (threadIdx.x < 4) box[threadIdx.x] = 0x7FFFFFFF;
+    __syncthreads();
+    const int base = blockIdx.x * kScatterChunk;
+    Footprint<T> fp[PER];
+    uint8_t fl[PER];
+    bool valid[PER];
+    int bb[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int t = base + threadIdx.x + j * kBlock;
+        valid[j] = t < d.nDens;
+        if (!valid[j]) continue;
+        const int i = d.densB[t];
+        d.densA[t] = i;
+        fl[j] = d.flags[i];
+        fp[j] = footprintOf<T>(d.x[i], d.y[i], d.w[i], d.h[i], fl[j], g);
+        if (fp[j].bx1 - fp[j].bx0 < 4 && fp[j].by1 - fp[j].by0 < 4) {
+            bb[0] = min(bb[0], fp[j].bx0);
+            bb[1] = min(bb[1], fp[j].by0);
+            bb[2] = min(bb[2], -fp[j].bx1);
+            bb[3] = min(bb[3], -fp[j].by1);
         }
+    }
+    // bounding box: wavefront shuffle minima, one LDS atomic per wave
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) bb[q] = min(bb[q], __shfl_xor(bb[q], off, 64));
+        if ((threadIdx.x & 63) == 0) atomicMin(&box[q], bb[q]);
+    }
+    __syncthreads();
+    const int wx0 = box[0], wy0 = box[1];
+    int ww = 0, wh = 0;
+    if (wx0 != 0x7FFFFFFF) {
+        ww = -box[2] - wx0 + 1;
+        wh = -box[3] - wy0 + 1;
+        if (ww > 64) ww = 64;
+        if (ww * wh > kWindowBins) wh = kWindowBins / ww;
+    }
+    const int wsize = ww * wh;
+    for (int e = threadIdx.x; e < wsize; e += kBlock) {
+        wn[e] = 0;
+        wf[e] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        if (!valid[j]) continue;
+        const bool filler = fl[j] & F_FILLER;
+        unsigned long long* grid = filler ? fillQ : nodeQ;
+        unsigned long long* win = filler ? wf : wn;
+        for (int bx = fp[j].bx0; bx <= fp[j].bx1; ++bx)
+            for (int by = fp[j].by0; by <= fp[j].by1; ++by) {
+                const T ov = binCharge<T>(bx, by, fp[j], g);
+                if (ov <= T(0)) continue;
+                const unsigned long long q = static_cast<unsigned long long>(toFixed(ov, qscale));
+                const int lx = bx - wx0, ly = by - wy0;
+                if (lx >= 0 && lx < ww && ly >= 0 && ly < wh)
+                    atomicAdd(&win[lx * wh + ly], q);
+                else
+                    atomicAdd(&grid[static_cast<long>(bx) * g.dim + by], q);
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < wsize; e += kBlock) {
+        const long gi = static_cast<long>(wx0 + e / wh) * g.dim + wy0 + e % wh;
+        if (wn[e]) atomicAdd(&nodeQ[gi], wn[e]);
+        if (wf[e]) atomicAdd(&fillQ[gi], wf[e]);
+    }
 }
 
 // ------------------------------------------------------------------ rho + tau
@@ -577,19 +751,31 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, int withNode,
                                                    Ctl* ctl, DevRow* rows) {
     if (ctl && ctl->done) return;
+    // the decision thread works on an LDS copy of the loop state: one
+    // cooperative copy in and out instead of dependent global round trips
+    constexpr int kCtlWords = (sizeof(Ctl) + 3) / 4;
+    __shared__ __attribute__((aligned(16))) unsigned ctlWords[kCtlWords];
+    if (ctl)
+        for (int w = threadIdx.x; w < kCtlWords; w += blockDim.x)
+            ctlWords[w] = reinterpret_cast<const unsigned*>(ctl)[w];
     double r0[1], r1[2], r2[2], r3[7];
     sumRegion<1>(ws.region(0), ws.blocks[0], r0);
     sumRegion<2>(ws.region(1), ws.blocks[1], r1);
     sumRegion<2>(ws.region(2), ws.blocks[2], r2);
-    if (withNode) sumRegion<7>(ws.region(3), ws.blocks[3], r3);
-    if (threadIdx.x != 0) return;
-    double* sc = ws.scalars;
-    sc[0] = denom > 0.0 ? r0[0] / denom : 0.0;
-    sc[1] = r1[0] + r2[0];
-    sc[2] = r1[1] + r2[1];
-    if (withNode)
-        for (int j = 0; j < 7; ++j) sc[3 + j] = r3[j];
-    if (ctl && withNode) decideStep<T>(*ctl, sc, rows);
+    if (withNode) sumRegion<7>(ws.region(3), ws.blocks[3], r3);   // ends with a barrier
+    __shared__ double sc[kNumScalars];
+    if (threadIdx.x == 0) {
+        sc[0] = denom > 0.0 ? r0[0] / denom : 0.0;
+        sc[1] = r1[0] + r2[0];
+        sc[2] = r1[1] + r2[1];
+        for (int j = 0; j < 7; ++j) sc[3 + j] = withNode ? r3[j] : 0.0;
+        for (int j = 0; j < kNumScalars; ++j) ws.scalars[j] = sc[j];
+        if (ctl && withNode) decideStep<T>(*reinterpret_cast<Ctl*>(ctlWords), sc, rows);
+    }
+    __syncthreads();
+    if (ctl && withNode)
+        for (int w = threadIdx.x; w < kCtlWords; w += blockDim.x)
+            reinterpret_cast<unsigned*>(ctl)[w] = ctlWords[w];
 }
 
 // ------------------------------------------------------------------ (d) update
@@ -656,7 +842,20 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
                    hipStream_t s) {
     if (d.nDens == 0) return;
-    k_scatter<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, qscale, nodeQ, fillQ, ctl);
+    k_scatter<T><<<(d.nDens + kScatterChunk - 1) / kScatterChunk, kBlock, 0, s>>>(
+        d, g, qscale, nodeQ, fillQ, ctl);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s) {
+    if (d.nDens == 0) return;
+    if (g.dim > kTileBins * 64) throw std::runtime_error("tile sort: grid wider than 1024 bins");
+    k_tile_count<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, ctl);
+    SPHIP_CHECK(hipGetLastError());
+    k_tile_scan<<<1, kBlock, 0, s>>>(d.tileCount, d.tileCursor, ctl);
+    SPHIP_CHECK(hipGetLastError());
+    k_tile_place<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -718,6 +917,8 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
                                    unsigned long long*, unsigned long long*, const Ctl*,         \
                                    hipStream_t);                                                 \
+    template void launchTileSort<T>(const DevDesign<T>&, const GridGeom<T>&, const Ctl*,         \
+                                    hipStream_t);                                                \
     template void launchFinalize<T>(const GridGeom<T>&, unsigned long long*, unsigned long long*, \
                                     const double*, const double*, double, double, double, T*,    \
                                     long long*, long long*, ReduceWs&, const Ctl*, hipStream_t); \
