@@ -585,20 +585,17 @@ void Engine<T>::build(const sp_hip_design& d) {
     }
     dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_)));
     dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_)));
-    // Nets are renumbered by degree (stable): the thread-per-net pass then
-    // has lanes with the same pin count in each wavefront while its slot reads
-    // and per-net state writes stay contiguous; the tail of large nets is a
-    // contiguous range at the end.  Pin slots are laid out in the new order.
+    // Nets are renumbered by degree (stable), so each WA size class is a
+    // contiguous range of nets and of pin slots; pin slots are laid out in
+    // the new net order.
     std::vector<int> netOrder(d.num_nets);
     std::iota(netOrder.begin(), netOrder.end(), 0);
     std::stable_sort(netOrder.begin(), netOrder.end(), [&](int a, int b) {
         return d.net2pin_start[a + 1] - d.net2pin_start[a] <
                d.net2pin_start[b + 1] - d.net2pin_start[b];
     });
-    std::vector<int> newNetId(d.num_nets);
-    for (int k = 0; k < d.num_nets; ++k) newNetId[netOrder[k]] = k;
     std::vector<int> netStart(d.num_nets + 1, 0);
-    std::vector<int> slotNode(d.num_pins);
+    std::vector<int> slotNode(d.num_pins), pinSlot(d.num_pins, -1);
     std::vector<float> slotOffX(d.num_pins), slotOffY(d.num_pins);
     {
         int t = 0;
@@ -610,36 +607,51 @@ void Engine<T>::build(const sp_hip_design& d) {
                 slotNode[t] = d.pin2node[p];
                 slotOffX[t] = d.pin_offset_x[p];
                 slotOffY[t] = d.pin_offset_y[p];
+                pinSlot[p] = t;
             }
         }
         netStart[d.num_nets] = t;
     }
-    std::vector<int> small, large;
-    for (int k = 0; k < d.num_nets; ++k)
-        (netStart[k + 1] - netStart[k] > kLargeNet ? large : small).push_back(k);
-    dd_.nSmallNets = static_cast<int>(small.size());
-    dd_.nLargeNets = static_cast<int>(large.size());
+    // size classes: first net of degree > 2^c (class c holds (2^c, 2^(c+1)],
+    // class 0 holds degree 2), wavefronts per class
+    {
+        auto firstWithDegAbove = [&](int deg) {
+            int lo = 0, hi = d.num_nets;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (netStart[mid + 1] - netStart[mid] > deg) hi = mid; else lo = mid + 1;
+            }
+            return lo;
+        };
+        dd_.waNetLo[0] = firstWithDegAbove(1);
+        for (int c = 1; c <= kWaClasses; ++c) dd_.waNetLo[c] = firstWithDegAbove(1 << c);
+        dd_.waNetLo[kWaClasses + 1] = d.num_nets;
+        dd_.waWaveBase[0] = 0;
+        for (int c = 0; c < kWaClasses; ++c) {
+            const int perWave = 64 >> (c + 1);
+            const int nets = dd_.waNetLo[c + 1] - dd_.waNetLo[c];
+            dd_.waWaveBase[c + 1] = dd_.waWaveBase[c] + (nets + perWave - 1) / perWave;
+        }
+        dd_.waWaveBase[kWaClasses + 1] =
+            dd_.waWaveBase[kWaClasses] + (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]);
+    }
     dd_.netStart = static_cast<int*>(own(dupload<int>(netStart, s_)));
     dd_.slotNode = static_cast<int*>(own(dupload<int>(slotNode, s_)));
     dd_.slotOffX = static_cast<T*>(own(dupload<T>(slotOffX, s_)));
     dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_)));
-    dd_.smallNets = static_cast<int*>(own(dupload<int>(small, s_)));
-    dd_.largeNets = static_cast<int*>(own(dupload<int>(large, s_)));
-    // node-ordered pin entries (net ids in the new numbering)
+    // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
+    // of pins that no net pass writes (no net, degree < 2 or above
+    // ignore_net_degree) stay zero.
     std::vector<int> nodeStart(d.node2pin_start, d.node2pin_start + total + 1);
-    std::vector<int> npNet(d.num_pins);
-    std::vector<float> npOffX(d.num_pins), npOffY(d.num_pins);
-    for (int t = 0; t < d.num_pins; ++t) {
-        const int p = d.flat_node2pin[t];
-        npNet[t] = d.pin2net[p] >= 0 ? newNetId[d.pin2net[p]] : -1;
-        npOffX[t] = d.pin_offset_x[p];
-        npOffY[t] = d.pin_offset_y[p];
+    std::vector<int> slotNp(d.num_pins, 0);
+    for (int q = 0; q < d.num_pins; ++q) {
+        const int p = d.flat_node2pin[q];
+        if (pinSlot[p] >= 0) slotNp[pinSlot[p]] = q;
     }
     dd_.nodeStart = static_cast<int*>(own(dupload<int>(nodeStart, s_)));
-    dd_.npNet = static_cast<int*>(own(dupload<int>(npNet, s_)));
-    dd_.npOffX = static_cast<T*>(own(dupload<T>(npOffX, s_)));
-    dd_.npOffY = static_cast<T*>(own(dupload<T>(npOffY, s_)));
-    dd_.stats = static_cast<NetStat<T>*>(own(dalloc<NetStat<T>>(std::max(1, d.num_nets))));
+    dd_.slotNp = static_cast<int*>(own(dupload<int>(slotNp, s_)));
+    dd_.pinGrad = static_cast<T*>(own(dalloc<T>(2 * static_cast<size_t>(std::max(1, d.num_pins)))));
+    SPHIP_CHECK(hipMemsetAsync(dd_.pinGrad, 0, 2 * static_cast<size_t>(std::max(1, d.num_pins)) * sizeof(T), s_));
 
     nodeQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
     fillQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
@@ -649,8 +661,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     // reduction workspace: 4 regions of maxBlocks x 8 doubles (one block
     // partial per region, see ReduceWs)
     const long maxBlocks = std::max<long>({(static_cast<long>(nb_) + kBlock - 1) / kBlock,
-                                           (dd_.nSmallNets + kBlock - 1L) / kBlock,
-                                           (dd_.nLargeNets + 3L) / 4,
+                                           (dd_.waWaveBase[kWaClasses + 1] + 3L) / 4,
                                            (n_ + kBlock - 1L) / kBlock, 1L});
     ws_.maxBlocks = static_cast<int>(maxBlocks) + 1;
     ws_.partials = static_cast<double*>(own(dalloc<double>(4L * ws_.maxBlocks * 8)));
@@ -679,10 +690,14 @@ void Engine<T>::build(const sp_hip_design& d) {
     bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + ts + 8 * 2);
     bytesPer_[SP_HIP_K_POISSON] = 0;
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
-    bytesPer_[SP_HIP_K_WA_NETS] = P * (4 + 2 * ts) + P * 2 * ts + E * (4 + 12 * ts);
+    // WA net pass: per slot node id, offsets, the node's x/y, entry id and
+    // the gradient write; netStart per net
+    bytesPer_[SP_HIP_K_WA_NETS] = P * (4 + 2 * ts + 2 * ts + 4 + 2 * ts) + E * 4;
+    // node pass: per node its index, x/y, w/h, flags, pin range, the
+    // preconditioner terms, u/up/gp reads and the grad write, four field bins
+    // (x and y); per pin entry its gradient
     bytesPer_[SP_HIP_K_NODE_GRAD] =
-        n_ * (4 + 2 * ts + 2 * ts + 1 + 8 + 4 + 4 + 6 * 2 * ts) + P * (4 + 2 * ts + 12 * ts) +
-        n_ * 4.0 * 2 * ts;
+        n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 8 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
     bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 2 * 2 * ts * 3 + 2 * 2 * ts + 2 * ts);
 }
 

@@ -7,18 +7,10 @@
 
 namespace sphip {
 
-// Per-net WA state written by pass 1 and read by the node pass
-// (wa_wirelength.cpp:130-167): extrema, b+-, centred c'+- for x and y.
-template <typename T>
-struct alignas(16) NetStat {
-    T mxX, mnX, mxY, mnY, bPX, bNX, cPX, cNX, bPY, bNY, cPY, cNY;
-};
-
 template <typename T>
 struct DevDesign {
     int total = 0, numMov = 0, numNodes = 0;
     int nOpt = 0, nDens = 0, numNets = 0, numPins = 0;
-    int nSmallNets = 0, nLargeNets = 0;
     T coreLx, coreLy, coreHx, coreHy;
     // nodes
     T* x = nullptr;          // current evaluation point, all nodes
@@ -37,22 +29,27 @@ struct DevDesign {
     int* tileCursor = nullptr;    // kTileKeys insertion cursors
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
-    // net -> pin slots (net order)
+    // net -> pin slots.  Nets are renumbered by degree (ascending), so each
+    // WA size class below is a contiguous range of nets and of slots.
     const int* netStart = nullptr;
     const int* slotNode = nullptr;
     const T* slotOffX = nullptr;
     const T* slotOffY = nullptr;
-    const int* smallNets = nullptr;   // nets of degree <= kLargeNet (thread per net)
-    const int* largeNets = nullptr;   // nets of degree > kLargeNet (wave per net)
-    // node -> pin entries
+    const int* slotNp = nullptr;      // node-pin entry of each slot (where its gradient goes)
+    // WA size classes: class c < kWaClasses holds the nets of degree in
+    // (2^c, 2^(c+1)] (class 0: degree 2), 2^(c+1) lanes per net; class
+    // kWaClasses the nets of degree > 2^kWaClasses, one wavefront each.
+    // Nets of class c: [waNetLo[c], waNetLo[c+1]); its wavefronts start at
+    // waWaveBase[c]; waWaveBase[kWaClasses + 1] is the total.
+    int waNetLo[8];
+    int waWaveBase[8];
+    // node -> pin entries; pinGrad holds the WA gradient (x, y) of each entry,
+    // written by the net pass (zero for pins of nets it skips)
     const int* nodeStart = nullptr;
-    const int* npNet = nullptr;       // net of each node-pin entry (-1 = none)
-    const T* npOffX = nullptr;
-    const T* npOffY = nullptr;
-    NetStat<T>* stats = nullptr;
+    T* pinGrad = nullptr;
 };
 
-constexpr int kLargeNet = 32;
+constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
 constexpr int kTileBins = 16;        // bins per tile side for the spatial sort
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
 constexpr int kScatterChunk = 256;   // density nodes per scatter block (1 per thread)
@@ -61,8 +58,8 @@ constexpr int kWindowBins = 1024;    // LDS window of the scatter block (per gri
 // FP64 reductions: every reducing kernel writes one partial per block (plain
 // stores, no atomics, no fences); k_reduce sums each region in block order
 // after the node pass.  Regions and their values per block:
-//   0 finalize (tau numerator), 1 WA small nets (WL, HPWL),
-//   2 WA large nets (WL, HPWL), 3 node pass (7 sums).
+//   0 finalize (tau numerator), 1 WA net pass (WL, HPWL), 2 unused,
+//   3 node pass (7 sums).
 struct ReduceWs {
     double* partials = nullptr;   // 4 regions of maxBlocks * 8 doubles
     double* scalars = nullptr;    // EvalScalars on the device

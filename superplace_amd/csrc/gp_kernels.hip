@@ -361,54 +361,6 @@ __device__ __host__ inline T invGammaOf(float gamma) {
 template <typename T>
 __device__ inline T tmax() { return sizeof(T) == 4 ? T(3.402823466e+38f) : T(1.7976931348623157e308); }
 
-// One net per thread (degree <= kLargeNet): extrema, max-shifted exponentials
-// and centred sums of WaWirelength::compute pass 1 (wa_wirelength.cpp:119-173);
-// the HPWL of the net (hpwl.cpp:20-37) from the same extrema.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_wa_small(DevDesign<T> d, T invG, int ignoreDeg,
-                                                     double* partials, const Ctl* ctl) {
-    if (ctl) {
-        if (ctl->done) return;
-        invG = invGammaOf<T>(ctl->gamma);
-    }
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (s < d.nSmallNets) {
-        const int k = d.smallNets[s];
-        const int b = d.netStart[k], e = d.netStart[k + 1];
-        const int deg = e - b;
-        NetStat<T> st{};
-        if (deg >= 2) {
-            T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
-            for (int t = b; t < e; ++t) {
-                const int nd = d.slotNode[t];
-                const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
-                mxX = fmax(mxX, px); mnX = fmin(mnX, px);
-                mxY = fmax(mxY, py); mnY = fmin(mnY, py);
-            }
-            v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
-            if (deg <= ignoreDeg) {
-                T bpx = 0, bnx = 0, cpx = 0, cnx = 0, bpy = 0, bny = 0, cpy = 0, cny = 0;
-                for (int t = b; t < e; ++t) {
-                    const int nd = d.slotNode[t];
-                    const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
-                    const T dxp = px - mxX, dxn = px - mnX;
-                    const T ap = texp<T>(dxp * invG), an = texp<T>(-dxn * invG);
-                    bpx += ap; cpx += dxp * ap; bnx += an; cnx += dxn * an;
-                    const T dyp = py - mxY, dyn = py - mnY;
-                    const T bp = texp<T>(dyp * invG), bn = texp<T>(-dyn * invG);
-                    bpy += bp; cpy += dyp * bp; bny += bn; cny += dyn * bn;
-                }
-                st = NetStat<T>{mxX, mnX, mxY, mnY, bpx, bnx, cpx, cnx, bpy, bny, cpy, cny};
-                v[0] = static_cast<double>((mxX - mnX) + cpx / bpx - cnx / bnx) +
-                       static_cast<double>((mxY - mnY) + cpy / bpy - cny / bny);
-            }
-        }
-        d.stats[k] = st;   // bPX == 0 marks a skipped net (wa_wirelength.cpp:123-128)
-    }
-    blockPartial<2>(v, partials);
-}
-
 template <typename T>
 __device__ inline T waveMax(T v) {
 #pragma unroll
@@ -427,63 +379,179 @@ __device__ inline T waveSum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+// Reductions over aligned groups of W lanes (xor offsets stay in the group).
+template <int W, typename T>
+__device__ inline T groupMax(T v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <int W, typename T>
+__device__ inline T groupMin(T v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <int W, typename T>
+__device__ inline T groupSum(T v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
-// One net per wavefront (degree > kLargeNet, the power-law tail): the same
-// quantities with lane-strided pins and shuffle trees.
+// Sums of one net over its pins, after the extrema (wa_wirelength.cpp:130-167):
+// b+- = sum a+-, c'+- = sum (p - extremum) a+-, per axis.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_wa_large(DevDesign<T> d, T invG, int ignoreDeg,
-                                                     double* partials, const Ctl* ctl) {
+struct WaSums {
+    T bpx, bnx, cpx, cnx, bpy, bny, cpy, cny;
+};
+
+// WA gradient of one pin from its exponentials and the net's sums: the
+// derivative of the max term minus that of the min term
+// (wa_wirelength.cpp:198-218), both axes.
+template <typename T>
+__device__ inline void pinGradOf(const WaSums<T>& s, T dxp, T dxn, T ax, T anx, T dyp, T dyn,
+                                 T ay, T any, T invG, T& gx, T& gy) {
+    const T tMaxX = ax * (s.bpx + (dxp * s.bpx - s.cpx) * invG) / (s.bpx * s.bpx);
+    const T tMinX = anx * (s.bnx + (s.cnx - dxn * s.bnx) * invG) / (s.bnx * s.bnx);
+    const T tMaxY = ay * (s.bpy + (dyp * s.bpy - s.cpy) * invG) / (s.bpy * s.bpy);
+    const T tMinY = any * (s.bny + (s.cny - dyn * s.bny) * invG) / (s.bny * s.bny);
+    gx = tMaxX - tMinX;
+    gy = tMaxY - tMinY;
+}
+
+template <typename T>
+__device__ inline double waOf(T mxX, T mnX, T mxY, T mnY, const WaSums<T>& s) {
+    return static_cast<double>((mxX - mnX) + s.cpx / s.bpx - s.cnx / s.bnx) +
+           static_cast<double>((mxY - mnY) + s.cpy / s.bpy - s.cny / s.bny);
+}
+
+// Size class with W = 2^LW lanes per net: one pin per lane, so a pin is
+// loaded once and its exponentials are computed once, for both the net's
+// sums and the pin's gradient.  Lanes past the degree carry neutral values.
+template <int LW, typename T>
+__device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lane, T invG,
+                                   int ignoreDeg, double (&v)[2]) {
+    constexpr int W = 1 << LW;
+    const int net = d.waNetLo[c] + ((gw - d.waWaveBase[c]) << (6 - LW)) + (lane >> LW);
+    const int j = lane & (W - 1);
+    int t = 0, deg = 0;
+    if (net < d.waNetLo[c + 1]) {
+        t = d.netStart[net];
+        deg = d.netStart[net + 1] - t;
+        t += j;
+    }
+    const bool pin = j < deg;
+    T px = 0, py = 0;
+    if (pin) {
+        const int nd = d.slotNode[t];
+        px = d.x[nd] + d.slotOffX[t];
+        py = d.y[nd] + d.slotOffY[t];
+    }
+    const T mxX = groupMax<W>(pin ? px : -tmax<T>()), mnX = groupMin<W>(pin ? px : tmax<T>());
+    const T mxY = groupMax<W>(pin ? py : -tmax<T>()), mnY = groupMin<W>(pin ? py : tmax<T>());
+    const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
+    const T ax = pin ? texp<T>(dxp * invG) : T(0), anx = pin ? texp<T>(-dxn * invG) : T(0);
+    const T ay = pin ? texp<T>(dyp * invG) : T(0), any = pin ? texp<T>(-dyn * invG) : T(0);
+    WaSums<T> s;
+    s.bpx = groupSum<W>(ax); s.cpx = groupSum<W>(dxp * ax);
+    s.bnx = groupSum<W>(anx); s.cnx = groupSum<W>(dxn * anx);
+    s.bpy = groupSum<W>(ay); s.cpy = groupSum<W>(dyp * ay);
+    s.bny = groupSum<W>(any); s.cny = groupSum<W>(dyn * any);
+    const bool wa = deg >= 2 && deg <= ignoreDeg;
+    if (j == 0 && deg >= 2) {
+        v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+        if (wa) v[0] = waOf<T>(mxX, mnX, mxY, mnY, s);
+    }
+    if (wa && pin) {
+        T gx, gy;
+        pinGradOf<T>(s, dxp, dxn, ax, anx, dyp, dyn, ay, any, invG, gx, gy);
+        const int q = d.slotNp[t];
+        d.pinGrad[2L * q] = gx;
+        d.pinGrad[2L * q + 1] = gy;
+    }
+}
+
+// Nets above the largest class (the power-law tail): one wavefront per net,
+// lane-strided pins, shuffle trees; the gradient loop recomputes the pins'
+// exponentials.
+template <typename T>
+__device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T invG, int ignoreDeg,
+                                  double (&v)[2]) {
+    const int net = d.waNetLo[kWaClasses] + (gw - d.waWaveBase[kWaClasses]);
+    if (net >= d.waNetLo[kWaClasses + 1]) return;
+    const int b = d.netStart[net], e = d.netStart[net + 1];
+    T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+    for (int t = b + lane; t < e; t += 64) {
+        const int nd = d.slotNode[t];
+        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        mxX = fmax(mxX, px); mnX = fmin(mnX, px);
+        mxY = fmax(mxY, py); mnY = fmin(mnY, py);
+    }
+    mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
+    if (lane == 0) v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+    if (e - b > ignoreDeg) return;
+    WaSums<T> s{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = b + lane; t < e; t += 64) {
+        const int nd = d.slotNode[t];
+        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
+        const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
+        const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
+        s.bpx += ax; s.cpx += dxp * ax; s.bnx += anx; s.cnx += dxn * anx;
+        s.bpy += ay; s.cpy += dyp * ay; s.bny += any; s.cny += dyn * any;
+    }
+    s.bpx = waveSum(s.bpx); s.cpx = waveSum(s.cpx); s.bnx = waveSum(s.bnx); s.cnx = waveSum(s.cnx);
+    s.bpy = waveSum(s.bpy); s.cpy = waveSum(s.cpy); s.bny = waveSum(s.bny); s.cny = waveSum(s.cny);
+    if (lane == 0) v[0] = waOf<T>(mxX, mnX, mxY, mnY, s);
+    for (int t = b + lane; t < e; t += 64) {
+        const int nd = d.slotNode[t];
+        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
+        const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
+        const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
+        T gx, gy;
+        pinGradOf<T>(s, dxp, dxn, ax, anx, dyp, dyn, ay, any, invG, gx, gy);
+        const int q = d.slotNp[t];
+        d.pinGrad[2L * q] = gx;
+        d.pinGrad[2L * q + 1] = gy;
+    }
+}
+
+// WaWirelength::compute (wa_wirelength.cpp:119-225) as one pin-parallel pass
+// over the net-ordered slots: per net the extrema, max-shifted exponentials,
+// centred sums and WA value (degree 2..ignoreDeg), the HPWL of every net of
+// degree >= 2 (hpwl.cpp:20-37), and each pin's gradient into pinGrad at the
+// pin's node-pin entry.  Each wavefront belongs to one size class.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_wa_pins(DevDesign<T> d, T invG, int ignoreDeg,
+                                                    double* partials, const Ctl* ctl) {
     if (ctl) {
         if (ctl->done) return;
         invG = invGammaOf<T>(ctl->gamma);
     }
     const int lane = threadIdx.x & 63;
-    const int s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int gw = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     double v[2] = {0.0, 0.0};
-    if (s < d.nLargeNets) {
-        const int k = d.largeNets[s];
-        const int b = d.netStart[k], e = d.netStart[k + 1];
-        const int deg = e - b;
-        T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
-        for (int t = b + lane; t < e; t += 64) {
-            const int nd = d.slotNode[t];
-            const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
-            mxX = fmax(mxX, px); mnX = fmin(mnX, px);
-            mxY = fmax(mxY, py); mnY = fmin(mnY, py);
-        }
-        mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
-        NetStat<T> st{};
-        double wl = 0.0;
-        if (deg <= ignoreDeg) {
-            T bpx = 0, bnx = 0, cpx = 0, cnx = 0, bpy = 0, bny = 0, cpy = 0, cny = 0;
-            for (int t = b + lane; t < e; t += 64) {
-                const int nd = d.slotNode[t];
-                const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
-                const T dxp = px - mxX, dxn = px - mnX;
-                const T ap = texp<T>(dxp * invG), an = texp<T>(-dxn * invG);
-                bpx += ap; cpx += dxp * ap; bnx += an; cnx += dxn * an;
-                const T dyp = py - mxY, dyn = py - mnY;
-                const T bp = texp<T>(dyp * invG), bn = texp<T>(-dyn * invG);
-                bpy += bp; cpy += dyp * bp; bny += bn; cny += dyn * bn;
-            }
-            bpx = waveSum(bpx); bnx = waveSum(bnx); cpx = waveSum(cpx); cnx = waveSum(cnx);
-            bpy = waveSum(bpy); bny = waveSum(bny); cpy = waveSum(cpy); cny = waveSum(cny);
-            st = NetStat<T>{mxX, mnX, mxY, mnY, bpx, bnx, cpx, cnx, bpy, bny, cpy, cny};
-            wl = static_cast<double>((mxX - mnX) + cpx / bpx - cnx / bnx) +
-                 static_cast<double>((mxY - mnY) + cpy / bpy - cny / bny);
-        }
-        if (lane == 0) {
-            d.stats[k] = st;
-            v[0] = wl;
-            v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
-        }
+    int c = 0;
+    while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
+    switch (c) {
+        case 0: waGroupPass<1, T>(d, 0, gw, lane, invG, ignoreDeg, v); break;
+        case 1: waGroupPass<2, T>(d, 1, gw, lane, invG, ignoreDeg, v); break;
+        case 2: waGroupPass<3, T>(d, 2, gw, lane, invG, ignoreDeg, v); break;
+        case 3: waGroupPass<4, T>(d, 3, gw, lane, invG, ignoreDeg, v); break;
+        case 4: waGroupPass<5, T>(d, 4, gw, lane, invG, ignoreDeg, v); break;
+        case 5: waGroupPass<6, T>(d, 5, gw, lane, invG, ignoreDeg, v); break;
+        case kWaClasses: waWavePass<T>(d, gw, lane, invG, ignoreDeg, v); break;
+        default: break;
     }
     blockPartial<2>(v, partials);
 }
 
 // ------------------------------------------------------------------ (c)+(d) node pass
 // For each optimised node k (node i = optIdx[k]):
-//  * WA gradient over the node's pins (wa_wirelength.cpp:183-225),
+//  * WA gradient: the sum of its pins' gradients, in node-pin order
+//    (wa_wirelength.cpp:183-225),
 //  * density force F_i = sum_bins q * xi (bin_grid.cpp:395-426),
 //  * preconditioned gradient grad = (dWL - lambda F) / max(1, deg + lambda area)
 //    (eplace.cpp:162-185),
@@ -513,29 +581,11 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
         const T xi = d.x[i], yi = d.y[i];
         T gx = 0, gy = 0;
         const bool hasWl = i < d.numMov;
-        if (hasWl) {
+        if (hasWl)
             for (int t = d.nodeStart[i]; t < d.nodeStart[i + 1]; ++t) {
-                const int net = d.npNet[t];
-                if (net < 0) continue;
-                const NetStat<T> st = d.stats[net];
-                if (st.bPX <= T(0)) continue;
-                const T px = xi + d.npOffX[t], py = yi + d.npOffY[t];
-                {
-                    const T dp = px - st.mxX, dn = px - st.mnX;
-                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
-                    const T tMax = ap * (st.bPX + (dp * st.bPX - st.cPX) * invG) / (st.bPX * st.bPX);
-                    const T tMin = an * (st.bNX + (st.cNX - dn * st.bNX) * invG) / (st.bNX * st.bNX);
-                    gx += tMax - tMin;
-                }
-                {
-                    const T dp = py - st.mxY, dn = py - st.mnY;
-                    const T ap = texp<T>(dp * invG), an = texp<T>(-dn * invG);
-                    const T tMax = ap * (st.bPY + (dp * st.bPY - st.cPY) * invG) / (st.bPY * st.bPY);
-                    const T tMin = an * (st.bNY + (st.cNY - dn * st.bNY) * invG) / (st.bNY * st.bNY);
-                    gy += tMax - tMin;
-                }
+                gx += d.pinGrad[2L * t];
+                gy += d.pinGrad[2L * t + 1];
             }
-        }
         T fx = 0, fy = 0;
         const uint8_t f = d.flags[i];
         if (!(f & (F_FIXED | F_NI))) {
@@ -876,14 +926,11 @@ template <typename T>
 void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& ws,
                   const Ctl* ctl, hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
-    ws.blocks[1] = static_cast<int>(gridFor(d.nSmallNets > 0 ? d.nSmallNets : 1));
-    k_wa_small<T><<<ws.blocks[1], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(1), ctl);
+    const int waves = d.waWaveBase[kWaClasses + 1];
+    ws.blocks[1] = std::max(1, (waves + kBlock / 64 - 1) / (kBlock / 64));
+    ws.blocks[2] = 0;
+    k_wa_pins<T><<<ws.blocks[1], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(1), ctl);
     SPHIP_CHECK(hipGetLastError());
-    ws.blocks[2] = (d.nLargeNets + 3) / 4;
-    if (d.nLargeNets > 0) {
-        k_wa_large<T><<<ws.blocks[2], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(2), ctl);
-        SPHIP_CHECK(hipGetLastError());
-    }
 }
 
 template <typename T>
