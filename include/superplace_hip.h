@@ -178,10 +178,15 @@ typedef struct sp_hip_gp_info {
     double movable_area_scaled;   /* tau denominator (bin_grid.cpp:182-188) */
 } sp_hip_gp_info;
 
-/* Per kernel family: total ms and launches measured with HIP events on the
- * engine's stream (enabled by sp_hip_gp_set_profiling). */
+/* Kernel families timed with HIP events on the engine's stream in the
+ * device-driven loop (enabled by sp_hip_gp_set_profiling, which makes
+ * loop_mode 2 launch its slots directly, as loop_mode 1 does): the family's
+ * first kernel records the start event as its dispatch begins and its last
+ * kernel the stop event as it ends (hipExtLaunchKernel).  Each family is one kernel, except POISSON (the
+ * transform products of one solve) and SORT (the two passes of the spatial
+ * sort). */
 enum { SP_HIP_K_SCATTER = 0, SP_HIP_K_FINALIZE, SP_HIP_K_POISSON, SP_HIP_K_WA_NETS,
-       SP_HIP_K_NODE_GRAD, SP_HIP_K_UPDATE, SP_HIP_K_COUNT };
+       SP_HIP_K_NODE_GRAD, SP_HIP_K_UPDATE, SP_HIP_K_SORT, SP_HIP_K_REDUCE, SP_HIP_K_COUNT };
 
 /* Builds the device copy of the design and the bin grid for one stage
  * (EPlace::EPlace + BinGrid::initialize, eplace.cpp:205-221). */

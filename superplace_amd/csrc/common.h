@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <stdexcept>
@@ -51,5 +52,42 @@ struct GridGeom {
     T stepX, stepY, originX, originY;
     T target;       // target density (charge weight of macros)
 };
+
+// Optional event pair timing the kernels of one family: while set, the first
+// kernel launched records `start` as it begins and every kernel records
+// `stop` as it ends (hipExtLaunchKernel: the timestamps of the dispatch
+// itself, without the launch gaps).  Only for launches outside graph capture.
+struct LaunchTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+inline thread_local LaunchTimer* tlsTimer = nullptr;
+
+template <typename... Params, typename... Args>
+inline void launchKernel(void (*kernel)(Params...), dim3 grid, dim3 block, uint32_t shm,
+                         hipStream_t s, Args... args) {
+    if (LaunchTimer* t = tlsTimer) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, s, t->start, t->stop, 0,
+                              static_cast<Params>(args)...);
+        t->start = nullptr;
+    } else {
+        kernel<<<grid, block, shm, s>>>(static_cast<Params>(args)...);
+    }
+}
+
+// Pairs stored interleaved (x0, y0, x1, y1, ...) and moved as one vector access.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+template <typename T>
+__host__ __device__ inline typename Vec2<T>::type load2(const T* p, long i) {
+    return reinterpret_cast<const typename Vec2<T>::type*>(p)[i];
+}
+template <typename T>
+__host__ __device__ inline void store2(T* p, long i, T a, T b) {
+    typename Vec2<T>::type v;
+    v.x = a;
+    v.y = b;
+    reinterpret_cast<typename Vec2<T>::type*>(p)[i] = v;
+}
 
 }  // namespace sphip

@@ -195,24 +195,14 @@ public:
 
     void positions(float* x, float* y) const override {
         std::vector<T> hx(total_), hy(total_);
-        SPHIP_CHECK(hipMemcpyAsync(hx.data(), dd_.x, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
-        SPHIP_CHECK(hipMemcpyAsync(hy.data(), dd_.y, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
-        SPHIP_CHECK(hipStreamSynchronize(s_));
+        downloadXY(hx.data(), hy.data());
         for (int i = 0; i < total_; ++i) {
             x[i] = static_cast<float>(hx[i]);
             y[i] = static_cast<float>(hy[i]);
         }
     }
 
-    void setProfiling(int mask) override {
-        profMask_ = mask;
-        if (mask && !evStart_[0])
-            for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
-                SPHIP_CHECK(hipEventCreate(&evStart_[f]));
-                SPHIP_CHECK(hipEventCreate(&evStop_[f]));
-            }
-    }
-
+    void setProfiling(int mask) override { profMask_ = mask; }
     void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const override {
         for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
             if (ms) ms[f] = kms_[f];
@@ -226,8 +216,7 @@ public:
     void evalHost(const float* x, const float* y, float lambda, float gamma,
                   sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) override {
         std::vector<T> hx(x, x + total_), hy(y, y + total_);
-        SPHIP_CHECK(hipMemcpyAsync(dd_.x, hx.data(), total_ * sizeof(T), hipMemcpyHostToDevice, s_));
-        SPHIP_CHECK(hipMemcpyAsync(dd_.y, hy.data(), total_ * sizeof(T), hipMemcpyHostToDevice, s_));
+        uploadXY(hx.data(), hy.data());
         std::vector<T> pos(2 * static_cast<size_t>(n_));
         for (int k = 0; k < n_; ++k) {
             pos[k] = hx[optIdxHost_[k]];
@@ -235,7 +224,6 @@ public:
         }
         SPHIP_CHECK(hipMemcpyAsync(u_, pos.data(), pos.size() * sizeof(T), hipMemcpyHostToDevice, s_));
         curPos_ = nullptr;   // x/y now hold the raw positions, not a clamped pos buffer
-        sortNext_ = true;
         const bool wantDbg = dbg != nullptr;
         long long *dNode = nullptr, *dFill = nullptr;
         T *dForce = nullptr, *dWl = nullptr;
@@ -305,7 +293,28 @@ private:
     void runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows, float lambda,
                    float gamma, double alpha0, double effRef, float targetTau, int maxIter,
                    int maxBt, float stepX);
-    void launchSlot(bool events, bool sort);
+    void launchSlot(bool events, int slot);
+    // node coordinates <-> host arrays (the device keeps them interleaved)
+    void downloadXY(T* hx, T* hy) const {
+        std::vector<T> xy(2 * static_cast<size_t>(total_));
+        SPHIP_CHECK(hipMemcpyAsync(xy.data(), dd_.xy, xy.size() * sizeof(T), hipMemcpyDeviceToHost, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+        for (int i = 0; i < total_; ++i) {
+            hx[i] = xy[2 * i];
+            hy[i] = xy[2 * i + 1];
+        }
+    }
+    void uploadXY(const T* hx, const T* hy) {
+        xyStage_.resize(2 * static_cast<size_t>(total_));
+        for (int i = 0; i < total_; ++i) {
+            xyStage_[2 * i] = hx[i];
+            xyStage_[2 * i + 1] = hy[i];
+        }
+        SPHIP_CHECK(hipMemcpyAsync(dd_.xy, xyStage_.data(), xyStage_.size() * sizeof(T),
+                                   hipMemcpyHostToDevice, s_));
+        SPHIP_CHECK(hipStreamSynchronize(s_));
+    }
+    std::vector<T> xyStage_;
     void build(const sp_hip_design& d);
     void release();
     EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
@@ -313,28 +322,48 @@ private:
                          T* dbgForce = nullptr, T* dbgWl = nullptr);
     void scatterPos(const T* pos);   // clamp + write into node coordinates
     double deliveredMetrics(double* hpwlOut);
-    void beginK(int f) {
-        if (profMask_ & (1 << f)) SPHIP_CHECK(hipEventRecord(evStart_[f], s_));
+    // HIP event pairs per kernel family and graph slot (created on demand)
+    void ensureEvents(int slots) {
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f)
+            while (static_cast<int>(evStart_[f].size()) < slots) {
+                hipEvent_t a = nullptr, b = nullptr;
+                SPHIP_CHECK(hipEventCreate(&a));
+                evStart_[f].push_back(a);
+                SPHIP_CHECK(hipEventCreate(&b));
+                evStop_[f].push_back(b);
+                // record once on the stream: the runtime times an event pair
+                // recorded by graph nodes only after a direct record
+                SPHIP_CHECK(hipEventRecord(a, s_));
+                SPHIP_CHECK(hipEventRecord(b, s_));
+            }
+        SPHIP_CHECK(hipStreamSynchronize(s_));
     }
-    void endK(int f) {
+    // The kernels of family f launched between beginK and endK carry the
+    // slot's event pair (launchKernel / hipExtLaunchKernel).
+    void beginK(int f, int slot) {
         if (profMask_ & (1 << f)) {
-            SPHIP_CHECK(hipEventRecord(evStop_[f], s_));
-            pendingK_[f] = true;
+            timer_ = LaunchTimer{evStart_[f][slot], evStop_[f][slot]};
+            tlsTimer = &timer_;
         }
     }
-    void harvestK() {
-        if (!profMask_) return;
-        for (int f = 0; f < SP_HIP_K_COUNT; ++f)
-            if (pendingK_[f]) {
+    void endK(int f, int /*slot*/) {
+        if (profMask_ & (1 << f)) tlsTimer = nullptr;
+    }
+    LaunchTimer timer_;
+    // Adds the durations of slots [0, nslots) of every profiled family.
+    void harvestK(int mask, int nslots) {
+        for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
+            if (!(mask & (1 << f))) continue;
+            for (int i = 0; i < nslots; ++i) {
                 float ms = 0;
-                SPHIP_CHECK(hipEventSynchronize(evStop_[f]));
-                SPHIP_CHECK(hipEventElapsedTime(&ms, evStart_[f], evStop_[f]));
+                SPHIP_CHECK(hipEventSynchronize(evStop_[f][i]));
+                SPHIP_CHECK(hipEventElapsedTime(&ms, evStart_[f][i], evStop_[f][i]));
                 kms_[f] += ms;
                 klaunch_[f] += 1;
                 kbytes_[f] += bytesPer_[f];
                 kflops_[f] += flopsPer_[f];
-                pendingK_[f] = false;
             }
+        }
     }
 
     sp_hip_gp_config cfg_;
@@ -369,13 +398,8 @@ private:
     int* hFlag_ = nullptr;   // pinned copy of ctl->done
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
-    int graphEvMask_ = 0;    // families timed by event nodes inside the graph
-    static constexpr int kSortEvery = 1;   // spatial re-sort period (evaluations)
-    long evalCount_ = 0;
-    bool sortNext_ = true;
     int profMask_ = 0;   // bit f: time kernel family f with HIP events
-    hipEvent_t evStart_[SP_HIP_K_COUNT] = {}, evStop_[SP_HIP_K_COUNT] = {};
-    bool pendingK_[SP_HIP_K_COUNT] = {};
+    std::vector<hipEvent_t> evStart_[SP_HIP_K_COUNT], evStop_[SP_HIP_K_COUNT];
     double kms_[SP_HIP_K_COUNT] = {}, kbytes_[SP_HIP_K_COUNT] = {}, kflops_[SP_HIP_K_COUNT] = {};
     int64_t klaunch_[SP_HIP_K_COUNT] = {};
     double bytesPer_[SP_HIP_K_COUNT] = {}, flopsPer_[SP_HIP_K_COUNT] = {};
@@ -400,9 +424,10 @@ void Engine<T>::release() {
     if (hScalars_) (void)hipHostFree(hScalars_);
     hScalars_ = nullptr;
     for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
-        if (evStart_[f]) (void)hipEventDestroy(evStart_[f]);
-        if (evStop_[f]) (void)hipEventDestroy(evStop_[f]);
-        evStart_[f] = evStop_[f] = nullptr;
+        for (hipEvent_t ev : evStart_[f]) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : evStop_[f]) (void)hipEventDestroy(ev);
+        evStart_[f].clear();
+        evStop_[f].clear();
     }
     if (s_) (void)hipStreamDestroy(s_);
     s_ = nullptr;
@@ -560,8 +585,14 @@ void Engine<T>::build(const sp_hip_design& d) {
     dd_.coreHx = T(d.core_hx);
     dd_.coreHy = T(d.core_hy);
     std::vector<float> xs(d.node_x, d.node_x + total), ys(d.node_y, d.node_y + total);
-    dd_.x = static_cast<T*>(own(dupload<T>(xs, s_)));
-    dd_.y = static_cast<T*>(own(dupload<T>(ys, s_)));
+    {
+        std::vector<float> xy(2 * static_cast<size_t>(total));
+        for (int i = 0; i < total; ++i) {
+            xy[2 * i] = xs[i];
+            xy[2 * i + 1] = ys[i];
+        }
+        dd_.xy = static_cast<T*>(own(dupload<T>(xy, s_)));
+    }
     std::vector<float> ws(d.node_w, d.node_w + total), hs(d.node_h, d.node_h + total);
     wHost_.assign(ws.begin(), ws.end());
     hHost_.assign(hs.begin(), hs.end());
@@ -570,12 +601,42 @@ void Engine<T>::build(const sp_hip_design& d) {
     std::vector<uint8_t> fl(d.node_flags, d.node_flags + total);
     dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_)));
     dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_)));
-    dd_.densIdx = static_cast<int*>(own(dupload<int>(dens, s_)));
-    dd_.densA = static_cast<int*>(own(dupload<int>(dens, s_)));
-    dd_.densB = static_cast<int*>(own(dupload<int>(dens, s_)));
-    dd_.tileCount = static_cast<int*>(own(dalloc<int>(kTileKeys)));
-    dd_.tileCursor = static_cast<int*>(own(dalloc<int>(kTileKeys)));
-    SPHIP_CHECK(hipMemsetAsync(dd_.tileCount, 0, kTileKeys * sizeof(int), s_));
+    {
+        // density records in node order to start with; the first sort orders them
+        std::vector<int> recId(dens.size());
+        std::vector<float> recWH(2 * dens.size());
+        for (size_t t = 0; t < dens.size(); ++t) {
+            const int i = dens[t];
+            if (i > kRecNode) throw InvalidArg("too many nodes");
+            recId[t] = i | ((d.node_flags[i] & F_FILLER) ? kRecFiller : 0) |
+                       ((d.node_flags[i] & F_MACRO) ? kRecMacro : 0);
+            recWH[2 * t] = d.node_w[i];
+            recWH[2 * t + 1] = d.node_h[i];
+        }
+        dd_.recIdA = static_cast<int*>(own(dupload<int>(recId, s_)));
+        dd_.recIdB = static_cast<int*>(own(dupload<int>(recId, s_)));
+        dd_.recWHA = static_cast<T*>(own(dupload<T>(recWH, s_)));
+        dd_.recWHB = static_cast<T*>(own(dupload<T>(recWH, s_)));
+        dd_.recXY = static_cast<T*>(own(dalloc<T>(2 * std::max<size_t>(dens.size(), 1))));
+    }
+    {
+        TileGrid& tg = dd_.tiles;
+        tg.tilesX = tg.tilesY = (dim_ + kTileBins - 1) / kTileBins;
+        int side = 1;   // Morton keys of tiles below side x side, side a power of two
+        while (side < tg.tilesX) side <<= 1;
+        tg.nKeys = side * side;
+        if (tg.nKeys > kTileKeys) throw InvalidArg("bin grid wider than 1024 bins");
+        tg.count = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
+        tg.fill = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
+        tg.start = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
+        tg.extraStart = static_cast<int*>(own(dalloc<int>(tg.nKeys + 1)));
+        const size_t stageLen = static_cast<size_t>(tg.tilesX) * tg.tilesY * 2 * kTileWin * kTileWin;
+        tg.stage = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(stageLen)));
+        for (int* a : {tg.count, tg.fill, tg.start})
+            SPHIP_CHECK(hipMemsetAsync(a, 0, tg.nKeys * sizeof(int), s_));
+        SPHIP_CHECK(hipMemsetAsync(tg.extraStart, 0, (tg.nKeys + 1) * sizeof(int), s_));
+        SPHIP_CHECK(hipMemsetAsync(tg.stage, 0, stageLen * 8, s_));
+    }
     std::vector<double> parea(n_);
     std::vector<float> pdeg(n_);
     for (int k = 0; k < n_; ++k) {
@@ -585,14 +646,28 @@ void Engine<T>::build(const sp_hip_design& d) {
     }
     dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_)));
     dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_)));
-    // Nets are renumbered by degree (stable), so each WA size class is a
-    // contiguous range of nets and of pin slots; pin slots are laid out in
-    // the new net order.
+    // Nets are renumbered by WA size class (degree 2, 3-4, 5-8, ..., then
+    // larger), so each class is a contiguous range of nets and of pin slots,
+    // and inside a class by their lowest node index, so that neighbouring
+    // nets gather coordinates of neighbouring nodes (netlists are local in
+    // node order).  Pin slots are laid out in the new net order.
+    auto degOf = [&](int k) { return d.net2pin_start[k + 1] - d.net2pin_start[k]; };
+    auto classOf = [&](int deg) {
+        int c = 0;
+        while (c < kWaClasses && deg > (2 << c)) ++c;
+        return deg < 2 ? -1 : c;
+    };
+    std::vector<int> netMinNode(d.num_nets, 0x7FFFFFFF);
+    for (int k = 0; k < d.num_nets; ++k)
+        for (int q = d.net2pin_start[k]; q < d.net2pin_start[k + 1]; ++q)
+            netMinNode[k] = std::min(netMinNode[k], d.pin2node[d.flat_net2pin[q]]);
     std::vector<int> netOrder(d.num_nets);
     std::iota(netOrder.begin(), netOrder.end(), 0);
     std::stable_sort(netOrder.begin(), netOrder.end(), [&](int a, int b) {
-        return d.net2pin_start[a + 1] - d.net2pin_start[a] <
-               d.net2pin_start[b + 1] - d.net2pin_start[b];
+        const int ca = classOf(degOf(a)), cb = classOf(degOf(b));
+        if (ca != cb) return ca < cb;
+        if (ca == kWaClasses) return degOf(a) < degOf(b);
+        return netMinNode[a] < netMinNode[b];
     });
     std::vector<int> netStart(d.num_nets + 1, 0);
     std::vector<int> slotNode(d.num_pins), pinSlot(d.num_pins, -1);
@@ -615,16 +690,14 @@ void Engine<T>::build(const sp_hip_design& d) {
     // size classes: first net of degree > 2^c (class c holds (2^c, 2^(c+1)],
     // class 0 holds degree 2), wavefronts per class
     {
-        auto firstWithDegAbove = [&](int deg) {
-            int lo = 0, hi = d.num_nets;
-            while (lo < hi) {
-                const int mid = (lo + hi) / 2;
-                if (netStart[mid + 1] - netStart[mid] > deg) hi = mid; else lo = mid + 1;
-            }
-            return lo;
-        };
-        dd_.waNetLo[0] = firstWithDegAbove(1);
-        for (int c = 1; c <= kWaClasses; ++c) dd_.waNetLo[c] = firstWithDegAbove(1 << c);
+        // class boundaries in the new order (classes are contiguous, ascending)
+        std::vector<int> cls(d.num_nets);
+        for (int k = 0; k < d.num_nets; ++k) cls[k] = classOf(netStart[k + 1] - netStart[k]);
+        for (int c = 0; c <= kWaClasses; ++c) {
+            int k = 0;
+            while (k < d.num_nets && cls[k] < c) ++k;
+            dd_.waNetLo[c] = k;
+        }
         dd_.waNetLo[kWaClasses + 1] = d.num_nets;
         dd_.waWaveBase[0] = 0;
         for (int c = 0; c < kWaClasses; ++c) {
@@ -685,9 +758,17 @@ void Engine<T>::build(const sp_hip_design& d) {
     SPHIP_CHECK(hipStreamSynchronize(s_));
 
     // algorithmic bytes / flops per launch (the roofline accounting)
-    const double P = d.num_pins, E = d.num_nets, ts = sizeof(T);
-    bytesPer_[SP_HIP_K_SCATTER] = dd_.nDens * (4 + 2 * ts + 2 * ts + 1) + 8.0 * 2 * nb_;
-    bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + ts + 8 * 2);
+    const double P = d.num_pins, E = d.num_nets, ts = sizeof(T), nd = dd_.nDens;
+    const double stageBytes = static_cast<double>(dd_.tiles.tilesX) * dd_.tiles.tilesY * 2 *
+                              kTileWin * kTileWin * 8;
+    // sort: pass 1 reads each record's node index and gathers (x, y); pass 2
+    // does the same, reads (w, h) and writes the record and (x, y) in order
+    bytesPer_[SP_HIP_K_SORT] = nd * ((4 + 2 * ts) + (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts));
+    // scatter: reads the ordered records and (x, y), writes the records back
+    // and every tile window
+    bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts) + stageBytes;
+    // finalize: atomic grids read and cleared, windows read, static terrain, rho
+    bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + 8 * 2 + ts) + stageBytes;
     bytesPer_[SP_HIP_K_POISSON] = 0;
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
     // WA net pass: per slot node id, offsets, the node's x/y, entry id and
@@ -698,38 +779,30 @@ void Engine<T>::build(const sp_hip_design& d) {
     // (x and y); per pin entry its gradient
     bytesPer_[SP_HIP_K_NODE_GRAD] =
         n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 8 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
-    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 2 * 2 * ts * 3 + 2 * 2 * ts + 2 * ts);
+    // update: index, w/h, A/G/B reads, v/u writes, (x, y) write
+    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts);
+    // reduce: the block partials of the evaluation (8 doubles each)
+    bytesPer_[SP_HIP_K_REDUCE] =
+        64.0 * ((nb_ + kBlock - 1) / kBlock + (dd_.waWaveBase[kWaClasses + 1] + 3) / 4 +
+                (n_ + kBlock - 1) / kBlock);
 }
 
 template <typename T>
 EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda,
                                 float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
                                 T* dbgWl) {
-    beginK(SP_HIP_K_SCATTER);
-    if (sortNext_ || evalCount_ % kSortEvery == 0) launchTileSort<T>(dd_, geom_, nullptr, s_);
-    sortNext_ = false;
-    ++evalCount_;
+    launchTileSort<T>(dd_, geom_, nullptr, s_);
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
-    endK(SP_HIP_K_SCATTER);
-    beginK(SP_HIP_K_FINALIZE);
-    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
-    endK(SP_HIP_K_FINALIZE);
-    beginK(SP_HIP_K_POISSON);
     poisson_->solve(false);
-    endK(SP_HIP_K_POISSON);
-    beginK(SP_HIP_K_WA_NETS);
     launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, nullptr, s_);
-    endK(SP_HIP_K_WA_NETS);
-    beginK(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), gamma, lambda, u, up,
                       gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
     launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
-    endK(SP_HIP_K_NODE_GRAD);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
-    harvestK();
     EvalScalars sc;
     std::memcpy(&sc, hScalars_, sizeof(sc));
     return sc;
@@ -737,9 +810,7 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
 
 template <typename T>
 void Engine<T>::scatterPos(const T* pos) {
-    beginK(SP_HIP_K_UPDATE);
     launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr, s_);
-    endK(SP_HIP_K_UPDATE);
     curPos_ = pos;
 }
 
@@ -748,7 +819,7 @@ template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
     launchTileSort<T>(dd_, geom_, nullptr, s_);
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
-    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
     launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
     launchReduce<T>(ws_, movableAreaScaled_, false, nullptr, nullptr, s_);
@@ -772,9 +843,7 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
     // gather -> clamp -> scatter of the initial positions (eplace.cpp:238-245)
     {
         std::vector<T> hx(total_), hy(total_);
-        SPHIP_CHECK(hipMemcpyAsync(hx.data(), dd_.x, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
-        SPHIP_CHECK(hipMemcpyAsync(hy.data(), dd_.y, total_ * sizeof(T), hipMemcpyDeviceToHost, s_));
-        SPHIP_CHECK(hipStreamSynchronize(s_));
+        downloadXY(hx.data(), hy.data());
         std::vector<T> pos(2 * static_cast<size_t>(n));
         for (int k = 0; k < n; ++k) {
             pos[k] = hx[optIdxHost_[k]];
@@ -895,10 +964,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
                 const double lip = dx / dg;
                 if (lip > 0.95 * alpha) break;
                 alpha = static_cast<double>(static_cast<T>(lip));
-                beginK(SP_HIP_K_UPDATE);
                 launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, nullptr, nullptr,
                                 s_);
-                endK(SP_HIP_K_UPDATE);
                 curPos_ = u_;
                 sc = evaluate(u_, uPrev_, gPrev_, g_, lambda, gamma);
                 checkFinite(sc);
@@ -927,10 +994,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         const T aT = static_cast<T>(a);
         const T aNext = T(0.5) * (T(1) + std::sqrt(T(4) * aT * aT + T(1)));
         const T mom = (aT - T(1)) / aNext;
-        beginK(SP_HIP_K_UPDATE);
         launchUpdate<T>(dd_, u_, g_, v_, alpha, static_cast<double>(mom), vNew_, uNew_, nullptr,
                         nullptr, s_);
-        endK(SP_HIP_K_UPDATE);
         // rotate (nesterov.cpp:109-118)
         std::swap(uPrev_, u_);
         std::swap(gPrev_, g_);
@@ -1023,18 +1088,20 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
 }
 
 template <typename T>
-void Engine<T>::launchSlot(bool events, bool sort) {
+void Engine<T>::launchSlot(bool events, int slot) {
     const T qscale = static_cast<T>(std::ldexp(1.0, fracBits_));
     const int* skip = reinterpret_cast<const int*>(reinterpret_cast<const char*>(dCtl_) +
                                                    offsetof(Ctl, done));
-    auto b = [&](int f) { if (events) beginK(f); };
-    auto e = [&](int f) { if (events) endK(f); };
-    if (sort) launchTileSort<T>(dd_, geom_, dCtl_, s_);
+    auto b = [&](int f) { if (events) beginK(f, slot); };
+    auto e = [&](int f) { if (events) endK(f, slot); };
+    b(SP_HIP_K_SORT);
+    launchTileSort<T>(dd_, geom_, dCtl_, s_);
+    e(SP_HIP_K_SORT);
     b(SP_HIP_K_SCATTER);
     launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, dCtl_, s_);
     e(SP_HIP_K_SCATTER);
     b(SP_HIP_K_FINALIZE);
-    launchFinalize<T>(geom_, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
+    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, dCtl_, s_);
     e(SP_HIP_K_FINALIZE);
     b(SP_HIP_K_POISSON);
@@ -1046,8 +1113,10 @@ void Engine<T>::launchSlot(bool events, bool sort) {
     b(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), 0.f, 0.f, nullptr,
                       nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, s_);
-    launchReduce<T>(ws_, movableAreaScaled_, true, dCtl_, dRows_, s_);
     e(SP_HIP_K_NODE_GRAD);
+    b(SP_HIP_K_REDUCE);
+    launchReduce<T>(ws_, movableAreaScaled_, true, dCtl_, dRows_, s_);
+    e(SP_HIP_K_REDUCE);
     b(SP_HIP_K_UPDATE);
     launchUpdate<T>(dd_, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, dCtl_, dBufs_, s_);
     e(SP_HIP_K_UPDATE);
@@ -1111,16 +1180,16 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     if (curPos_ != u_) scatterPos(u_);
     SPHIP_CHECK(hipStreamSynchronize(s_));
 
-    const bool graph = cfg_.loop_mode == 2;
-    const int slots = graph ? std::max(1, cfg_.graph_slots) : (profMask_ ? 1 : std::max(1, cfg_.graph_slots));
+    // Profiling launches the slots directly (as loop_mode 1): the profiled
+    // kernels carry event pairs, which a captured graph does not time.
+    const bool graph = cfg_.loop_mode == 2 && profMask_ == 0;
+    const int slots = std::max(1, cfg_.graph_slots);
+    if (profMask_) ensureEvents(slots);
     if (graph && !graphExec_) {
-        // capture `slots` iterations; families in profMask_ get event nodes in the first slot
-        graphEvMask_ = profMask_;
-        const int saved = profMask_;
+        // capture `slots` iterations
         SPHIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         try {
-            for (int i = 0; i < slots; ++i)
-                launchSlot(i == 0 && graphEvMask_ != 0, i % kSortEvery == 0);
+            for (int i = 0; i < slots; ++i) launchSlot(false, i);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(s_, &g);
@@ -1129,9 +1198,8 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         }
         SPHIP_CHECK(hipStreamEndCapture(s_, &graph_));
         SPHIP_CHECK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
-        profMask_ = saved;
-        for (int f = 0; f < SP_HIP_K_COUNT; ++f) pendingK_[f] = false;
     }
+    const int evMask = graph ? 0 : profMask_;
     const long maxLaunches = (static_cast<long>(maxIter) * maxBt + 16) / slots + 2;
     const auto t0 = std::chrono::steady_clock::now();
     for (long launch = 0;; ++launch) {
@@ -1139,21 +1207,14 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         if (graph) {
             SPHIP_CHECK(hipGraphLaunch(graphExec_, s_));
         } else {
-            for (int i = 0; i < slots; ++i)
-                launchSlot(profMask_ != 0, (launch * slots + i) % kSortEvery == 0);
+            for (int i = 0; i < slots; ++i) launchSlot(evMask != 0, i);
         }
         SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
                                    sizeof(int), hipMemcpyDeviceToHost, s_));
         SPHIP_CHECK(hipStreamSynchronize(s_));
         const bool done = *hFlag_ != 0;
-        if (graph && graphEvMask_ && !done) {
-            for (int f = 0; f < SP_HIP_K_COUNT; ++f)
-                if (graphEvMask_ & (1 << f)) pendingK_[f] = true;
-            harvestK();
-        } else if (!graph && profMask_ && !done) {
-            harvestK();
-        }
-        for (int f = 0; f < SP_HIP_K_COUNT; ++f) pendingK_[f] = false;
+        // only launches whose slots all ran a full evaluation are timed
+        if (evMask && !done) harvestK(evMask, slots);
         if (done) break;
     }
     res->seconds_loop =

@@ -7,26 +7,43 @@
 
 namespace sphip {
 
+// Spatial tiles of kTileBins x kTileBins bins.  Each tile owns a window of
+// kTileWin x kTileWin bins (the tile and a halo of kTileHalo bins) in
+// `stage`, which the scatter writes with plain stores and the finalize pass
+// sums into the bins; the per-tile counters are zero between evaluations.
+struct TileGrid {
+    int tilesX = 0, tilesY = 0;   // tiles per grid side (the last ones may be partial)
+    int nKeys = 0;                // Morton key range (a power of 4 >= tilesX * tilesY)
+    int* count = nullptr;         // nodes per key (pass 1)
+    int* fill = nullptr;          // placement cursor per key (pass 2)
+    int* start = nullptr;         // first position of each key in densB
+    int* extraStart = nullptr;    // nKeys + 1: first extra scatter block of each key
+    unsigned long long* stage = nullptr;   // [tile][node | filler][kTileWin^2]
+};
+
 template <typename T>
 struct DevDesign {
     int total = 0, numMov = 0, numNodes = 0;
     int nOpt = 0, nDens = 0, numNets = 0, numPins = 0;
     T coreLx, coreLy, coreHx, coreHy;
     // nodes
-    T* x = nullptr;          // current evaluation point, all nodes
-    T* y = nullptr;
+    T* xy = nullptr;         // current evaluation point of all nodes, (x, y) interleaved
     const T* w = nullptr;
     const T* h = nullptr;
     const uint8_t* flags = nullptr;
     const int* optIdx = nullptr;     // pos index k -> node i
-    const int* densIdx = nullptr;    // nodes with a density contribution
-    // the density nodes in spatial order (Morton order of 16x16-bin tiles of
-    // their centre bin), refreshed periodically by launchTileSort: the
-    // scatter reads densB and keeps densA = densB for the next sort
-    int* densA = nullptr;
-    int* densB = nullptr;
-    int* tileCount = nullptr;     // kTileKeys counters (zero between sorts)
-    int* tileCursor = nullptr;    // kTileKeys insertion cursors
+    // The density nodes in spatial order (by tile of their centre bin, tiles
+    // in Morton order) as records: node index with the filler / macro flags
+    // in bits 30 / 29 (recId), and (w, h) interleaved (recWH).  launchTileSort
+    // reads the previous order (A) and writes the new one (B) together with
+    // the nodes' current (x, y) (recXY); the scatter reads B and copies the
+    // records back to A.
+    int* recIdA = nullptr;
+    int* recIdB = nullptr;
+    T* recWHA = nullptr;
+    T* recWHB = nullptr;
+    T* recXY = nullptr;
+    TileGrid tiles;
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
     // net -> pin slots.  Nets are renumbered by degree (ascending), so each
@@ -50,10 +67,12 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
-constexpr int kTileBins = 16;        // bins per tile side for the spatial sort
+constexpr int kTileBins = 16;        // bins per tile side
+constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
+constexpr int kTileWin = kTileBins + 2 * kTileHalo;
+constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 1;
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
-constexpr int kScatterChunk = 256;   // density nodes per scatter block (1 per thread)
-constexpr int kWindowBins = 1024;    // LDS window of the scatter block (per grid)
+constexpr int kTileChunk = 1024;     // nodes per scatter block (a tile's owner takes the first)
 
 // FP64 reductions: every reducing kernel writes one partial per block (plain
 // stores, no atomics, no fences); k_reduce sums each region in block order
@@ -114,6 +133,11 @@ struct Ctl {
 // host-driven mode).  With ctl, kernels exit at once when ctl->done is set,
 // gamma/lambda come from ctl, and the node pass / update resolve their state
 // vectors through the table `bufs` (kNumBufs device pointers).
+// Density scatter in tile order (after launchTileSort): one owner block per
+// tile accumulates the fixed-point charges of the tile's first kTileChunk
+// nodes in its window and stores the window to tiles.stage; extra blocks take
+// the further chunks of crowded tiles and add their windows to nodeQ / fillQ
+// with atomics, as do charges that fall outside a window (large nodes).
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
@@ -124,11 +148,13 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
 template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s);
 
+// Sums the staged windows and the atomic grids into node / filler density,
+// then rho and the tau numerator; clears the atomic grids and tile counters.
 template <typename T>
-void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
-                    const double* terminal, const double* base, double invQ, double binArea,
-                    double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
-                    ReduceWs& ws, const Ctl* ctl, hipStream_t s);
+void launchFinalize(const GridGeom<T>& g, const TileGrid& tg, unsigned long long* nodeQ,
+                    unsigned long long* fillQ, const double* terminal, const double* base,
+                    double invQ, double binArea, double rhoShift, T* rho, long long* dbgNode,
+                    long long* dbgFill, ReduceWs& ws, const Ctl* ctl, hipStream_t s);
 
 template <typename T>
 void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& ws,

@@ -127,12 +127,20 @@ __device__ inline unsigned tileKeyOf(T x, T y, const GridGeom<T>& g) {
     return mortonKey(static_cast<unsigned>(bx / kTileBins), static_cast<unsigned>(by / kTileBins));
 }
 
-// Counting sort by tile key.  Each block takes kBlock nodes of the current
-// order, counts its keys in an LDS histogram (remembering the distinct keys),
-// then touches each distinct key once in global memory: pass 1 adds the
-// block's counts, pass 3 reserves the block's range at the key's cursor and
-// hands out ranks from LDS.  The order within a tile is arbitrary, which the
-// integer density sums do not see.
+__device__ inline unsigned mortonCoord(unsigned k) {   // the even bits of k
+    unsigned c = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) c |= ((k >> (2 * b)) & 1u) << b;
+    return c;
+}
+
+// Counting sort by tile key.  Each block takes kBlock nodes of the previous
+// order (nearly sorted already, so a block sees few keys), counts its keys in
+// an LDS histogram (remembering the distinct keys), then touches each
+// distinct key once in global memory: pass 1 adds the block's counts, pass 2
+// reserves the block's range at the key's cursor and hands out ranks from
+// LDS.  The order within a tile is arbitrary, which the integer density sums
+// do not see.
 struct TileHist {
     int count[kTileKeys];
     int keys[kBlock];
@@ -141,20 +149,58 @@ struct TileHist {
 
 template <typename T>
 __device__ inline unsigned blockTileHist(const DevDesign<T>& d, const GridGeom<T>& g, TileHist& h,
-                                         int* outNode) {
-    for (int k = threadIdx.x; k < kTileKeys; k += kBlock) h.count[k] = 0;
+                                         int* outRec, typename Vec2<T>::type* outXY) {
+    for (int k = threadIdx.x; k < d.tiles.nKeys; k += kBlock) h.count[k] = 0;
     if (threadIdx.x == 0) h.nKeys = 0;
     __syncthreads();
     const int t = blockIdx.x * kBlock + threadIdx.x;
     unsigned key = 0xFFFFFFFFu;
     if (t < d.nDens) {
-        const int i = d.densA[t];
-        *outNode = i;
-        key = tileKeyOf<T>(d.x[i], d.y[i], g);
+        const int rec = d.recIdA[t];
+        const auto p = load2(d.xy, rec & kRecNode);
+        *outRec = rec;
+        *outXY = p;
+        key = tileKeyOf<T>(p.x, p.y, g);
         if (atomicAdd(&h.count[key], 1) == 0) h.keys[atomicAdd(&h.nKeys, 1)] = key;
     }
     __syncthreads();
     return key;
+}
+
+// Exclusive prefix sums of src[0..n) into dst (LDS, kTileKeys entries),
+// n <= kTileKeys; src may alias dst.  Returns the total.  Block-wide.
+__device__ inline int blockExclusiveScan(const int* src, int n, int* dst, int* waveTot) {
+    const int per = (n + kBlock - 1) / kBlock;
+    const int t0 = threadIdx.x * per;
+    int sum = 0;
+    for (int j = 0; j < per; ++j) {
+        const int k = t0 + j;
+        const int v = k < n ? src[k] : 0;
+        dst[k] = v;
+        sum += v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) waveTot[wave] = inc;
+    __syncthreads();
+    int run = inc - sum, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) run += waveTot[w];
+        total += waveTot[w];
+    }
+    for (int j = 0; j < per; ++j) {
+        const int k = t0 + j;
+        const int v = dst[k];
+        dst[k] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
 }
 
 template <typename T>
@@ -162,156 +208,162 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<
                                                        const Ctl* ctl) {
     if (ctl && ctl->done) return;
     __shared__ TileHist h;
-    int node = 0;
-    blockTileHist<T>(d, g, h, &node);
+    int rec = 0;
+    typename Vec2<T>::type p;
+    blockTileHist<T>(d, g, h, &rec, &p);
     for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
         const int key = h.keys[q];
-        atomicAdd(&d.tileCount[key], h.count[key]);
+        atomicAdd(&d.tiles.count[key], h.count[key]);
     }
 }
 
-// Pass 2 (one block): exclusive scan of the counts into cursors; clears counts.
-__global__ __launch_bounds__(kBlock) void k_tile_scan(int* count, int* cursor, const Ctl* ctl) {
-    if (ctl && ctl->done) return;
-    constexpr int per = kTileKeys / kBlock;
-    const int t0 = threadIdx.x * per;
-    int local[per];
-    int sum = 0;
-#pragma unroll
-    for (int j = 0; j < per; ++j) {
-        local[j] = count[t0 + j];
-        sum += local[j];
-    }
-    // inclusive scan of per-thread sums: wave shuffles, then the wave totals
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int inc = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += v;
-    }
-    __shared__ int waveTot[kBlock / 64];
-    if (lane == 63) waveTot[wave] = inc;
-    __syncthreads();
-    int offset = 0;
-    for (int w = 0; w < wave; ++w) offset += waveTot[w];
-    int run = offset + inc - sum;   // exclusive prefix of this thread's first key
-#pragma unroll
-    for (int j = 0; j < per; ++j) {
-        cursor[t0 + j] = run;
-        run += local[j];
-        count[t0 + j] = 0;
-    }
-}
-
+// Pass 2: every block scans the counts itself (a few KB from L2), so no
+// separate scan launch is needed.  Block 0 also publishes the key starts and
+// the extra scatter blocks of crowded tiles.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<T> g,
                                                        const Ctl* ctl) {
     if (ctl && ctl->done) return;
     __shared__ TileHist h;
-    int node = 0;
-    const unsigned key = blockTileHist<T>(d, g, h, &node);
-    // reserve this block's range per key; reuse count[] as the range base
+    __shared__ int prefix[kTileKeys];
+    __shared__ int waveTot[kBlock / 64];
+    const TileGrid& tg = d.tiles;
+    int rec = 0;
+    typename Vec2<T>::type p;
+    const unsigned key = blockTileHist<T>(d, g, h, &rec, &p);
+    blockExclusiveScan(tg.count, tg.nKeys, prefix, waveTot);
     for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
         const int k = h.keys[q];
-        h.count[k] = atomicAdd(&d.tileCursor[k], h.count[k]);
+        h.count[k] = prefix[k] + atomicAdd(&tg.fill[k], h.count[k]);
     }
     __syncthreads();
-    if (key != 0xFFFFFFFFu) d.densB[atomicAdd(&h.count[key], 1)] = node;
+    if (key != 0xFFFFFFFFu) {
+        const int t = blockIdx.x * kBlock + threadIdx.x;
+        const int pos = atomicAdd(&h.count[key], 1);
+        const auto wh = load2(d.recWHA, t);
+        d.recIdB[pos] = rec;
+        store2(d.recWHB, pos, wh.x, wh.y);
+        store2(d.recXY, pos, p.x, p.y);
+    }
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) tg.start[k] = prefix[k];
+        __syncthreads();
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
+            const int c = tg.count[k];
+            prefix[k] = c > kTileChunk ? (c - 1) / kTileChunk : 0;
+        }
+        __syncthreads();
+        const int total = blockExclusiveScan(prefix, tg.nKeys, prefix, waveTot);
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) tg.extraStart[k] = prefix[k];
+        if (threadIdx.x == 0) tg.extraStart[tg.nKeys] = total;
+    }
 }
 
 // ------------------------------------------------------------------ (a) scatter
-// A block takes kScatterChunk consecutive nodes of the spatial order and
-// accumulates their fixed-point charges into an LDS window (the bin bounding
-// box of the chunk's small footprints, clipped to kWindowBins), then adds the
-// window's non-zero bins to the grids with one global atomic each.  Charges of
-// footprints outside the window go straight to global atomics.  The scatter
-// also copies its part of the order back (densA = densB) for the next sort.
+// Blocks [0, tiles) own one tile each and take its first kTileChunk nodes;
+// blocks past that take chunk j >= 1 of a crowded tile (found in extraStart).
+// A block accumulates its nodes' fixed-point charges in the tile's window in
+// LDS; an owner stores the whole window to the stage (plain stores, every
+// evaluation, zeros included), an extra block adds its non-zero window bins
+// to the atomic grids.  Charges outside the window (nodes wider than the
+// halo) go to the atomic grids directly.  The block also copies its part of
+// the records back (A = B) for the next sort.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
                                                     unsigned long long* fillQ, const Ctl* ctl) {
     if (ctl && ctl->done) return;
-    constexpr int PER = kScatterChunk / kBlock;
-    __shared__ unsigned long long wn[kWindowBins], wf[kWindowBins];
-    __shared__ int box[4];   // min bx, min by, -max bx, -max by of small footprints
-    if (threadIdx.x < 4) box[threadIdx.x] = 0x7FFFFFFF;
+    constexpr int PER = kTileChunk / kBlock;
+    constexpr int WIN2 = kTileWin * kTileWin;
+    __shared__ unsigned long long win[2 * WIN2];   // node window, then filler window
+    const TileGrid& tg = d.tiles;
+    const int nTiles = tg.tilesX * tg.tilesY;
+    int tx, ty, key, chunk;
+    if (static_cast<int>(blockIdx.x) < nTiles) {
+        tx = blockIdx.x / tg.tilesY;
+        ty = blockIdx.x - tx * tg.tilesY;
+        key = static_cast<int>(mortonKey(tx, ty));
+        chunk = 0;
+    } else {
+        const int e = blockIdx.x - nTiles;
+        if (e >= tg.extraStart[tg.nKeys]) return;
+        int lo = 0, hi = tg.nKeys - 1;   // last key with extraStart <= e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tg.extraStart[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        key = lo;
+        chunk = e - tg.extraStart[lo] + 1;
+        tx = static_cast<int>(mortonCoord(key));
+        ty = static_cast<int>(mortonCoord(key >> 1));
+    }
+    for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) win[e] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kScatterChunk;
+    const int cnt = tg.count[key], st = tg.start[key];
+    const int lo = st + chunk * kTileChunk;
+    const int hi = st + min(cnt, (chunk + 1) * kTileChunk);
+    const int wx0 = tx * kTileBins - kTileHalo, wy0 = ty * kTileBins - kTileHalo;
     Footprint<T> fp[PER];
     uint8_t fl[PER];
     bool valid[PER];
-    int bb[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int t = base + threadIdx.x + j * kBlock;
-        valid[j] = t < d.nDens;
+        const int t = lo + threadIdx.x + j * kBlock;
+        valid[j] = t < hi;
         if (!valid[j]) continue;
-        const int i = d.densB[t];
-        d.densA[t] = i;
-        fl[j] = d.flags[i];
-        fp[j] = footprintOf<T>(d.x[i], d.y[i], d.w[i], d.h[i], fl[j], g);
-        if (fp[j].bx1 - fp[j].bx0 < 4 && fp[j].by1 - fp[j].by0 < 4) {
-            bb[0] = min(bb[0], fp[j].bx0);
-            bb[1] = min(bb[1], fp[j].by0);
-            bb[2] = min(bb[2], -fp[j].bx1);
-            bb[3] = min(bb[3], -fp[j].by1);
-        }
+        const int rec = d.recIdB[t];
+        const auto wh = load2(d.recWHB, t);
+        const auto p = load2(d.recXY, t);
+        d.recIdA[t] = rec;
+        store2(d.recWHA, t, wh.x, wh.y);
+        fl[j] = static_cast<uint8_t>(((rec & kRecFiller) ? F_FILLER : 0) |
+                                     ((rec & kRecMacro) ? F_MACRO : 0));
+        fp[j] = footprintOf<T>(p.x, p.y, wh.x, wh.y, fl[j], g);
     }
-    // bounding box: wavefront shuffle minima, one LDS atomic per wave
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) bb[q] = min(bb[q], __shfl_xor(bb[q], off, 64));
-        if ((threadIdx.x & 63) == 0) atomicMin(&box[q], bb[q]);
-    }
-    __syncthreads();
-    const int wx0 = box[0], wy0 = box[1];
-    int ww = 0, wh = 0;
-    if (wx0 != 0x7FFFFFFF) {
-        ww = -box[2] - wx0 + 1;
-        wh = -box[3] - wy0 + 1;
-        if (ww > 64) ww = 64;
-        if (ww * wh > kWindowBins) wh = kWindowBins / ww;
-    }
-    const int wsize = ww * wh;
-    for (int e = threadIdx.x; e < wsize; e += kBlock) {
-        wn[e] = 0;
-        wf[e] = 0;
-    }
-    __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         if (!valid[j]) continue;
         const bool filler = fl[j] & F_FILLER;
         unsigned long long* grid = filler ? fillQ : nodeQ;
-        unsigned long long* win = filler ? wf : wn;
+        unsigned long long* w = win + (filler ? WIN2 : 0);
         for (int bx = fp[j].bx0; bx <= fp[j].bx1; ++bx)
             for (int by = fp[j].by0; by <= fp[j].by1; ++by) {
                 const T ov = binCharge<T>(bx, by, fp[j], g);
                 if (ov <= T(0)) continue;
                 const unsigned long long q = static_cast<unsigned long long>(toFixed(ov, qscale));
                 const int lx = bx - wx0, ly = by - wy0;
-                if (lx >= 0 && lx < ww && ly >= 0 && ly < wh)
-                    atomicAdd(&win[lx * wh + ly], q);
+                if (lx >= 0 && lx < kTileWin && ly >= 0 && ly < kTileWin)
+                    atomicAdd(&w[lx * kTileWin + ly], q);
                 else
                     atomicAdd(&grid[static_cast<long>(bx) * g.dim + by], q);
             }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < wsize; e += kBlock) {
-        const long gi = static_cast<long>(wx0 + e / wh) * g.dim + wy0 + e % wh;
-        if (wn[e]) atomicAdd(&nodeQ[gi], wn[e]);
-        if (wf[e]) atomicAdd(&fillQ[gi], wf[e]);
+    if (chunk == 0) {
+        unsigned long long* out = tg.stage + static_cast<long>(blockIdx.x) * 2 * WIN2;
+        for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) out[e] = win[e];
+    } else {
+        for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) {
+            const unsigned long long v = win[e];
+            if (!v) continue;
+            const int r = e < WIN2 ? e : e - WIN2;
+            const long gi = static_cast<long>(wx0 + r / kTileWin) * g.dim + wy0 + r % kTileWin;
+            atomicAdd(&(e < WIN2 ? nodeQ : fillQ)[gi], v);
+        }
     }
 }
 
 // ------------------------------------------------------------------ rho + tau
-// rho = (node + filler + terminal + base) / binArea (bin_grid.cpp:361-368);
-// the tau numerator sum_b max(0, (node + terminal + base)/area - target) area
-// (bin_grid.cpp:373-393, fillers excluded).  Clears the integer grids.
+// Node / filler density of a bin = the atomic grid plus the staged windows
+// that cover it (its own tile's, and a neighbour's for halo bins), all
+// integers, so the sum is exact.  Then rho = (node + filler + terminal +
+// base) / binArea (bin_grid.cpp:361-368) and the tau numerator
+// sum_b max(0, (node + terminal + base)/area - target) area
+// (bin_grid.cpp:373-393, fillers excluded).  Clears the atomic grids and the
+// tile counters for the next evaluation.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned long long* nodeQ,
+__global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, TileGrid tg,
+                                                     unsigned long long* nodeQ,
                                                      unsigned long long* fillQ,
                                                      const double* terminal, const double* base,
                                                      double invQ, double binArea, double rhoShift,
@@ -319,18 +371,38 @@ __global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, unsigned lon
                                                      long long* dbgFill, double* partials,
                                                      const Ctl* ctl) {
     if (ctl && ctl->done) return;
+    constexpr int WIN2 = kTileWin * kTileWin;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = g.dim * g.dim;
     double v[1] = {0.0};
+    if (b < tg.nKeys) {
+        tg.count[b] = 0;
+        tg.fill[b] = 0;
+    }
     if (b < nb) {
-        const long long qn = static_cast<long long>(nodeQ[b]);
-        const long long qf = static_cast<long long>(fillQ[b]);
-        const double node = static_cast<double>(qn) * invQ;
-        const double fill = static_cast<double>(qf) * invQ;
+        const int bx = b / g.dim, by = b - bx * g.dim;
+        unsigned long long sn = nodeQ[b], sf = fillQ[b];
         nodeQ[b] = 0;
         fillQ[b] = 0;
+        const int tx = bx / kTileBins, ty = by / kTileBins;
+        const int ux = bx - tx * kTileBins + kTileHalo, uy = by - ty * kTileBins + kTileHalo;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int sx = ux - dx * kTileBins;
+            if (sx < 0 || sx >= kTileWin || tx + dx < 0 || tx + dx >= tg.tilesX) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int sy = uy - dy * kTileBins;
+                if (sy < 0 || sy >= kTileWin || ty + dy < 0 || ty + dy >= tg.tilesY) continue;
+                const long at = static_cast<long>((tx + dx) * tg.tilesY + ty + dy) * 2 * WIN2 +
+                                sx * kTileWin + sy;
+                sn += tg.stage[at];
+                sf += tg.stage[at + WIN2];
+            }
+        }
+        const long long qn = static_cast<long long>(sn);
+        const long long qf = static_cast<long long>(sf);
+        const double node = static_cast<double>(qn) * invQ;
+        const double fill = static_cast<double>(qf) * invQ;
         const double tb = terminal[b], bb = base[b];
-        const int bx = b / g.dim, by = b - bx * g.dim;
         // a constant shift only changes the DC term, which the solve discards;
         // it keeps the transform inputs small (smaller FP32 rounding)
         rho[static_cast<long>(bx) * g.ld + by] =
@@ -444,9 +516,9 @@ __device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lan
     const bool pin = j < deg;
     T px = 0, py = 0;
     if (pin) {
-        const int nd = d.slotNode[t];
-        px = d.x[nd] + d.slotOffX[t];
-        py = d.y[nd] + d.slotOffY[t];
+        const auto p = load2(d.xy, d.slotNode[t]);
+        px = p.x + d.slotOffX[t];
+        py = p.y + d.slotOffY[t];
     }
     const T mxX = groupMax<W>(pin ? px : -tmax<T>()), mnX = groupMin<W>(pin ? px : tmax<T>());
     const T mxY = groupMax<W>(pin ? py : -tmax<T>()), mnY = groupMin<W>(pin ? py : tmax<T>());
@@ -483,8 +555,8 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
     const int b = d.netStart[net], e = d.netStart[net + 1];
     T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
     for (int t = b + lane; t < e; t += 64) {
-        const int nd = d.slotNode[t];
-        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        const auto p = load2(d.xy, d.slotNode[t]);
+        const T px = p.x + d.slotOffX[t], py = p.y + d.slotOffY[t];
         mxX = fmax(mxX, px); mnX = fmin(mnX, px);
         mxY = fmax(mxY, py); mnY = fmin(mnY, py);
     }
@@ -493,8 +565,8 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
     if (e - b > ignoreDeg) return;
     WaSums<T> s{0, 0, 0, 0, 0, 0, 0, 0};
     for (int t = b + lane; t < e; t += 64) {
-        const int nd = d.slotNode[t];
-        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        const auto p = load2(d.xy, d.slotNode[t]);
+        const T px = p.x + d.slotOffX[t], py = p.y + d.slotOffY[t];
         const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
         const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
         const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
@@ -505,8 +577,8 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
     s.bpy = waveSum(s.bpy); s.cpy = waveSum(s.cpy); s.bny = waveSum(s.bny); s.cny = waveSum(s.cny);
     if (lane == 0) v[0] = waOf<T>(mxX, mnX, mxY, mnY, s);
     for (int t = b + lane; t < e; t += 64) {
-        const int nd = d.slotNode[t];
-        const T px = d.x[nd] + d.slotOffX[t], py = d.y[nd] + d.slotOffY[t];
+        const auto p = load2(d.xy, d.slotNode[t]);
+        const T px = p.x + d.slotOffX[t], py = p.y + d.slotOffY[t];
         const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
         const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
         const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
@@ -578,7 +650,8 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     if (k < n) {
         const int i = d.optIdx[k];
-        const T xi = d.x[i], yi = d.y[i];
+        const auto pi = load2(d.xy, i);
+        const T xi = pi.x, yi = pi.y;
         T gx = 0, gy = 0;
         const bool hasWl = i < d.numMov;
         if (hasWl)
@@ -879,8 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
         if (vOut) vOut[j] = vv[a];
         if (uOut) uOut[j] = uu[a];
     }
-    d.x[i] = uOut ? uu[0] : vv[0];
-    d.y[i] = uOut ? uu[1] : vv[1];
+    store2(d.xy, i, uOut ? uu[0] : vv[0], uOut ? uu[1] : vv[1]);
 }
 
 long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
@@ -891,9 +963,8 @@ template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
                    hipStream_t s) {
-    if (d.nDens == 0) return;
-    k_scatter<T><<<(d.nDens + kScatterChunk - 1) / kScatterChunk, kBlock, 0, s>>>(
-        d, g, qscale, nodeQ, fillQ, ctl);
+    const int blocks = d.tiles.tilesX * d.tiles.tilesY + (d.nDens + kTileChunk - 1) / kTileChunk;
+    launchKernel(k_scatter<T>, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -901,24 +972,22 @@ template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s) {
     if (d.nDens == 0) return;
     if (g.dim > kTileBins * 64) throw std::runtime_error("tile sort: grid wider than 1024 bins");
-    k_tile_count<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, ctl);
+    launchKernel(k_tile_count<T>, gridFor(d.nDens), kBlock, 0, s, d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
-    k_tile_scan<<<1, kBlock, 0, s>>>(d.tileCount, d.tileCursor, ctl);
-    SPHIP_CHECK(hipGetLastError());
-    k_tile_place<T><<<gridFor(d.nDens), kBlock, 0, s>>>(d, g, ctl);
+    launchKernel(k_tile_place<T>, gridFor(d.nDens), kBlock, 0, s, d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void launchFinalize(const GridGeom<T>& g, unsigned long long* nodeQ, unsigned long long* fillQ,
-                    const double* terminal, const double* base, double invQ, double binArea,
-                    double rhoShift, T* rho, long long* dbgNode, long long* dbgFill,
-                    ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
+void launchFinalize(const GridGeom<T>& g, const TileGrid& tg, unsigned long long* nodeQ,
+                    unsigned long long* fillQ, const double* terminal, const double* base,
+                    double invQ, double binArea, double rhoShift, T* rho, long long* dbgNode,
+                    long long* dbgFill, ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
     const long nb = static_cast<long>(g.dim) * g.dim;
-    ws.blocks[0] = static_cast<int>(gridFor(nb));
-    k_finalize<T><<<ws.blocks[0], kBlock, 0, s>>>(g, nodeQ, fillQ, terminal, base, invQ, binArea,
-                                                 rhoShift, rho, dbgNode, dbgFill, ws.region(0),
-                                                 ctl);
+    ws.blocks[0] = static_cast<int>(gridFor(std::max<long>(nb, tg.nKeys)));
+    launchKernel(k_finalize<T>, ws.blocks[0], kBlock, 0, s, g, tg, nodeQ, fillQ, terminal, base, invQ,
+                                                 binArea, rhoShift, rho, dbgNode, dbgFill,
+                                                 ws.region(0), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -929,7 +998,7 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
     const int waves = d.waWaveBase[kWaClasses + 1];
     ws.blocks[1] = std::max(1, (waves + kBlock / 64 - 1) / (kBlock / 64));
     ws.blocks[2] = 0;
-    k_wa_pins<T><<<ws.blocks[1], kBlock, 0, s>>>(d, invG, ignoreDeg, ws.region(1), ctl);
+    launchKernel(k_wa_pins<T>, ws.blocks[1], kBlock, 0, s, d, invG, ignoreDeg, ws.region(1), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -940,7 +1009,7 @@ void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, co
                     hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
     ws.blocks[3] = static_cast<int>(gridFor(d.nOpt));
-    k_node_grad<T><<<ws.blocks[3], kBlock, 0, s>>>(d, g, ex, ey, invG, lambda, u, up, gp, grad,
+    launchKernel(k_node_grad<T>, ws.blocks[3], kBlock, 0, s, d, g, ex, ey, invG, lambda, u, up, gp, grad,
                                                   dbgForce, dbgWl, ws.region(3), ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
 }
@@ -948,14 +1017,14 @@ void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, co
 template <typename T>
 void launchReduce(ReduceWs& ws, double denom, bool withNode, Ctl* ctl, DevRow* rows,
                   hipStream_t s) {
-    k_reduce<T><<<1, kBlock, 0, s>>>(ws, denom, withNode ? 1 : 0, ctl, rows);
+    launchKernel(k_reduce<T>, 1, kBlock, 0, s, ws, denom, withNode ? 1 : 0, ctl, rows);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
                   double m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs, hipStream_t s) {
-    k_update<T><<<gridFor(d.nOpt), kBlock, 0, s>>>(d, A, G, B, static_cast<T>(alpha),
+    launchKernel(k_update<T>, gridFor(d.nOpt), kBlock, 0, s, d, A, G, B, static_cast<T>(alpha),
                                                    static_cast<T>(m), vOut, uOut, ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
 }
@@ -966,7 +1035,8 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
                                    hipStream_t);                                                 \
     template void launchTileSort<T>(const DevDesign<T>&, const GridGeom<T>&, const Ctl*,         \
                                     hipStream_t);                                                \
-    template void launchFinalize<T>(const GridGeom<T>&, unsigned long long*, unsigned long long*, \
+    template void launchFinalize<T>(const GridGeom<T>&, const TileGrid&, unsigned long long*,     \
+                                    unsigned long long*, \
                                     const double*, const double*, double, double, double, T*,    \
                                     long long*, long long*, ReduceWs&, const Ctl*, hipStream_t); \
     template void launchWaNets<T>(const DevDesign<T>&, float, int, ReduceWs&, const Ctl*,        \

@@ -196,10 +196,10 @@ void gemmDispatch(int batch, const GemmBatch<T>& gb, int M, int N, int K, int ld
     const long tiles64 = static_cast<long>(M / 64) * (N / 64) * batch;
     if (tiles64 >= 256) {
         dim3 grid(N / 64, M / 64, batch);
-        k_gemm<T, 2, 2, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi, skip);
+        launchKernel(k_gemm<T, 2, 2, EPI>, grid, 256, 0, s, gb, M, N, K, lda, ldb, ldc, epi, skip);
     } else {
         dim3 grid(N / 32, M / 32, batch);
-        k_gemm<T, 1, 1, EPI><<<grid, 256, 0, s>>>(gb, M, N, K, lda, ldb, ldc, epi, skip);
+        launchKernel(k_gemm<T, 1, 1, EPI>, grid, 256, 0, s, gb, M, N, K, lda, ldb, ldc, epi, skip);
     }
     SPHIP_CHECK(hipGetLastError());
 }

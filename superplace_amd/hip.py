@@ -27,7 +27,8 @@ HEADER = HERE.parent / "include" / "superplace_hip.h"
 
 SP_HIP_OK, SP_HIP_ENODEV, SP_HIP_EINVAL, SP_HIP_EOOM, SP_HIP_EDEVICE, SP_HIP_ENONFINITE = range(6)
 SP_HIP_FLAG_FP64 = 1
-KERNEL_FAMILIES = ["scatter", "finalize", "poisson", "wa_nets", "node_grad", "update"]
+KERNEL_FAMILIES = ["scatter", "finalize", "poisson", "wa_nets", "node_grad", "update", "sort",
+                   "reduce"]
 
 _P = C.c_void_p
 
