@@ -209,10 +209,10 @@ int sp_hip_gp_kernel_stats(const sp_hip_gp* gp, double* ms, int64_t* launches, d
                            double* flops);
 void sp_hip_gp_destroy(sp_hip_gp* gp);
 
-/* Batched C = A * B of row-major matrices (dims multiples of 64) on the
- * matrix cores, the primitive of the Poisson transforms.  Device pointers,
- * float (or double with SP_HIP_FLAG_FP64); synchronous.  No counterpart in
- * the reference (FFTW / cuFFT there). */
+/* Batched C = A * B^T of row-major matrices (A m x k, B n x k, C m x n; dims
+ * multiples of 64) on the matrix cores, the primitive of the Poisson
+ * transforms.  Device pointers, float (or double with SP_HIP_FLAG_FP64);
+ * synchronous.  No counterpart in the reference (FFTW / cuFFT there). */
 int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b,
                 void* const* c, int m, int n, int k);
 

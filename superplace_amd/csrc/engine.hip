@@ -269,8 +269,18 @@ public:
                 fetchT(dbg->rho, poisson_->rho(), 0, poisson_->ld(), dim_, dim_);
                 if (dbg->rho)
                     for (size_t b = 0; b < nb_; ++b) dbg->rho[b] += rhoShift_;
-                fetchT(dbg->field_x, poisson_->fieldX(), 0, poisson_->ld(), dim_, dim_);
-                fetchT(dbg->field_y, poisson_->fieldY(), 0, poisson_->ld(), dim_, dim_);
+                if (dbg->field_x || dbg->field_y) {
+                    std::vector<T> tmp(2 * static_cast<size_t>(poisson_->rowsPadded()) * poisson_->ld());
+                    SPHIP_CHECK(hipMemcpyAsync(tmp.data(), poisson_->fieldXY(), tmp.size() * sizeof(T),
+                                               hipMemcpyDeviceToHost, s_));
+                    SPHIP_CHECK(hipStreamSynchronize(s_));
+                    for (int r = 0; r < dim_; ++r)
+                        for (int c = 0; c < dim_; ++c) {
+                            const size_t at = 2 * (static_cast<size_t>(r) * poisson_->ld() + c);
+                            if (dbg->field_x) dbg->field_x[static_cast<size_t>(r) * dim_ + c] = tmp[at];
+                            if (dbg->field_y) dbg->field_y[static_cast<size_t>(r) * dim_ + c] = tmp[at + 1];
+                        }
+                }
                 fetchT(dbg->force, dForce, 2 * static_cast<size_t>(n_));
                 fetchT(dbg->grad_wl, dWl, 2 * static_cast<size_t>(n_));
                 fetchT(dbg->grad, g_, 2 * static_cast<size_t>(n_));
@@ -312,6 +322,7 @@ private:
         }
         SPHIP_CHECK(hipMemcpyAsync(dd_.xy, xyStage_.data(), xyStage_.size() * sizeof(T),
                                    hipMemcpyHostToDevice, s_));
+        launchRecRefresh<T>(dd_, s_);
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
     std::vector<T> xyStage_;
@@ -355,6 +366,7 @@ private:
         for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
             if (!(mask & (1 << f))) continue;
             for (int i = 0; i < nslots; ++i) {
+                if (f == SP_HIP_K_SORT && i % kSortPeriod != 0) continue;   // no sort in slot i
                 float ms = 0;
                 SPHIP_CHECK(hipEventSynchronize(evStop_[f][i]));
                 SPHIP_CHECK(hipEventElapsedTime(&ms, evStart_[f][i], evStop_[f][i]));
@@ -569,7 +581,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     geom_.originY = T(originY_);
     geom_.target = T(target);
 
-    poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_);
+    poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_, true);
     geom_.ld = poisson_->ld();
 
     // device design
@@ -617,7 +629,16 @@ void Engine<T>::build(const sp_hip_design& d) {
         dd_.recIdB = static_cast<int*>(own(dupload<int>(recId, s_)));
         dd_.recWHA = static_cast<T*>(own(dupload<T>(recWH, s_)));
         dd_.recWHB = static_cast<T*>(own(dupload<T>(recWH, s_)));
-        dd_.recXY = static_cast<T*>(own(dalloc<T>(2 * std::max<size_t>(dens.size(), 1))));
+        std::vector<float> recXY(2 * dens.size());
+        std::vector<int> recPos(total, -1);
+        for (size_t t = 0; t < dens.size(); ++t) {
+            recXY[2 * t] = d.node_x[dens[t]];
+            recXY[2 * t + 1] = d.node_y[dens[t]];
+            recPos[dens[t]] = static_cast<int>(t);
+        }
+        dd_.recXYA = static_cast<T*>(own(dupload<T>(recXY, s_)));
+        dd_.recXYB = static_cast<T*>(own(dupload<T>(recXY, s_)));
+        dd_.recPos = static_cast<int*>(own(dupload<int>(recPos, s_)));
     }
     {
         TileGrid& tg = dd_.tiles;
@@ -629,10 +650,11 @@ void Engine<T>::build(const sp_hip_design& d) {
         tg.count = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
         tg.fill = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
         tg.start = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
+        tg.size = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
         tg.extraStart = static_cast<int*>(own(dalloc<int>(tg.nKeys + 1)));
         const size_t stageLen = static_cast<size_t>(tg.tilesX) * tg.tilesY * 2 * kTileWin * kTileWin;
         tg.stage = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(stageLen)));
-        for (int* a : {tg.count, tg.fill, tg.start})
+        for (int* a : {tg.count, tg.fill, tg.start, tg.size})
             SPHIP_CHECK(hipMemsetAsync(a, 0, tg.nKeys * sizeof(int), s_));
         SPHIP_CHECK(hipMemsetAsync(tg.extraStart, 0, (tg.nKeys + 1) * sizeof(int), s_));
         SPHIP_CHECK(hipMemsetAsync(tg.stage, 0, stageLen * 8, s_));
@@ -761,12 +783,11 @@ void Engine<T>::build(const sp_hip_design& d) {
     const double P = d.num_pins, E = d.num_nets, ts = sizeof(T), nd = dd_.nDens;
     const double stageBytes = static_cast<double>(dd_.tiles.tilesX) * dd_.tiles.tilesY * 2 *
                               kTileWin * kTileWin * 8;
-    // sort: pass 1 reads each record's node index and gathers (x, y); pass 2
-    // does the same, reads (w, h) and writes the record and (x, y) in order
-    bytesPer_[SP_HIP_K_SORT] = nd * ((4 + 2 * ts) + (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts));
-    // scatter: reads the ordered records and (x, y), writes the records back
-    // and every tile window
-    bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts) + stageBytes;
+    // sort: pass 1 reads each record's node index and (x, y); pass 2 reads the
+    // whole record, writes it in the new order and the node's record position
+    bytesPer_[SP_HIP_K_SORT] = nd * ((4 + 2 * ts) + (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts + 4));
+    // scatter: reads the ordered records, writes them back and every tile window
+    bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts) + stageBytes;
     // finalize: atomic grids read and cleared, windows read, static terrain, rho
     bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + 8 * 2 + ts) + stageBytes;
     bytesPer_[SP_HIP_K_POISSON] = 0;
@@ -779,12 +800,13 @@ void Engine<T>::build(const sp_hip_design& d) {
     // (x and y); per pin entry its gradient
     bytesPer_[SP_HIP_K_NODE_GRAD] =
         n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 8 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
-    // update: index, w/h, A/G/B reads, v/u writes, (x, y) write
-    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts);
-    // reduce: the block partials of the evaluation (8 doubles each)
+    // update: index, w/h, A/G/B reads, v/u writes, (x, y) writes (node and record)
+    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts + 4 + 2 * ts);
+    // reduce: the block partials of the evaluation (1, 2 and 7 doubles)
     bytesPer_[SP_HIP_K_REDUCE] =
-        64.0 * ((nb_ + kBlock - 1) / kBlock + (dd_.waWaveBase[kWaClasses + 1] + 3) / 4 +
-                (n_ + kBlock - 1) / kBlock);
+        8.0 * ((nb_ + kBlock - 1) / kBlock +
+               2.0 * std::min(kWaMaxBlocks, (dd_.waWaveBase[kWaClasses + 1] + 3) / 4) +
+               7.0 * ((n_ + kBlock - 1) / kBlock));
 }
 
 template <typename T>
@@ -792,12 +814,13 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
                                 float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
                                 T* dbgWl) {
     launchTileSort<T>(dd_, geom_, nullptr, s_);
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
+                     nullptr, s_);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
     poisson_->solve(false);
     launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, nullptr, s_);
-    launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), gamma, lambda, u, up,
+    launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), gamma, lambda, u, up,
                       gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
     launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
@@ -818,7 +841,8 @@ void Engine<T>::scatterPos(const T* pos) {
 template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
     launchTileSort<T>(dd_, geom_, nullptr, s_);
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, nullptr, s_);
+    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
+                     nullptr, s_);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
     launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
@@ -1094,11 +1118,16 @@ void Engine<T>::launchSlot(bool events, int slot) {
                                                    offsetof(Ctl, done));
     auto b = [&](int f) { if (events) beginK(f, slot); };
     auto e = [&](int f) { if (events) endK(f, slot); };
-    b(SP_HIP_K_SORT);
-    launchTileSort<T>(dd_, geom_, dCtl_, s_);
-    e(SP_HIP_K_SORT);
+    // the spatial order is rebuilt every kSortPeriod slots (slot 0 of every
+    // graph launch included); in between the scatter uses the last order
+    const bool sort = slot % kSortPeriod == 0;
+    if (sort) {
+        b(SP_HIP_K_SORT);
+        launchTileSort<T>(dd_, geom_, dCtl_, s_);
+        e(SP_HIP_K_SORT);
+    }
     b(SP_HIP_K_SCATTER);
-    launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, dCtl_, s_);
+    launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, sort, dCtl_, s_);
     e(SP_HIP_K_SCATTER);
     b(SP_HIP_K_FINALIZE);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
@@ -1111,7 +1140,7 @@ void Engine<T>::launchSlot(bool events, int slot) {
     launchWaNets<T>(dd_, 0.f, cfg_.ignore_net_degree, ws_, dCtl_, s_);
     e(SP_HIP_K_WA_NETS);
     b(SP_HIP_K_NODE_GRAD);
-    launchNodeGrad<T>(dd_, geom_, poisson_->fieldX(), poisson_->fieldY(), 0.f, 0.f, nullptr,
+    launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), 0.f, 0.f, nullptr,
                       nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, s_);
     e(SP_HIP_K_NODE_GRAD);
     b(SP_HIP_K_REDUCE);
@@ -1493,11 +1522,11 @@ int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b
         if (flags & SP_HIP_FLAG_FP64)
             sphip::launchGemm<double>(batch, reinterpret_cast<const double* const*>(a),
                                       reinterpret_cast<const double* const*>(b),
-                                      reinterpret_cast<double* const*>(c), m, n, k, k, n, n, nullptr);
+                                      reinterpret_cast<double* const*>(c), m, n, k, k, k, n, nullptr);
         else
             sphip::launchGemm<float>(batch, reinterpret_cast<const float* const*>(a),
                                      reinterpret_cast<const float* const*>(b),
-                                     reinterpret_cast<float* const*>(c), m, n, k, k, n, n, nullptr);
+                                     reinterpret_cast<float* const*>(c), m, n, k, k, k, n, nullptr);
         SPHIP_CHECK(hipDeviceSynchronize());
     });
 }

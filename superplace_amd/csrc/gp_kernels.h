@@ -14,9 +14,10 @@ namespace sphip {
 struct TileGrid {
     int tilesX = 0, tilesY = 0;   // tiles per grid side (the last ones may be partial)
     int nKeys = 0;                // Morton key range (a power of 4 >= tilesX * tilesY)
-    int* count = nullptr;         // nodes per key (pass 1)
+    int* count = nullptr;         // nodes per key (pass 1; cleared by the finalize pass)
     int* fill = nullptr;          // placement cursor per key (pass 2)
-    int* start = nullptr;         // first position of each key in densB
+    int* start = nullptr;         // first record position of each key (last sort)
+    int* size = nullptr;          // records of each key (last sort)
     int* extraStart = nullptr;    // nKeys + 1: first extra scatter block of each key
     unsigned long long* stage = nullptr;   // [tile][node | filler][kTileWin^2]
 };
@@ -34,15 +35,18 @@ struct DevDesign {
     const int* optIdx = nullptr;     // pos index k -> node i
     // The density nodes in spatial order (by tile of their centre bin, tiles
     // in Morton order) as records: node index with the filler / macro flags
-    // in bits 30 / 29 (recId), and (w, h) interleaved (recWH).  launchTileSort
-    // reads the previous order (A) and writes the new one (B) together with
-    // the nodes' current (x, y) (recXY); the scatter reads B and copies the
-    // records back to A.
+    // in bits 30 / 29 (recId), (w, h) and (x, y) interleaved (recWH, recXY).
+    // launchTileSort reads the previous order (A) and writes the new one (B);
+    // the scatter copies B back to A.  recPos[i] is node i's record position
+    // (-1: no density record), through which the update keeps recXYA current,
+    // so the sort reads positions in record order instead of gathering them.
     int* recIdA = nullptr;
     int* recIdB = nullptr;
     T* recWHA = nullptr;
     T* recWHB = nullptr;
-    T* recXY = nullptr;
+    T* recXYA = nullptr;
+    T* recXYB = nullptr;
+    int* recPos = nullptr;
     TileGrid tiles;
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
@@ -67,12 +71,14 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
+constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
 constexpr int kTileBins = 16;        // bins per tile side
 constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;
 constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 1;
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
 constexpr int kTileChunk = 1024;     // nodes per scatter block (a tile's owner takes the first)
+constexpr int kSortPeriod = 4;       // graph slots per spatial re-sort
 
 // FP64 reductions: every reducing kernel writes one partial per block (plain
 // stores, no atomics, no fences); k_reduce sums each region in block order
@@ -138,15 +144,23 @@ struct Ctl {
 // nodes in its window and stores the window to tiles.stage; extra blocks take
 // the further chunks of crowded tiles and add their windows to nodeQ / fillQ
 // with atomics, as do charges that fall outside a window (large nodes).
+// `sorted`: launchTileSort ran for this evaluation, so the scatter reads the
+// new order (B) and copies it to A; otherwise it reads A, the order of the
+// last sort with positions kept current by the update (charges of nodes that
+// left their tile's window since then go through the atomic grids).
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
-                   unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
-                   hipStream_t s);
+                   unsigned long long* nodeQ, unsigned long long* fillQ, bool sorted,
+                   const Ctl* ctl, hipStream_t s);
 
 // Re-sorts the density nodes by tile (counting sort; the order inside a tile
 // is arbitrary, which the integer density sums do not see).
 template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s);
+
+// Reloads recXYA from the node coordinates (after the host wrote them).
+template <typename T>
+void launchRecRefresh(const DevDesign<T>& d, hipStream_t s);
 
 // Sums the staged windows and the atomic grids into node / filler density,
 // then rho and the tau numerator; clears the atomic grids and tile counters.
@@ -161,7 +175,7 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
                   const Ctl* ctl, hipStream_t s);
 
 template <typename T>
-void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
+void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
                     T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
                     hipStream_t s);

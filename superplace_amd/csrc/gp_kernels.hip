@@ -45,17 +45,6 @@ __device__ inline void blockPartial(double (&v)[NV], double* partials) {
         for (int j = 0; j < NV; ++j) partials[static_cast<long>(blockIdx.x) * NV + j] = v[j];
 }
 
-// Sum of `nb` block partials of NV values (single block, all threads).
-template <int NV>
-__device__ inline void sumRegion(const double* partials, int nb, double (&out)[NV]) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) out[j] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-        for (int j = 0; j < NV; ++j) out[j] += partials[static_cast<long>(b) * NV + j];
-    blockSum<NV>(out);
-}
-
 // ------------------------------------------------------------------ footprint
 template <typename T>
 struct Footprint {
@@ -157,7 +146,7 @@ __device__ inline unsigned blockTileHist(const DevDesign<T>& d, const GridGeom<T
     unsigned key = 0xFFFFFFFFu;
     if (t < d.nDens) {
         const int rec = d.recIdA[t];
-        const auto p = load2(d.xy, rec & kRecNode);
+        const auto p = load2(d.recXYA, t);
         *outRec = rec;
         *outXY = p;
         key = tileKeyOf<T>(p.x, p.y, g);
@@ -243,10 +232,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
         const auto wh = load2(d.recWHA, t);
         d.recIdB[pos] = rec;
         store2(d.recWHB, pos, wh.x, wh.y);
-        store2(d.recXY, pos, p.x, p.y);
+        store2(d.recXYB, pos, p.x, p.y);
+        d.recPos[rec & kRecNode] = pos;
     }
     if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) tg.start[k] = prefix[k];
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
+            tg.start[k] = prefix[k];
+            tg.size[k] = tg.count[k];
+        }
         __syncthreads();
         for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
             const int c = tg.count[k];
@@ -271,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
-                                                    unsigned long long* fillQ, const Ctl* ctl) {
+                                                    unsigned long long* fillQ, int sorted,
+                                                    const Ctl* ctl) {
     if (ctl && ctl->done) return;
     constexpr int PER = kTileChunk / kBlock;
     constexpr int WIN2 = kTileWin * kTileWin;
@@ -286,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
         chunk = 0;
     } else {
         const int e = blockIdx.x - nTiles;
-        if (e >= tg.extraStart[tg.nKeys]) return;
+        if (e >= tg.extraStart[tg.nKeys]) return;   // no chunk for this block
         int lo = 0, hi = tg.nKeys - 1;   // last key with extraStart <= e
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -299,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
     }
     for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) win[e] = 0;
     __syncthreads();
-    const int cnt = tg.count[key], st = tg.start[key];
+    const int cnt = tg.size[key], st = tg.start[key];
     const int lo = st + chunk * kTileChunk;
     const int hi = st + min(cnt, (chunk + 1) * kTileChunk);
     const int wx0 = tx * kTileBins - kTileHalo, wy0 = ty * kTileBins - kTileHalo;
@@ -311,11 +305,20 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
         const int t = lo + threadIdx.x + j * kBlock;
         valid[j] = t < hi;
         if (!valid[j]) continue;
-        const int rec = d.recIdB[t];
-        const auto wh = load2(d.recWHB, t);
-        const auto p = load2(d.recXY, t);
-        d.recIdA[t] = rec;
-        store2(d.recWHA, t, wh.x, wh.y);
+        int rec;
+        typename Vec2<T>::type wh, p;
+        if (sorted) {
+            rec = d.recIdB[t];
+            wh = load2(d.recWHB, t);
+            p = load2(d.recXYB, t);
+            d.recIdA[t] = rec;
+            store2(d.recWHA, t, wh.x, wh.y);
+            store2(d.recXYA, t, p.x, p.y);
+        } else {
+            rec = d.recIdA[t];
+            wh = load2(d.recWHA, t);
+            p = load2(d.recXYA, t);
+        }
         fl[j] = static_cast<uint8_t>(((rec & kRecFiller) ? F_FILLER : 0) |
                                      ((rec & kRecMacro) ? F_MACRO : 0));
         fp[j] = footprintOf<T>(p.x, p.y, wh.x, wh.y, fl[j], g);
@@ -532,8 +535,8 @@ __device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lan
     s.bny = groupSum<W>(any); s.cny = groupSum<W>(dyn * any);
     const bool wa = deg >= 2 && deg <= ignoreDeg;
     if (j == 0 && deg >= 2) {
-        v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
-        if (wa) v[0] = waOf<T>(mxX, mnX, mxY, mnY, s);
+        v[1] += static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+        if (wa) v[0] += waOf<T>(mxX, mnX, mxY, mnY, s);
     }
     if (wa && pin) {
         T gx, gy;
@@ -561,7 +564,7 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
         mxY = fmax(mxY, py); mnY = fmin(mnY, py);
     }
     mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
-    if (lane == 0) v[1] = static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+    if (lane == 0) v[1] += static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
     if (e - b > ignoreDeg) return;
     WaSums<T> s{0, 0, 0, 0, 0, 0, 0, 0};
     for (int t = b + lane; t < e; t += 64) {
@@ -575,7 +578,7 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
     }
     s.bpx = waveSum(s.bpx); s.cpx = waveSum(s.cpx); s.bnx = waveSum(s.bnx); s.cnx = waveSum(s.cnx);
     s.bpy = waveSum(s.bpy); s.cpy = waveSum(s.cpy); s.bny = waveSum(s.bny); s.cny = waveSum(s.cny);
-    if (lane == 0) v[0] = waOf<T>(mxX, mnX, mxY, mnY, s);
+    if (lane == 0) v[0] += waOf<T>(mxX, mnX, mxY, mnY, s);
     for (int t = b + lane; t < e; t += 64) {
         const auto p = load2(d.xy, d.slotNode[t]);
         const T px = p.x + d.slotOffX[t], py = p.y + d.slotOffY[t];
@@ -603,19 +606,24 @@ __global__ __launch_bounds__(kBlock) void k_wa_pins(DevDesign<T> d, T invG, int 
         invG = invGammaOf<T>(ctl->gamma);
     }
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     double v[2] = {0.0, 0.0};
-    int c = 0;
-    while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
-    switch (c) {
-        case 0: waGroupPass<1, T>(d, 0, gw, lane, invG, ignoreDeg, v); break;
-        case 1: waGroupPass<2, T>(d, 1, gw, lane, invG, ignoreDeg, v); break;
-        case 2: waGroupPass<3, T>(d, 2, gw, lane, invG, ignoreDeg, v); break;
-        case 3: waGroupPass<4, T>(d, 3, gw, lane, invG, ignoreDeg, v); break;
-        case 4: waGroupPass<5, T>(d, 4, gw, lane, invG, ignoreDeg, v); break;
-        case 5: waGroupPass<6, T>(d, 5, gw, lane, invG, ignoreDeg, v); break;
-        case kWaClasses: waWavePass<T>(d, gw, lane, invG, ignoreDeg, v); break;
-        default: break;
+    // wavefront tasks are strided over the grid (a bounded number of block
+    // partials); each task belongs to one size class
+    const int tasks = d.waWaveBase[kWaClasses + 1];
+    for (int gw = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); gw < tasks;
+         gw += gridDim.x * (kBlock / 64)) {
+        int c = 0;
+        while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
+        switch (c) {
+            case 0: waGroupPass<1, T>(d, 0, gw, lane, invG, ignoreDeg, v); break;
+            case 1: waGroupPass<2, T>(d, 1, gw, lane, invG, ignoreDeg, v); break;
+            case 2: waGroupPass<3, T>(d, 2, gw, lane, invG, ignoreDeg, v); break;
+            case 3: waGroupPass<4, T>(d, 3, gw, lane, invG, ignoreDeg, v); break;
+            case 4: waGroupPass<5, T>(d, 4, gw, lane, invG, ignoreDeg, v); break;
+            case 5: waGroupPass<6, T>(d, 5, gw, lane, invG, ignoreDeg, v); break;
+            case kWaClasses: waWavePass<T>(d, gw, lane, invG, ignoreDeg, v); break;
+            default: break;
+        }
     }
     blockPartial<2>(v, partials);
 }
@@ -631,8 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_wa_pins(DevDesign<T> d, T invG, int 
 //    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
 //    (nesterov.cpp:25-44), plus a count of non-finite gradients.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* ex,
-                                                      const T* ey, T invG, float lambda,
+__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* exy,
+                                                      T invG, float lambda,
                                                       const T* u, const T* up, const T* gp, T* grad,
                                                       T* dbgForce, T* dbgWl, double* partials,
                                                       const Ctl* ctl, T* const* bufs) {
@@ -649,33 +657,65 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
     const int n = d.nOpt;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     if (k < n) {
+        // every load that does not depend on another is issued first
         const int i = d.optIdx[k];
+        const double u0 = u[k], u1 = u[n + k];
+        const double up0 = up ? static_cast<double>(up[k]) : 0.0;
+        const double up1 = up ? static_cast<double>(up[n + k]) : 0.0;
+        const double gp0 = gp ? static_cast<double>(gp[k]) : 0.0;
+        const double gp1 = gp ? static_cast<double>(gp[n + k]) : 0.0;
+        const double pdeg = d.precDeg[k], parea = d.precArea[k];
         const auto pi = load2(d.xy, i);
-        const T xi = pi.x, yi = pi.y;
-        T gx = 0, gy = 0;
-        const bool hasWl = i < d.numMov;
-        if (hasWl)
-            for (int t = d.nodeStart[i]; t < d.nodeStart[i + 1]; ++t) {
-                gx += d.pinGrad[2L * t];
-                gy += d.pinGrad[2L * t + 1];
-            }
-        T fx = 0, fy = 0;
+        const T wi = d.w[i], hi = d.h[i];
         const uint8_t f = d.flags[i];
-        if (!(f & (F_FIXED | F_NI))) {
-            const Footprint<T> fp = footprintOf<T>(xi, yi, d.w[i], d.h[i], f, g);
-            for (int bx = fp.bx0; bx <= fp.bx1; ++bx)
-                for (int by = fp.by0; by <= fp.by1; ++by) {
-                    const T q = binCharge<T>(bx, by, fp, g);
-                    if (q <= T(0)) continue;
-                    const long idx = static_cast<long>(bx) * g.ld + by;
-                    fx += q * ex[idx];
-                    fy += q * ey[idx];
+        const bool hasWl = i < d.numMov;
+        const int t0 = hasWl ? d.nodeStart[i] : 0, t1 = hasWl ? d.nodeStart[i + 1] : 0;
+        const T xi = pi.x, yi = pi.y;
+        // WA gradient: the node's pin entries, four loads in flight at a time
+        T gx = 0, gy = 0;
+        for (int t = t0; t < t1; t += 4) {
+            typename Vec2<T>::type pg[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (t + j < t1) pg[j] = load2(d.pinGrad, t + j);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (t + j < t1) {
+                    gx += pg[j].x;
+                    gy += pg[j].y;
                 }
         }
+        // density force over the footprint (bins in x-major order)
+        T fx = 0, fy = 0;
+        if (!(f & (F_FIXED | F_NI))) {
+            const Footprint<T> fp = footprintOf<T>(xi, yi, wi, hi, f, g);
+            const int nby = fp.by1 - fp.by0 + 1;
+            const int nbin = (fp.bx1 - fp.bx0 + 1) * nby;
+            for (int e0 = 0; e0 < nbin; e0 += 4) {
+                T q[4], fxv[4], fyv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    q[j] = T(0);
+                    if (e0 + j < nbin) {
+                        const int bx = fp.bx0 + (e0 + j) / nby, by = fp.by0 + (e0 + j) % nby;
+                        q[j] = binCharge<T>(bx, by, fp, g);
+                        if (q[j] > T(0)) {
+                            const auto fe = load2(exy, static_cast<long>(bx) * g.ld + by);
+                            fxv[j] = fe.x;
+                            fyv[j] = fe.y;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (q[j] > T(0)) {
+                        fx += q[j] * fxv[j];
+                        fy += q[j] * fyv[j];
+                    }
+            }
+        }
         const T lam = static_cast<T>(lambda);
-        const T precond = static_cast<T>(
-            1.0 / fmax(1.0, static_cast<double>(d.precDeg[k]) +
-                                static_cast<double>(lambda) * d.precArea[k]));
+        const T precond = static_cast<T>(1.0 / fmax(1.0, pdeg + static_cast<double>(lambda) * parea));
         const T g0 = precond * (gx - lam * fx);
         const T g1 = precond * (gy - lam * fy);
         grad[k] = g0;
@@ -684,11 +724,9 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
         if (dbgWl) { dbgWl[k] = gx; dbgWl[n + k] = gy; }
         v[0] = hasWl ? fabs(static_cast<double>(gx)) + fabs(static_cast<double>(gy)) : 0.0;
         v[1] = fabs(static_cast<double>(fx)) + fabs(static_cast<double>(fy));
-        const double u0 = u[k], u1 = u[n + k];
-        const double du0 = u0 - (up ? static_cast<double>(up[k]) : 0.0);
-        const double du1 = u1 - (up ? static_cast<double>(up[n + k]) : 0.0);
-        const double dg0 = static_cast<double>(g0) - (gp ? static_cast<double>(gp[k]) : 0.0);
-        const double dg1 = static_cast<double>(g1) - (gp ? static_cast<double>(gp[n + k]) : 0.0);
+        const double du0 = u0 - up0, du1 = u1 - up1;
+        const double dg0 = static_cast<double>(g0) - gp0;
+        const double dg1 = static_cast<double>(g1) - gp1;
         v[2] = du0 * du0 + du1 * du1;
         v[3] = dg0 * dg0 + dg1 * dg1;
         v[4] = du0 * dg0 + du1 * dg1;
@@ -881,17 +919,33 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
     if (ctl)
         for (int w = threadIdx.x; w < kCtlWords; w += blockDim.x)
             ctlWords[w] = reinterpret_cast<const unsigned*>(ctl)[w];
-    double r0[1], r1[2], r2[2], r3[7];
-    sumRegion<1>(ws.region(0), ws.blocks[0], r0);
-    sumRegion<2>(ws.region(1), ws.blocks[1], r1);
-    sumRegion<2>(ws.region(2), ws.blocks[2], r2);
-    if (withNode) sumRegion<7>(ws.region(3), ws.blocks[3], r3);   // ends with a barrier
+    // all regions in one pass (every load independent), one block sum
+    double r[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) r[j] = 0.0;
+    {
+        const double* p0 = ws.region(0);
+        const double* p1 = ws.region(1);
+        const double* p2 = ws.region(2);
+        const double* p3 = ws.region(3);
+        const int n3 = withNode ? ws.blocks[3] : 0;
+        const int nmax = max(max(ws.blocks[0], ws.blocks[1]), max(ws.blocks[2], n3));
+        for (int b = threadIdx.x; b < nmax; b += blockDim.x) {
+            if (b < ws.blocks[0]) r[0] += p0[b];
+            if (b < ws.blocks[1]) { r[1] += p1[2L * b]; r[2] += p1[2L * b + 1]; }
+            if (b < ws.blocks[2]) { r[3] += p2[2L * b]; r[4] += p2[2L * b + 1]; }
+            if (b < n3)
+#pragma unroll
+                for (int j = 0; j < 7; ++j) r[5 + j] += p3[7L * b + j];
+        }
+    }
+    blockSum<12>(r);   // ends with a barrier
     __shared__ double sc[kNumScalars];
     if (threadIdx.x == 0) {
-        sc[0] = denom > 0.0 ? r0[0] / denom : 0.0;
-        sc[1] = r1[0] + r2[0];
-        sc[2] = r1[1] + r2[1];
-        for (int j = 0; j < 7; ++j) sc[3 + j] = withNode ? r3[j] : 0.0;
+        sc[0] = denom > 0.0 ? r[0] / denom : 0.0;
+        sc[1] = r[1];
+        sc[2] = r[2];
+        for (int j = 0; j < 7; ++j) sc[3 + j] = withNode ? r[5 + j] : 0.0;
         for (int j = 0; j < kNumScalars; ++j) ws.scalars[j] = sc[j];
         if (ctl && withNode) decideStep<T>(*reinterpret_cast<Ctl*>(ctlWords), sc, rows);
     }
@@ -952,7 +1006,18 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
         if (vOut) vOut[j] = vv[a];
         if (uOut) uOut[j] = uu[a];
     }
-    store2(d.xy, i, uOut ? uu[0] : vv[0], uOut ? uu[1] : vv[1]);
+    const T nx = uOut ? uu[0] : vv[0], ny = uOut ? uu[1] : vv[1];
+    store2(d.xy, i, nx, ny);
+    const int rp = d.recPos[i];
+    if (rp >= 0) store2(d.recXYA, rp, nx, ny);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rec_refresh(DevDesign<T> d) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nDens) return;
+    const auto p = load2(d.xy, d.recIdA[t] & kRecNode);
+    store2(d.recXYA, t, p.x, p.y);
 }
 
 long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
@@ -961,10 +1026,18 @@ long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
 
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
-                   unsigned long long* nodeQ, unsigned long long* fillQ, const Ctl* ctl,
-                   hipStream_t s) {
+                   unsigned long long* nodeQ, unsigned long long* fillQ, bool sorted,
+                   const Ctl* ctl, hipStream_t s) {
     const int blocks = d.tiles.tilesX * d.tiles.tilesY + (d.nDens + kTileChunk - 1) / kTileChunk;
-    launchKernel(k_scatter<T>, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ, ctl);
+    launchKernel(k_scatter<T>, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ,
+                 sorted ? 1 : 0, ctl);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launchRecRefresh(const DevDesign<T>& d, hipStream_t s) {
+    if (d.nDens == 0) return;
+    launchKernel(k_rec_refresh<T>, gridFor(d.nDens), kBlock, 0, s, d);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -996,21 +1069,21 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
                   const Ctl* ctl, hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
     const int waves = d.waWaveBase[kWaClasses + 1];
-    ws.blocks[1] = std::max(1, (waves + kBlock / 64 - 1) / (kBlock / 64));
+    ws.blocks[1] = std::max(1, std::min(kWaMaxBlocks, (waves + kBlock / 64 - 1) / (kBlock / 64)));
     ws.blocks[2] = 0;
     launchKernel(k_wa_pins<T>, ws.blocks[1], kBlock, 0, s, d, invG, ignoreDeg, ws.region(1), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* ex, const T* ey,
+void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
                     T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
                     hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
     ws.blocks[3] = static_cast<int>(gridFor(d.nOpt));
-    launchKernel(k_node_grad<T>, ws.blocks[3], kBlock, 0, s, d, g, ex, ey, invG, lambda, u, up, gp, grad,
-                                                  dbgForce, dbgWl, ws.region(3), ctl, bufs);
+    launchKernel(k_node_grad<T>, ws.blocks[3], kBlock, 0, s, d, g, exy, invG, lambda, u, up, gp,
+                 grad, dbgForce, dbgWl, ws.region(3), ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -1031,8 +1104,9 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
 
 #define SPHIP_INST(T)                                                                            \
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
-                                   unsigned long long*, unsigned long long*, const Ctl*,         \
+                                   unsigned long long*, unsigned long long*, bool, const Ctl*,   \
                                    hipStream_t);                                                 \
+    template void launchRecRefresh<T>(const DevDesign<T>&, hipStream_t);                        \
     template void launchTileSort<T>(const DevDesign<T>&, const GridGeom<T>&, const Ctl*,         \
                                     hipStream_t);                                                \
     template void launchFinalize<T>(const GridGeom<T>&, const TileGrid&, unsigned long long*,     \
@@ -1041,7 +1115,7 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
                                     long long*, long long*, ReduceWs&, const Ctl*, hipStream_t); \
     template void launchWaNets<T>(const DevDesign<T>&, float, int, ReduceWs&, const Ctl*,        \
                                   hipStream_t);                                                  \
-    template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*, const T*, \
+    template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*,           \
                                     float, float, const T*, const T*, const T*, T*, T*, T*,      \
                                     ReduceWs&, const Ctl*, T* const*, hipStream_t);              \
     template void launchReduce<T>(ReduceWs&, double, bool, Ctl*, DevRow*, hipStream_t);         \

@@ -21,7 +21,10 @@ namespace sphip {
 template <typename T>
 class PoissonPlan {
 public:
-    PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream);
+    // interleaved: the field is written as (xi_x, xi_y) pairs into fieldXY()
+    // (element 2 (x * ld + y) + axis) instead of fieldX() / fieldY().
+    PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
+                bool interleaved = false);
     ~PoissonPlan();
     PoissonPlan(const PoissonPlan&) = delete;
     PoissonPlan& operator=(const PoissonPlan&) = delete;
@@ -39,6 +42,7 @@ public:
     T* phi() { return phi_; }
     T* fieldX() { return ex_; }
     T* fieldY() { return ey_; }
+    T* fieldXY() { return exy_; }
     hipStream_t stream() const { return stream_; }
     // Floating-point operations of one solve (dense products actually executed).
     double flops(bool withPhi) const;
@@ -46,7 +50,8 @@ public:
 private:
     int nx_, ny_, mx_, my_;
     hipStream_t stream_;
-    T *rho_ = nullptr, *phi_ = nullptr, *ex_ = nullptr, *ey_ = nullptr;
+    bool interleaved_;
+    T *rho_ = nullptr, *phi_ = nullptr, *ex_ = nullptr, *ey_ = nullptr, *exy_ = nullptr;
     T *t0_ = nullptr, *p_ = nullptr, *t1_ = nullptr, *t2_ = nullptr;
     T *cyT_ = nullptr, *cx_ = nullptr, *ayT_ = nullptr, *bsv_ = nullptr, *sxu_ = nullptr,
       *ax_ = nullptr;
@@ -54,8 +59,8 @@ private:
     T invNorm_;
 };
 
-// Batched C = A * B (row-major, dims multiples of 64) on MFMA; used by the
-// plan and exposed for the GEMM unit test.
+// Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on
+// MFMA; the plan's primitive, exposed for the GEMM unit test.
 template <typename T>
 void launchGemm(int batch, const T* const* A, const T* const* B, T* const* C, int M, int N,
                 int K, int lda, int ldb, int ldc, hipStream_t stream);

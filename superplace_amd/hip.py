@@ -231,7 +231,8 @@ def make_poisson_backend(cfg: BackendConfig) -> HipPoissonBackend:
 
 
 def gemm(a_ptrs, b_ptrs, c_ptrs, m, n, k, fp64=False):
-    """Batched C = A B on device pointers (row-major, multiples of 64)."""
+    """Batched C = A B^T on device pointers (A m x k, B n x k, row-major,
+    multiples of 64)."""
     batch = len(a_ptrs)
     arr = lambda ps: (_P * batch)(*[_P(p) for p in ps])
     check(lib().sp_hip_gemm(SP_HIP_FLAG_FP64 if fp64 else 0, batch, arr(a_ptrs), arr(b_ptrs),

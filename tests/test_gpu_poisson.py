@@ -137,12 +137,12 @@ def test_gemm_matches_numpy(fp64, mnk):
     rng = np.random.default_rng(3)
     m, n, k = mnk
     A = [rng.standard_normal((m, k)).astype(dt) for _ in range(3)]
-    B = [rng.standard_normal((k, n)).astype(dt) for _ in range(3)]
+    B = [rng.standard_normal((n, k)).astype(dt) for _ in range(3)]   # C = A B^T
     dA, dB = [DevArray(a) for a in A], [DevArray(b) for b in B]
     dC = [DevArray.zeros((m, n), dt) for _ in range(3)]
     hip.gemm([a.ptr for a in dA], [b.ptr for b in dB], [c.ptr for c in dC], m, n, k, fp64=fp64)
     for a, b, c in zip(A, B, dC):
-        ref = a.astype(np.float64) @ b.astype(np.float64)
+        ref = a.astype(np.float64) @ b.astype(np.float64).T
         err = np.abs(c.get().astype(np.float64) - ref).max() / np.abs(ref).max()
         # FP32: k products accumulated in FP32, about sqrt(k) * 2^-24 relative
         assert err < (1e-13 if fp64 else 1e-5), err
