@@ -180,9 +180,10 @@ typedef struct sp_hip_gp_info {
 
 /* Kernel families timed with HIP events on the engine's stream in the
  * device-driven loop (enabled by sp_hip_gp_set_profiling, which makes
- * loop_mode 2 launch its slots directly, as loop_mode 1 does): the family's
- * first kernel records the start event as its dispatch begins and its last
- * kernel the stop event as it ends (hipExtLaunchKernel).  Each family is one kernel, except POISSON (the
+ * loop_mode 2 launch its slots directly, as loop_mode 1 does): every kernel
+ * of a family carries its own start/stop event pair (hipExtLaunchKernel, the
+ * dispatch's own timestamps), and one launch of the family counts the sum of
+ * its kernels' durations.  Each family is one kernel, except POISSON (the
  * transform products of one solve) and SORT (the two passes of the spatial
  * sort). */
 enum { SP_HIP_K_SCATTER = 0, SP_HIP_K_FINALIZE, SP_HIP_K_POISSON, SP_HIP_K_WA_NETS,

@@ -53,22 +53,26 @@ struct GridGeom {
     T target;       // target density (charge weight of macros)
 };
 
-// Optional event pair timing the kernels of one family: while set, the first
-// kernel launched records `start` as it begins and every kernel records
-// `stop` as it ends (hipExtLaunchKernel: the timestamps of the dispatch
-// itself, without the launch gaps).  Only for launches outside graph capture.
+// Optional event pairs timing the kernels of one family: while set, the
+// k-th kernel launched records start[k] as its dispatch begins and stop[k]
+// as it ends (hipExtLaunchKernel: the timestamps of the dispatch itself,
+// without the gaps between launches).  Only for launches outside graph
+// capture; at most kMaxTimed kernels per family.
+constexpr int kMaxTimed = 4;
 struct LaunchTimer {
-    hipEvent_t start = nullptr, stop = nullptr;
+    hipEvent_t start[kMaxTimed] = {}, stop[kMaxTimed] = {};
+    int used = 0;
 };
 inline thread_local LaunchTimer* tlsTimer = nullptr;
 
 template <typename... Params, typename... Args>
 inline void launchKernel(void (*kernel)(Params...), dim3 grid, dim3 block, uint32_t shm,
                          hipStream_t s, Args... args) {
-    if (LaunchTimer* t = tlsTimer) {
-        hipExtLaunchKernelGGL(kernel, grid, block, shm, s, t->start, t->stop, 0,
+    LaunchTimer* t = tlsTimer;
+    if (t && t->used < kMaxTimed) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, s, t->start[t->used], t->stop[t->used], 0,
                               static_cast<Params>(args)...);
-        t->start = nullptr;
+        ++t->used;
     } else {
         kernel<<<grid, block, shm, s>>>(static_cast<Params>(args)...);
     }

@@ -333,43 +333,56 @@ private:
                          T* dbgForce = nullptr, T* dbgWl = nullptr);
     void scatterPos(const T* pos);   // clamp + write into node coordinates
     double deliveredMetrics(double* hpwlOut);
-    // HIP event pairs per kernel family and graph slot (created on demand)
+    // HIP event pairs per kernel family, graph slot and kernel of the family
+    // (created on demand; evStart_[f][slot * kMaxTimed + k])
     void ensureEvents(int slots) {
         for (int f = 0; f < SP_HIP_K_COUNT; ++f)
-            while (static_cast<int>(evStart_[f].size()) < slots) {
+            while (static_cast<int>(evStart_[f].size()) < slots * kMaxTimed) {
                 hipEvent_t a = nullptr, b = nullptr;
                 SPHIP_CHECK(hipEventCreate(&a));
                 evStart_[f].push_back(a);
                 SPHIP_CHECK(hipEventCreate(&b));
                 evStop_[f].push_back(b);
-                // record once on the stream: the runtime times an event pair
-                // recorded by graph nodes only after a direct record
-                SPHIP_CHECK(hipEventRecord(a, s_));
-                SPHIP_CHECK(hipEventRecord(b, s_));
             }
-        SPHIP_CHECK(hipStreamSynchronize(s_));
+        evUsed_.assign(static_cast<size_t>(SP_HIP_K_COUNT) * slots, 0);
     }
     // The kernels of family f launched between beginK and endK carry the
-    // slot's event pair (launchKernel / hipExtLaunchKernel).
+    // slot's event pairs (launchKernel / hipExtLaunchKernel).
     void beginK(int f, int slot) {
         if (profMask_ & (1 << f)) {
-            timer_ = LaunchTimer{evStart_[f][slot], evStop_[f][slot]};
+            timer_ = LaunchTimer{};
+            for (int k = 0; k < kMaxTimed; ++k) {
+                timer_.start[k] = evStart_[f][slot * kMaxTimed + k];
+                timer_.stop[k] = evStop_[f][slot * kMaxTimed + k];
+            }
             tlsTimer = &timer_;
         }
     }
-    void endK(int f, int /*slot*/) {
-        if (profMask_ & (1 << f)) tlsTimer = nullptr;
+    void endK(int f, int slot) {
+        if (profMask_ & (1 << f)) {
+            tlsTimer = nullptr;
+            evUsed_[static_cast<size_t>(f) * (evUsed_.size() / SP_HIP_K_COUNT) + slot] = timer_.used;
+        }
     }
     LaunchTimer timer_;
-    // Adds the durations of slots [0, nslots) of every profiled family.
+    std::vector<int> evUsed_;   // kernels timed per (family, slot) in the last launch of the slot
+    // Adds, per profiled family, the summed kernel durations of slots
+    // [0, nslots) (one launch of the family per slot that ran it).
     void harvestK(int mask, int nslots) {
+        const size_t perF = evUsed_.size() / SP_HIP_K_COUNT;
         for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
             if (!(mask & (1 << f))) continue;
             for (int i = 0; i < nslots; ++i) {
-                if (f == SP_HIP_K_SORT && i % kSortPeriod != 0) continue;   // no sort in slot i
-                float ms = 0;
-                SPHIP_CHECK(hipEventSynchronize(evStop_[f][i]));
-                SPHIP_CHECK(hipEventElapsedTime(&ms, evStart_[f][i], evStop_[f][i]));
+                const int used = evUsed_[f * perF + i];
+                if (used == 0) continue;   // the family did not run in slot i
+                double ms = 0;
+                for (int k = 0; k < used; ++k) {
+                    float part = 0;
+                    SPHIP_CHECK(hipEventSynchronize(evStop_[f][i * kMaxTimed + k]));
+                    SPHIP_CHECK(hipEventElapsedTime(&part, evStart_[f][i * kMaxTimed + k],
+                                                    evStop_[f][i * kMaxTimed + k]));
+                    ms += part;
+                }
                 kms_[f] += ms;
                 klaunch_[f] += 1;
                 kbytes_[f] += bytesPer_[f];
