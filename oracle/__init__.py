@@ -243,6 +243,23 @@ def r2r_1d(kind, x):
     return out
 
 
+def r2r_1d_fast(kind, x):
+    """The FFT path of the same transform (power-of-two length)."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    L = lib()
+    L.ora_r2r_1d_fast.argtypes = [C.c_int, C.c_int, _P, _P]
+    L.ora_r2r_1d_fast(int(kind), len(x), _ptr(x), _ptr(out))
+    return out
+
+
+def set_fast_poisson(on: bool):
+    """Selects the FFT transforms (power-of-two grids) for poisson / eplace_run."""
+    L = lib()
+    L.ora_set_fast_poisson.argtypes = [C.c_int]
+    L.ora_set_fast_poisson(1 if on else 0)
+
+
 def wa(d, gamma, ignore_net_degree=100, precision=64, x=None, y=None):
     ds = design_struct(d)
     x = _f32(d.node_x if x is None else x)

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -332,9 +333,139 @@ void applyX(const std::vector<double>& M, int nx, int ny, const double* in, doub
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same three transforms in O(n log n) for power-of-two n (the CPU
+// baseline; FFTW's role in the reference).  Each is one complex FFT of
+// length 2n (iterative radix-2, double):
+//   REDFT10: Y_k = 2 Re(e^{-i pi k / 2n} F_k),  F = DFT_2n(X zero-padded)
+//   REDFT01: Y_k = Re(G_k),  G = inverse DFT_2n (unscaled) of c_j X_j e^{i pi j / 2n},
+//            c_0 = 1, c_j = 2
+//   RODFT01: Y_k = Im(H_k),  H = inverse DFT_2n (unscaled) of d_m X_{m-1} e^{i pi m / 2n},
+//            m = 1..n, d_m = 2 (m < n), d_n = 1
+// (cos(pi (j+1/2) k / n) = Re e^{-i pi (2j+1) k / 2n}, and likewise for the
+// other two).  Checked against the direct sums in the oracle tests.
+struct Fft {
+    int n = 0;   // complex length (2 * transform length)
+    std::vector<std::complex<double>> tw;
+    std::vector<int> rev;
+    explicit Fft(int len) : n(len), tw(len / 2), rev(len) {
+        const double pi = 3.14159265358979323846;
+        for (int k = 0; k < n / 2; ++k) tw[k] = std::polar(1.0, -2.0 * pi * k / n);
+        int lg = 0;
+        while ((1 << lg) < n) ++lg;
+        for (int i = 0; i < n; ++i) {
+            int r = 0;
+            for (int b = 0; b < lg; ++b) r |= ((i >> b) & 1) << (lg - 1 - b);
+            rev[i] = r;
+        }
+    }
+    // in place; inverse = conjugate twiddles, no scaling
+    void run(std::complex<double>* a, bool inverse) const {
+        for (int i = 0; i < n; ++i)
+            if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+        for (int len = 2; len <= n; len <<= 1) {
+            const int step = n / len;
+            for (int i = 0; i < n; i += len)
+                for (int j = 0; j < len / 2; ++j) {
+                    std::complex<double> w = tw[j * step];
+                    if (inverse) w = std::conj(w);
+                    const std::complex<double> u = a[i + j], v = a[i + j + len / 2] * w;
+                    a[i + j] = u + v;
+                    a[i + j + len / 2] = u - v;
+                }
+        }
+    }
+};
+
+bool isPow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+// One transform of `kind` (as r2rMatrix) on a length-n vector, scratch of 2n.
+void r2rFast(const Fft& f, int kind, int n, const double* in, double* out,
+             std::complex<double>* buf) {
+    const double pi = 3.14159265358979323846;
+    const int m = 2 * n;
+    std::fill(buf, buf + m, std::complex<double>(0.0, 0.0));
+    if (kind == 0) {
+        for (int j = 0; j < n; ++j) buf[j] = in[j];
+        f.run(buf, false);
+        for (int k = 0; k < n; ++k)
+            out[k] = 2.0 * (std::polar(1.0, -pi * k / m) * buf[k]).real();
+    } else if (kind == 1) {
+        for (int j = 0; j < n; ++j) buf[j] = (j == 0 ? 1.0 : 2.0) * in[j] * std::polar(1.0, pi * j / m);
+        f.run(buf, true);
+        for (int k = 0; k < n; ++k) out[k] = buf[k].real();
+    } else {
+        for (int mm = 1; mm <= n; ++mm)
+            buf[mm] = (mm < n ? 2.0 : 1.0) * in[mm - 1] * std::polar(1.0, pi * mm / m);
+        f.run(buf, true);
+        for (int k = 0; k < n; ++k) out[k] = buf[k].imag();
+    }
+}
+
+// out[x][.] = transform of in[x][.] (fast axis) or of in[.][y] (slow axis)
+void applyFast(int kind, bool slowAxis, int nx, int ny, const double* in, double* out) {
+    const int n = slowAxis ? nx : ny, lines = slowAxis ? ny : nx;
+    const Fft f(2 * n);
+#pragma omp parallel
+    {
+        std::vector<std::complex<double>> buf(2 * n);
+        std::vector<double> a(n), b(n);
+#pragma omp for schedule(static)
+        for (int l = 0; l < lines; ++l) {
+            for (int t = 0; t < n; ++t)
+                a[t] = slowAxis ? in[static_cast<size_t>(t) * ny + l] : in[static_cast<size_t>(l) * ny + t];
+            r2rFast(f, kind, n, a.data(), b.data(), buf.data());
+            for (int t = 0; t < n; ++t) {
+                if (slowAxis) out[static_cast<size_t>(t) * ny + l] = b[t];
+                else out[static_cast<size_t>(l) * ny + t] = b[t];
+            }
+        }
+    }
+}
+
+bool gFastPoisson = false;   // ora_set_fast_poisson
+
+// The solve of `poisson` with the O(n log n) transforms (same steps).
+void poissonFast(int nx, int ny, double stepX, double stepY, const double* rho, double* phi,
+                 double* ex, double* ey) {
+    const size_t n = static_cast<size_t>(nx) * ny;
+    const double pi = 3.14159265358979323846;
+    std::vector<double> tmp(n), spec(n), work(n);
+    applyFast(0, false, nx, ny, rho, tmp.data());
+    applyFast(0, true, nx, ny, tmp.data(), spec.data());
+    std::vector<double> u(nx), v(ny);
+    for (int j = 0; j < nx; ++j) u[j] = pi * j / (nx * stepX);
+    for (int k = 0; k < ny; ++k) v[k] = pi * k / (ny * stepY);
+    const double invNorm = 1.0 / (4.0 * nx * ny);
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k) {
+            const double den = u[j] * u[j] + v[k] * v[k];
+            double& p = spec[static_cast<size_t>(j) * ny + k];
+            p = den > 0 ? p * invNorm / den : 0.0;
+        }
+    applyFast(1, false, nx, ny, spec.data(), tmp.data());
+    applyFast(1, true, nx, ny, tmp.data(), phi);
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k)
+            work[static_cast<size_t>(j) * ny + k] =
+                (j == nx - 1) ? 0.0 : spec[static_cast<size_t>(j + 1) * ny + k] * u[j + 1];
+    applyFast(1, false, nx, ny, work.data(), tmp.data());
+    applyFast(2, true, nx, ny, tmp.data(), ex);
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < ny; ++k)
+            work[static_cast<size_t>(j) * ny + k] =
+                (k == ny - 1) ? 0.0 : spec[static_cast<size_t>(j) * ny + k + 1] * v[k + 1];
+    applyFast(2, false, nx, ny, work.data(), tmp.data());
+    applyFast(1, true, nx, ny, tmp.data(), ey);
+}
+
 // CpuPoissonBackend::solve, cpu_poisson.cpp:149-207, evaluated in double.
 void poisson(int nx, int ny, double stepX, double stepY, const double* rho, double* phi,
              double* ex, double* ey) {
+    if (gFastPoisson && isPow2(nx) && isPow2(ny)) {
+        poissonFast(nx, ny, stepX, stepY, rho, phi, ex, ey);
+        return;
+    }
     const size_t n = static_cast<size_t>(nx) * ny;
     const double pi = 3.14159265358979323846;
     std::vector<double> tmp(n), spec(n), work(n);
@@ -979,6 +1110,14 @@ void ora_poisson(int nx, int ny, double stepX, double stepY, const double* rho, 
         }
     }
 }
+
+void ora_r2r_1d_fast(int kind, int n, const double* in, double* out) {
+    const Fft f(2 * n);
+    std::vector<std::complex<double>> buf(2 * n);
+    r2rFast(f, kind, n, in, out, buf.data());
+}
+
+void ora_set_fast_poisson(int on) { gFastPoisson = on != 0; }
 
 void ora_r2r_1d(int kind, int n, const double* in, double* out) {
     const auto m = r2rMatrix(kind, n);

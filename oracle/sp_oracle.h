@@ -92,6 +92,12 @@ void ora_poisson(int nx, int ny, double step_x, double step_y, const double* rho
 /* The three 1-D transforms themselves (length n, unnormalised, FFTW's
  * definitions); kind 0 = REDFT10, 1 = REDFT01, 2 = RODFT01. */
 void ora_r2r_1d(int kind, int n, const double* in, double* out);
+/* The same transforms through one complex FFT of length 2n (n a power of
+ * two): the O(n log n) path of the CPU baseline. */
+void ora_r2r_1d_fast(int kind, int n, const double* in, double* out);
+/* on != 0: ora_poisson and the stage loop use the FFT path for power-of-two
+ * grids (the CPU baseline); 0 (default): direct summation. */
+void ora_set_fast_poisson(int on);
 
 /* ---- wirelength ------------------------------------------------------------ */
 /* computeGamma, src/gp/wa_wirelength.cpp:64-75 (FP32 as in the reference). */
