@@ -26,16 +26,24 @@ def rel(a, b):
 
 
 def designs():
+    # crowd: 5,000 cells started inside a few bins, so single 16x16-bin tiles
+    # hold more than one scatter block's worth of nodes (the extra-block path),
+    # on a 100x100 grid whose last tiles are partial
     return {"fixture": D.with_fillers(D.fixture_eplace(3000, 12)),
-            "synth2k": D.config_small()}
+            "synth2k": D.config_small(),
+            "crowd": D.synth_design(5000, 4, 5500, 670.0, 0.7, init_sigma_frac=0.01, seed=3,
+                                    name="crowd")}
+
+
+BIN_DIM = {"fixture": 0, "synth2k": 128, "crowd": 100}
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("name", ["fixture", "synth2k"])
+@pytest.mark.parametrize("name", ["fixture", "synth2k", "crowd"])
 def test_eval_matches_oracle(name, prec):
     d = designs()[name]
     tol = TOL32 if prec == 32 else TOL64
-    eng = hip.GpEngine(d, hip.GpConfig(precision=prec, gp_bin_dim=128 if name == "synth2k" else 0))
+    eng = hip.GpEngine(d, hip.GpConfig(precision=prec, gp_bin_dim=BIN_DIM[name]))
     lam, gam = 0.37, 25.0
     sc, out = eng.eval(lam=lam, gamma=gam, debug=True)
     print(name, prec, "dim", eng.dim, "frac", eng.frac_bits, flush=True)
