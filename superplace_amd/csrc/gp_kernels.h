@@ -71,7 +71,7 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
-constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
+constexpr int kWaMaxBlocks = 1024;   // the WA pass strides its wavefront tasks over at most these
 constexpr int kTileBins = 16;        // bins per tile side
 constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;

@@ -14,6 +14,7 @@ namespace sphip {
 namespace {
 
 // ------------------------------------------------------------------ reductions
+constexpr int kReduceBatch = 8;   // block partials per thread loaded together by k_reduce
 template <int NV>
 __device__ inline void blockSum(double (&v)[NV]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -930,13 +931,26 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
         const double* p3 = ws.region(3);
         const int n3 = withNode ? ws.blocks[3] : 0;
         const int nmax = max(max(ws.blocks[0], ws.blocks[1]), max(ws.blocks[2], n3));
-        for (int b = threadIdx.x; b < nmax; b += blockDim.x) {
-            if (b < ws.blocks[0]) r[0] += p0[b];
-            if (b < ws.blocks[1]) { r[1] += p1[2L * b]; r[2] += p1[2L * b + 1]; }
-            if (b < ws.blocks[2]) { r[3] += p2[2L * b]; r[4] += p2[2L * b + 1]; }
-            if (b < n3)
+        // rows of kReduceBatch strides at a time, so that all their loads are
+        // in flight together (one memory latency per batch, not per row)
+        for (int b0 = threadIdx.x; b0 < nmax; b0 += kReduceBatch * kBlock) {
+            double v[kReduceBatch][12];
 #pragma unroll
-                for (int j = 0; j < 7; ++j) r[5 + j] += p3[7L * b + j];
+            for (int q = 0; q < kReduceBatch; ++q) {
+                const int b = b0 + q * kBlock;
+#pragma unroll
+                for (int j = 0; j < 12; ++j) v[q][j] = 0.0;
+                if (b < ws.blocks[0]) v[q][0] = p0[b];
+                if (b < ws.blocks[1]) { v[q][1] = p1[2L * b]; v[q][2] = p1[2L * b + 1]; }
+                if (b < ws.blocks[2]) { v[q][3] = p2[2L * b]; v[q][4] = p2[2L * b + 1]; }
+                if (b < n3)
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) v[q][5 + j] = p3[7L * b + j];
+            }
+#pragma unroll
+            for (int q = 0; q < kReduceBatch; ++q)
+#pragma unroll
+                for (int j = 0; j < 12; ++j) r[j] += v[q][j];
         }
     }
     blockSum<12>(r);   // ends with a barrier
