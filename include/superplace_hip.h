@@ -210,6 +210,36 @@ int sp_hip_gp_kernel_stats(const sp_hip_gp* gp, double* ms, int64_t* launches, d
                            double* flops);
 void sp_hip_gp_destroy(sp_hip_gp* gp);
 
+/* ---------------------------------------------------------------- BookShelf
+ * Replaces BookshelfReader (src/io/bookshelf_reader.h:20-41: read at :23,
+ * readPlacement at :26) and BookshelfWriter::writePl
+ * (src/io/bookshelf_writer.h:15).  Host code; no GPU is needed.  The reader
+ * fills the same database as the reference: node order [movable | fixed],
+ * centres, macros = movable nodes taller than 1.5 row heights, net->pin and
+ * node->pin CSR.  Errors (missing file, unknown node in a net, malformed
+ * number) return SP_HIP_EINVAL with the message. */
+typedef struct sp_bookshelf sp_bookshelf;
+
+int sp_bookshelf_read(const char* aux_path, sp_bookshelf** out);
+/* A view of the database for sp_hip_gp_create (num_fillers = 0); the
+ * pointers stay valid until sp_bookshelf_destroy. */
+int sp_bookshelf_design(const sp_bookshelf* b, sp_hip_design* view);
+/* Per node orientation code (N E S W FN FE FS FW = 0..7), per net weight,
+ * per row site step and site count, and the row height (most frequent). */
+int sp_bookshelf_extra(const sp_bookshelf* b, const uint8_t** orient,
+                       const float** net_weight, const float** row_step,
+                       const int32_t** row_sites, float* row_height);
+/* which = 0: node names, 1: net names, each followed by '\n', into buf of
+ * cap bytes; *need = bytes required (buf may be NULL to ask). */
+int sp_bookshelf_names(const sp_bookshelf* b, int which, char* buf, int64_t cap, int64_t* need);
+/* readPlacement: positions (and /FIXED flags) from another .pl, by name. */
+int sp_bookshelf_read_pl(sp_bookshelf* b, const char* pl_path);
+/* writePl: the num_nodes node centres x, y as lower-left corners, with the
+ * reference's exact format (bookshelf_writer.cpp:16-37). */
+int sp_bookshelf_write_pl(const sp_bookshelf* b, const char* path, const float* x,
+                          const float* y);
+void sp_bookshelf_destroy(sp_bookshelf* b);
+
 /* Batched C = A * B^T of row-major matrices (A m x k, B n x k, C m x n; dims
  * multiples of 64) on the matrix cores, the primitive of the Poisson
  * transforms.  Device pointers, float (or double with SP_HIP_FLAG_FP64);

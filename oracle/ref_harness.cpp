@@ -1,7 +1,7 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
-// src/db/place_db.cpp), compiled from where they lie by oracle/Makefile into
-// oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
+// src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp),
+// compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
 //
 // The PlaceDB is filled through its public builder API (addNode,
@@ -13,6 +13,8 @@
 
 #include "db/hpwl.h"
 #include "db/place_db.h"
+#include "io/bookshelf_reader.h"
+#include "io/bookshelf_writer.h"
 #include "gp/wa_wirelength.h"
 #include "numeric/optimizer.h"
 
@@ -99,5 +101,95 @@ double ref_nesterov_quadratic(int n, const float* coef, const float* target, flo
     if (evals) *evals = opt->evaluations();
     return opt->objective();
 }
+
+// BookshelfReader::read (bookshelf_reader.cpp:430-463) into a handle; the
+// arrays are then copied out by ref_bookshelf_get.
+void* ref_bookshelf_read(const char* aux) {
+    auto* db = new sp::PlaceDB();
+    try {
+        sp::BookshelfReader reader;
+        reader.read(aux, *db);
+    } catch (...) {
+        delete db;
+        return nullptr;
+    }
+    return db;
+}
+
+// counts: numMovable, numNodes, numPins, numNets, numRows; core: lx ly hx hy,
+// rowHeight
+void ref_bookshelf_counts(void* h, int32_t* counts, float* core) {
+    const auto* db = static_cast<sp::PlaceDB*>(h);
+    counts[0] = db->numMovable;
+    counts[1] = db->numNodes;
+    counts[2] = db->numPins;
+    counts[3] = db->numNets;
+    counts[4] = static_cast<int32_t>(db->rows.size());
+    core[0] = db->coreRegion.lx;
+    core[1] = db->coreRegion.ly;
+    core[2] = db->coreRegion.hx;
+    core[3] = db->coreRegion.hy;
+    core[4] = db->rowHeight;
+}
+
+// Node arrays (numNodes), pin arrays (numPins), CSR, rows (ly, height, lx,
+// hx, step, numSites as float) and net weights.
+void ref_bookshelf_get(void* h, float* x, float* y, float* w, float* hh, uint8_t* flags,
+                       uint8_t* orient, float* offx, float* offy, int32_t* pin2node,
+                       int32_t* pin2net, int32_t* net2pinStart, int32_t* flatNet2pin,
+                       int32_t* node2pinStart, int32_t* flatNode2pin, float* rows,
+                       float* netWeight) {
+    const auto* db = static_cast<sp::PlaceDB*>(h);
+    for (int i = 0; i < db->numNodes; ++i) {
+        x[i] = db->node_x[i];
+        y[i] = db->node_y[i];
+        w[i] = db->node_w[i];
+        hh[i] = db->node_h[i];
+        flags[i] = db->node_flags[i];
+        orient[i] = db->node_orient[i];
+    }
+    for (int p = 0; p < db->numPins; ++p) {
+        offx[p] = db->pin_offset_x[p];
+        offy[p] = db->pin_offset_y[p];
+        pin2node[p] = db->pin2node[p];
+        pin2net[p] = db->pin2net[p];
+        flatNet2pin[p] = db->flat_net2pin[p];
+        flatNode2pin[p] = db->flat_node2pin[p];
+    }
+    for (int k = 0; k <= db->numNets; ++k) net2pinStart[k] = db->net2pin_start[k];
+    for (int i = 0; i <= db->numNodes; ++i) node2pinStart[i] = db->node2pin_start[i];
+    for (size_t r = 0; r < db->rows.size(); ++r) {
+        const auto& row = db->rows[r];
+        rows[6 * r + 0] = row.ly;
+        rows[6 * r + 1] = row.height;
+        rows[6 * r + 2] = row.lx;
+        rows[6 * r + 3] = row.hx();
+        rows[6 * r + 4] = row.step;
+        rows[6 * r + 5] = static_cast<float>(row.numSites);
+    }
+    for (int k = 0; k < db->numNets; ++k) netWeight[k] = db->net_weight[k];
+}
+
+const char* ref_bookshelf_node_name(void* h, int i) {
+    return static_cast<sp::PlaceDB*>(h)->node_name[i].c_str();
+}
+
+// BookshelfWriter::writePl (bookshelf_writer.cpp:16-37) after setting the
+// node centres to x, y.
+int ref_bookshelf_write_pl(void* h, const char* path, const float* x, const float* y) {
+    auto* db = static_cast<sp::PlaceDB*>(h);
+    for (int i = 0; i < db->numNodes; ++i) {
+        db->node_x[i] = x[i];
+        db->node_y[i] = y[i];
+    }
+    try {
+        sp::BookshelfWriter::writePl(path, *db);
+    } catch (...) {
+        return 1;
+    }
+    return 0;
+}
+
+void ref_bookshelf_free(void* h) { delete static_cast<sp::PlaceDB*>(h); }
 
 }  // extern "C"

@@ -17,7 +17,7 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libsuperplace_hip.so"
 OBJ_DIR = HERE / "_build"
-SOURCES = ["poisson.hip", "gp_kernels.hip", "engine.hip"]
+SOURCES = ["poisson.hip", "gp_kernels.hip", "engine.hip", "bookshelf.cpp"]
 ARCH = os.environ.get("SP_HIP_ARCH", "gfx950")
 
 
@@ -42,7 +42,7 @@ def build(verbose: bool = False, jobs: int = 4) -> pathlib.Path:
     procs = []
     objs = []
     for s in srcs:
-        obj = OBJ_DIR / (s.replace(".hip", ".o"))
+        obj = OBJ_DIR / (s.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
         cmd = [cc, *flags(), "-c", str(CSRC / s), "-o", str(obj)]
         if verbose:

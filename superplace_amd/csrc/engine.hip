@@ -18,52 +18,12 @@
 #include <vector>
 
 #include "../../include/superplace_hip.h"
+#include "abi.h"
 #include "gp_kernels.h"
 #include "poisson.h"
 
 namespace sphip {
 namespace {
-
-thread_local std::string g_lastError;
-
-struct InvalidArg : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct NonFinite : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct NoDevice : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return SP_HIP_OK;
-    } catch (const InvalidArg& e) {
-        g_lastError = e.what();
-        return SP_HIP_EINVAL;
-    } catch (const std::invalid_argument& e) {
-        g_lastError = e.what();
-        return SP_HIP_EINVAL;
-    } catch (const NonFinite& e) {
-        g_lastError = e.what();
-        return SP_HIP_ENONFINITE;
-    } catch (const NoDevice& e) {
-        g_lastError = e.what();
-        return SP_HIP_ENODEV;
-    } catch (const HipError& e) {
-        g_lastError = e.what();
-        return std::strstr(e.what(), "out of memory") ? SP_HIP_EOOM : SP_HIP_EDEVICE;
-    } catch (const std::bad_alloc& e) {
-        g_lastError = "host allocation failed";
-        return SP_HIP_EOOM;
-    } catch (const std::exception& e) {
-        g_lastError = e.what();
-        return SP_HIP_EINTERNAL;
-    }
-}
 
 void requireDevice() {
     int n = 0;
@@ -768,9 +728,11 @@ void Engine<T>::build(const sp_hip_design& d) {
 
     // reduction workspace: 4 regions of maxBlocks x 8 doubles (one block
     // partial per region, see ReduceWs)
-    const long maxBlocks = std::max<long>({(static_cast<long>(nb_) + kBlock - 1) / kBlock,
-                                           (dd_.waWaveBase[kWaClasses + 1] + 3L) / 4,
-                                           (n_ + kBlock - 1L) / kBlock, 1L});
+    const long maxBlocks = std::max<long>(
+        {(static_cast<long>(nb_) + kBlock - 1) / kBlock, (dd_.waWaveBase[kWaClasses + 1] + 3L) / 4,
+         (n_ + kBlock - 1L) / kBlock,
+         static_cast<long>(dd_.tiles.tilesX) * dd_.tiles.tilesY + (dd_.nDens + kTileChunk - 1) / kTileChunk,
+         1L});
     ws_.maxBlocks = static_cast<int>(maxBlocks) + 1;
     ws_.partials = static_cast<double*>(own(dalloc<double>(4L * ws_.maxBlocks * 8)));
     ws_.scalars = static_cast<double*>(own(dalloc<double>(kNumScalars)));
@@ -828,7 +790,7 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
                                 T* dbgWl) {
     launchTileSort<T>(dd_, geom_, nullptr, s_);
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
-                     nullptr, s_);
+                     ws_, nullptr, s_);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
     poisson_->solve(false);
@@ -855,7 +817,7 @@ template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
     launchTileSort<T>(dd_, geom_, nullptr, s_);
     launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
-                     nullptr, s_);
+                     ws_, nullptr, s_);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
     launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
@@ -1131,16 +1093,17 @@ void Engine<T>::launchSlot(bool events, int slot) {
                                                    offsetof(Ctl, done));
     auto b = [&](int f) { if (events) beginK(f, slot); };
     auto e = [&](int f) { if (events) endK(f, slot); };
-    // the spatial order is rebuilt every kSortPeriod slots (slot 0 of every
-    // graph launch included); in between the scatter uses the last order
-    const bool sort = slot % kSortPeriod == 0;
-    if (sort) {
+    // every kSortPeriod-th slot holds the sort passes; they return at once
+    // unless the previous scatter found too many nodes outside their tile
+    // window (ctl->sortNext)
+    const bool sortSlot = slot % kSortPeriod == 0;
+    if (sortSlot) {
         b(SP_HIP_K_SORT);
         launchTileSort<T>(dd_, geom_, dCtl_, s_);
         e(SP_HIP_K_SORT);
     }
     b(SP_HIP_K_SCATTER);
-    launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, sort, dCtl_, s_);
+    launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, sortSlot, ws_, dCtl_, s_);
     e(SP_HIP_K_SCATTER);
     b(SP_HIP_K_FINALIZE);
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
@@ -1198,6 +1161,8 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     c.bestTau = std::numeric_limits<float>::infinity();
     c.refTau = std::numeric_limits<float>::infinity();
     c.effRef = effRef;
+    c.sortNext = 1;
+    c.spillLimit = kSpillFrac * dd_.nDens;
     if (maxIter <= 0) {   // no step at all: deliver the clamped start
         res->iterations = 0;
         scatterPos(u_);

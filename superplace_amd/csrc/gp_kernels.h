@@ -78,13 +78,15 @@ constexpr int kTileWin = kTileBins + 2 * kTileHalo;
 constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 1;
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
 constexpr int kTileChunk = 1024;     // nodes per scatter block (a tile's owner takes the first)
-constexpr int kSortPeriod = 4;       // graph slots per spatial re-sort
+constexpr double kSpillFrac = 0.05;  // re-sort once this fraction of nodes left their tile window
+constexpr int kSortPeriod = 4;       // ... checked at every fourth evaluation of the device loop
+constexpr int kSortBlocks = 512;     // grid of the sort passes (they stride over record chunks)
 
 // FP64 reductions: every reducing kernel writes one partial per block (plain
 // stores, no atomics, no fences); k_reduce sums each region in block order
 // after the node pass.  Regions and their values per block:
-//   0 finalize (tau numerator), 1 WA net pass (WL, HPWL), 2 unused,
-//   3 node pass (7 sums).
+//   0 finalize (tau numerator), 1 WA net pass (WL, HPWL), 2 scatter (nodes
+//   with charge outside their tile window), 3 node pass (7 sums).
 struct ReduceWs {
     double* partials = nullptr;   // 4 regions of maxBlocks * 8 doubles
     double* scalars = nullptr;    // EvalScalars on the device
@@ -132,6 +134,8 @@ struct Ctl {
     float lambda, gamma;      // schedule values used by the next evaluation
     float targetTau, stepX, minGain, bestTau, refTau;
     int sinceImprove, havePrev;
+    int sortNext;             // the next evaluation re-sorts the density records
+    double spillLimit;        // ... when more nodes than this left their tile window
     double prevHpwl, effRef;
 };
 
@@ -144,17 +148,21 @@ struct Ctl {
 // nodes in its window and stores the window to tiles.stage; extra blocks take
 // the further chunks of crowded tiles and add their windows to nodeQ / fillQ
 // with atomics, as do charges that fall outside a window (large nodes).
-// `sorted`: launchTileSort ran for this evaluation, so the scatter reads the
-// new order (B) and copies it to A; otherwise it reads A, the order of the
-// last sort with positions kept current by the update (charges of nodes that
-// left their tile's window since then go through the atomic grids).
+// `sorted` (with ctl: and ctl->sortNext): launchTileSort ran for this
+// evaluation, so the scatter reads the new order (B) and copies it to A;
+// otherwise it reads A, the order of the last sort with positions kept
+// current by the update (charges of nodes that left their tile's window
+// since then go through the atomic grids).  The count of such nodes per
+// block goes to region 2 of ws; k_reduce re-sorts once it passes
+// ctl->spillLimit.
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, bool sorted,
-                   const Ctl* ctl, hipStream_t s);
+                   ReduceWs& ws, const Ctl* ctl, hipStream_t s);
 
 // Re-sorts the density nodes by tile (counting sort; the order inside a tile
-// is arbitrary, which the integer density sums do not see).
+// is arbitrary, which the integer density sums do not see).  With ctl, the
+// passes return at once unless ctl->sortNext is set.
 template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s);
 

@@ -139,11 +139,11 @@ struct TileHist {
 
 template <typename T>
 __device__ inline unsigned blockTileHist(const DevDesign<T>& d, const GridGeom<T>& g, TileHist& h,
-                                         int* outRec, typename Vec2<T>::type* outXY) {
+                                         int chunk, int* outRec, typename Vec2<T>::type* outXY) {
     for (int k = threadIdx.x; k < d.tiles.nKeys; k += kBlock) h.count[k] = 0;
     if (threadIdx.x == 0) h.nKeys = 0;
     __syncthreads();
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int t = chunk * kBlock + threadIdx.x;
     unsigned key = 0xFFFFFFFFu;
     if (t < d.nDens) {
         const int rec = d.recIdA[t];
@@ -193,17 +193,23 @@ __device__ inline int blockExclusiveScan(const int* src, int n, int* dst, int* w
     return total;
 }
 
+// Both passes stride over chunks of kBlock records with a bounded grid, so
+// that a skipped sort (ctl->sortNext clear) costs a small launch.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<T> g,
                                                        const Ctl* ctl) {
-    if (ctl && ctl->done) return;
+    if (ctl && (ctl->done || !ctl->sortNext)) return;
     __shared__ TileHist h;
-    int rec = 0;
-    typename Vec2<T>::type p;
-    blockTileHist<T>(d, g, h, &rec, &p);
-    for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
-        const int key = h.keys[q];
-        atomicAdd(&d.tiles.count[key], h.count[key]);
+    const int chunks = (d.nDens + kBlock - 1) / kBlock;
+    for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        int rec = 0;
+        typename Vec2<T>::type p;
+        blockTileHist<T>(d, g, h, chunk, &rec, &p);
+        for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
+            const int key = h.keys[q];
+            atomicAdd(&d.tiles.count[key], h.count[key]);
+        }
+        __syncthreads();
     }
 }
 
@@ -213,28 +219,32 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<T> g,
                                                        const Ctl* ctl) {
-    if (ctl && ctl->done) return;
+    if (ctl && (ctl->done || !ctl->sortNext)) return;
     __shared__ TileHist h;
     __shared__ int prefix[kTileKeys];
     __shared__ int waveTot[kBlock / 64];
     const TileGrid& tg = d.tiles;
-    int rec = 0;
-    typename Vec2<T>::type p;
-    const unsigned key = blockTileHist<T>(d, g, h, &rec, &p);
     blockExclusiveScan(tg.count, tg.nKeys, prefix, waveTot);
-    for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
-        const int k = h.keys[q];
-        h.count[k] = prefix[k] + atomicAdd(&tg.fill[k], h.count[k]);
-    }
-    __syncthreads();
-    if (key != 0xFFFFFFFFu) {
-        const int t = blockIdx.x * kBlock + threadIdx.x;
-        const int pos = atomicAdd(&h.count[key], 1);
-        const auto wh = load2(d.recWHA, t);
-        d.recIdB[pos] = rec;
-        store2(d.recWHB, pos, wh.x, wh.y);
-        store2(d.recXYB, pos, p.x, p.y);
-        d.recPos[rec & kRecNode] = pos;
+    const int chunks = (d.nDens + kBlock - 1) / kBlock;
+    for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        int rec = 0;
+        typename Vec2<T>::type p;
+        const unsigned key = blockTileHist<T>(d, g, h, chunk, &rec, &p);
+        for (int q = threadIdx.x; q < h.nKeys; q += kBlock) {
+            const int k = h.keys[q];
+            h.count[k] = prefix[k] + atomicAdd(&tg.fill[k], h.count[k]);
+        }
+        __syncthreads();
+        if (key != 0xFFFFFFFFu) {
+            const int t = chunk * kBlock + threadIdx.x;
+            const int pos = atomicAdd(&h.count[key], 1);
+            const auto wh = load2(d.recWHA, t);
+            d.recIdB[pos] = rec;
+            store2(d.recWHB, pos, wh.x, wh.y);
+            store2(d.recXYB, pos, p.x, p.y);
+            d.recPos[rec & kRecNode] = pos;
+        }
+        __syncthreads();
     }
     if (blockIdx.x == 0) {
         for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
@@ -266,8 +276,11 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
                                                     unsigned long long* fillQ, int sorted,
-                                                    const Ctl* ctl) {
-    if (ctl && ctl->done) return;
+                                                    double* partials, const Ctl* ctl) {
+    if (ctl) {
+        if (ctl->done) return;
+        sorted = sorted && ctl->sortNext;
+    }
     constexpr int PER = kTileChunk / kBlock;
     constexpr int WIN2 = kTileWin * kTileWin;
     __shared__ unsigned long long win[2 * WIN2];   // node window, then filler window
@@ -281,7 +294,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
         chunk = 0;
     } else {
         const int e = blockIdx.x - nTiles;
-        if (e >= tg.extraStart[tg.nKeys]) return;   // no chunk for this block
+        if (e >= tg.extraStart[tg.nKeys]) {   // no chunk for this block
+            if (threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+            return;
+        }
         int lo = 0, hi = tg.nKeys - 1;   // last key with extraStart <= e
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -324,25 +340,30 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
                                      ((rec & kRecMacro) ? F_MACRO : 0));
         fp[j] = footprintOf<T>(p.x, p.y, wh.x, wh.y, fl[j], g);
     }
+    double spill[1] = {0.0};   // nodes with charge outside the window
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         if (!valid[j]) continue;
         const bool filler = fl[j] & F_FILLER;
         unsigned long long* grid = filler ? fillQ : nodeQ;
         unsigned long long* w = win + (filler ? WIN2 : 0);
+        bool out = false;
         for (int bx = fp[j].bx0; bx <= fp[j].bx1; ++bx)
             for (int by = fp[j].by0; by <= fp[j].by1; ++by) {
                 const T ov = binCharge<T>(bx, by, fp[j], g);
                 if (ov <= T(0)) continue;
                 const unsigned long long q = static_cast<unsigned long long>(toFixed(ov, qscale));
                 const int lx = bx - wx0, ly = by - wy0;
-                if (lx >= 0 && lx < kTileWin && ly >= 0 && ly < kTileWin)
+                if (lx >= 0 && lx < kTileWin && ly >= 0 && ly < kTileWin) {
                     atomicAdd(&w[lx * kTileWin + ly], q);
-                else
+                } else {
                     atomicAdd(&grid[static_cast<long>(bx) * g.dim + by], q);
+                    out = true;
+                }
             }
+        if (out) spill[0] += 1.0;
     }
-    __syncthreads();
+    blockPartial<1>(spill, partials);   // ends with a barrier
     if (chunk == 0) {
         unsigned long long* out = tg.stage + static_cast<long>(blockIdx.x) * 2 * WIN2;
         for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) out[e] = win[e];
@@ -942,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
                 for (int j = 0; j < 12; ++j) v[q][j] = 0.0;
                 if (b < ws.blocks[0]) v[q][0] = p0[b];
                 if (b < ws.blocks[1]) { v[q][1] = p1[2L * b]; v[q][2] = p1[2L * b + 1]; }
-                if (b < ws.blocks[2]) { v[q][3] = p2[2L * b]; v[q][4] = p2[2L * b + 1]; }
+                if (b < ws.blocks[2]) v[q][3] = p2[b];
                 if (b < n3)
 #pragma unroll
                     for (int j = 0; j < 7; ++j) v[q][5 + j] = p3[7L * b + j];
@@ -961,7 +982,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
         sc[2] = r[2];
         for (int j = 0; j < 7; ++j) sc[3 + j] = withNode ? r[5 + j] : 0.0;
         for (int j = 0; j < kNumScalars; ++j) ws.scalars[j] = sc[j];
-        if (ctl && withNode) decideStep<T>(*reinterpret_cast<Ctl*>(ctlWords), sc, rows);
+        if (ctl && withNode) {
+            Ctl& c = *reinterpret_cast<Ctl*>(ctlWords);
+            c.sortNext = r[3] > c.spillLimit ? 1 : 0;   // re-sort before the next scatter
+            decideStep<T>(c, sc, rows);
+        }
     }
     __syncthreads();
     if (ctl && withNode)
@@ -1041,10 +1066,11 @@ long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, bool sorted,
-                   const Ctl* ctl, hipStream_t s) {
+                   ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
     const int blocks = d.tiles.tilesX * d.tiles.tilesY + (d.nDens + kTileChunk - 1) / kTileChunk;
+    ws.blocks[2] = blocks;
     launchKernel(k_scatter<T>, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ,
-                 sorted ? 1 : 0, ctl);
+                 sorted ? 1 : 0, ws.region(2), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -1059,9 +1085,10 @@ template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s) {
     if (d.nDens == 0) return;
     if (g.dim > kTileBins * 64) throw std::runtime_error("tile sort: grid wider than 1024 bins");
-    launchKernel(k_tile_count<T>, gridFor(d.nDens), kBlock, 0, s, d, g, ctl);
+    const int blocks = static_cast<int>(std::min<long>(gridFor(d.nDens), kSortBlocks));
+    launchKernel(k_tile_count<T>, blocks, kBlock, 0, s, d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
-    launchKernel(k_tile_place<T>, gridFor(d.nDens), kBlock, 0, s, d, g, ctl);
+    launchKernel(k_tile_place<T>, blocks, kBlock, 0, s, d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -1084,7 +1111,6 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
     const T invG = invGammaOf<T>(gamma);
     const int waves = d.waWaveBase[kWaClasses + 1];
     ws.blocks[1] = std::max(1, std::min(kWaMaxBlocks, (waves + kBlock / 64 - 1) / (kBlock / 64)));
-    ws.blocks[2] = 0;
     launchKernel(k_wa_pins<T>, ws.blocks[1], kBlock, 0, s, d, invG, ignoreDeg, ws.region(1), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
@@ -1118,8 +1144,8 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
 
 #define SPHIP_INST(T)                                                                            \
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
-                                   unsigned long long*, unsigned long long*, bool, const Ctl*,   \
-                                   hipStream_t);                                                 \
+                                   unsigned long long*, unsigned long long*, bool, ReduceWs&,    \
+                                   const Ctl*, hipStream_t);                                                 \
     template void launchRecRefresh<T>(const DevDesign<T>&, hipStream_t);                        \
     template void launchTileSort<T>(const DevDesign<T>&, const GridGeom<T>&, const Ctl*,         \
                                     hipStream_t);                                                \

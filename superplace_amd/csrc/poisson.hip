@@ -142,7 +142,6 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
                 av[v] = *reinterpret_cast<const V*>(arow + k0 + v * VE);
                 bv[v] = *reinterpret_cast<const V*>(brow + k0 + v * VE);
             }
-#pragma unroll 4
             for (int kb = k0; kb < k1; kb += 16) {
                 const int kn = kb + 16 < k1 ? kb + 16 : kb;
                 V an[NV], bn[NV];

@@ -43,6 +43,10 @@ class NoDevice(HipError):
     pass
 
 
+class InvalidArgument(HipError):
+    """An argument or input file the library rejected (SP_HIP_EINVAL)."""
+
+
 class NonFinite(HipError):
     """InvariantError("non_finite", ...) of the reference (src/util/invariant.h:10-33)."""
 
@@ -110,7 +114,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_hip_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -141,6 +145,13 @@ def lib():
     L.sp_hip_gp_destroy.argtypes = [_P]
     L.sp_hip_gemm.argtypes = [C.c_int, C.c_int, _P, _P, _P, C.c_int, C.c_int, C.c_int]
     L.sp_hip_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.sp_bookshelf_read.argtypes = [C.c_char_p, C.POINTER(_P)]
+    L.sp_bookshelf_design.argtypes = [_P, _P]
+    L.sp_bookshelf_extra.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.sp_bookshelf_names.argtypes = [_P, C.c_int, _P, C.c_int64, C.POINTER(C.c_int64)]
+    L.sp_bookshelf_read_pl.argtypes = [_P, C.c_char_p]
+    L.sp_bookshelf_write_pl.argtypes = [_P, C.c_char_p, _P, _P]
+    L.sp_bookshelf_destroy.argtypes = [_P]
     _lib = L
     return L
 
@@ -153,6 +164,8 @@ def check(code: int) -> None:
         raise NoDevice(code, msg)
     if code == SP_HIP_ENONFINITE:
         raise NonFinite(code, msg)
+    if code == SP_HIP_EINVAL:
+        raise InvalidArgument(code, msg)
     raise HipError(code, msg)
 
 
