@@ -240,6 +240,29 @@ int sp_bookshelf_write_pl(const sp_bookshelf* b, const char* path, const float* 
                           const float* y);
 void sp_bookshelf_destroy(sp_bookshelf* b);
 
+/* ------------------------------------------------------------------ Fillers
+ * Replaces initializeFillers (src/gp/filler.h:29, filler.cpp:47-139) and
+ * scatterFillers (filler.h:32, filler.cpp:33-45) for a caller that holds the
+ * design as arrays: the library cannot grow the caller's arrays, so the
+ * plan (FillerInfo, filler.h:18-25) and the positions are two calls, and
+ * the caller appends `count` nodes of width x height with flag 4 (filler)
+ * after its fixed nodes.  Host code; no GPU is needed.  Results are
+ * bit-identical to the reference's on the same design and seed. */
+typedef struct sp_filler_info {
+    int32_t count;
+    float width, height;
+    double whitespace_area, node_area_scaled, total_filler_area;
+} sp_filler_info;
+/* The plan for a design without fillers (d->num_fillers is ignored; only
+ * nodes below num_nodes count) at target density (> 0). */
+int sp_filler_plan(const sp_hip_design* d, float target_density, sp_filler_info* info);
+/* Centres of `count` fillers of width x height drawn uniformly inside the
+ * design's core with the reference's generator (std::mt19937 seeded with
+ * `seed`, i.e. Config::random_seed + the caller's offset; a float uniform
+ * draw for x, then for y, per filler). */
+int sp_filler_scatter(const sp_hip_design* d, int count, float width, float height,
+                      uint64_t seed, float* x, float* y);
+
 /* Batched C = A * B^T of row-major matrices (A m x k, B n x k, C m x n; dims
  * multiples of 64) on the matrix cores, the primitive of the Poisson
  * transforms.  Device pointers, float (or double with SP_HIP_FLAG_FP64);

@@ -1,6 +1,7 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
-// src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp),
+// src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp,
+// src/gp/filler.cpp),
 // compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
 //
@@ -13,10 +14,12 @@
 
 #include "db/hpwl.h"
 #include "db/place_db.h"
+#include "gp/filler.h"
 #include "io/bookshelf_reader.h"
 #include "io/bookshelf_writer.h"
 #include "gp/wa_wirelength.h"
 #include "numeric/optimizer.h"
+#include "util/config.h"
 
 namespace {
 
@@ -191,5 +194,41 @@ int ref_bookshelf_write_pl(void* h, const char* path, const float* x, const floa
 }
 
 void ref_bookshelf_free(void* h) { delete static_cast<sp::PlaceDB*>(h); }
+
+// initializeFillers (filler.cpp:47-139) on a database read by
+// ref_bookshelf_read, with Config{target_density, random_seed}.  info:
+// count, width, height, whitespace, nodeAreaScaled, totalFillerArea; the
+// filler centres go to x, y (at most cap).  Returns the count.
+int ref_fillers(void* h, float target, uint64_t seed, double* info, float* x, float* y, int cap) {
+    auto* db = static_cast<sp::PlaceDB*>(h);
+    sp::Config cfg;
+    cfg.target_density = target;
+    cfg.random_seed = seed;
+    const sp::FillerInfo fi = sp::initializeFillers(*db, cfg);
+    info[0] = fi.count;
+    info[1] = fi.width;
+    info[2] = fi.height;
+    info[3] = fi.whitespaceArea;
+    info[4] = fi.nodeAreaScaled;
+    info[5] = fi.totalFillerArea;
+    for (int k = 0; k < db->numFillers && k < cap; ++k) {
+        x[k] = db->node_x[db->numNodes + k];
+        y[k] = db->node_y[db->numNodes + k];
+    }
+    return db->numFillers;
+}
+
+// scatterFillers (filler.cpp:33-45) again with a seed offset, on the fillers
+// the last ref_fillers inserted.
+void ref_fillers_rescatter(void* h, uint64_t seed, uint64_t offset, float* x, float* y) {
+    auto* db = static_cast<sp::PlaceDB*>(h);
+    sp::Config cfg;
+    cfg.random_seed = seed;
+    sp::scatterFillers(*db, cfg, offset);
+    for (int k = 0; k < db->numFillers; ++k) {
+        x[k] = db->node_x[db->numNodes + k];
+        y[k] = db->node_y[db->numNodes + k];
+    }
+}
 
 }  // extern "C"

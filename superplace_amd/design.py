@@ -5,10 +5,11 @@ centre coordinates, node order [movable | fixed | filler], pins as node centre
 plus offset, and two CSRs (net -> pins, node -> pins) built by the two-pass
 rule of PlaceDB::finalizeCSR (src/db/place_db.cpp:67-95).
 
-Synthetic designs use a portable counter-based RNG (splitmix64) so that the
-same seed gives the same design on every machine; the reference's Rng
-(src/util/rng.h:15-18) relies on libstdc++'s uniform_real_distribution, whose
-output is implementation defined.
+Synthetic netlists use a portable counter-based RNG (splitmix64) so that the
+same seed gives the same design on every machine.  Fillers are planned and
+scattered by the library exactly as the reference does (its Rng,
+src/util/rng.h:15-18: std::mt19937 with libstdc++'s
+uniform_real_distribution<float>).
 """
 from __future__ import annotations
 
@@ -149,63 +150,35 @@ def make_design(node_x, node_y, node_w, node_h, node_flags, num_movable, pin2nod
 
 # ----------------------------------------------------------------------------- fillers
 def filler_plan(d: Design, target_density: float):
-    """Filler count and size, src/gp/filler.cpp:47-121.
+    """Filler count and size (initializeFillers, src/gp/filler.cpp:47-121),
+    computed by the library (sp_filler_plan) on the nodes below num_nodes.
 
     Returns (count, width, height, whitespace, node_area_scaled, filler_area)."""
-    f32 = np.float32
-    overlap = 0.0
-    fixed = [i for i in range(d.num_movable, d.num_nodes) if not (d.node_flags[i] & F_NI)]
-    if fixed:
-        order = np.argsort(d.row_ly, kind="stable")
-        rly = d.row_ly[order]
-        rhy = (d.row_ly + d.row_height)[order]
-        rlx = d.row_lx[order]
-        rhx = d.row_hx[order]
-        for i in fixed:
-            lx = f32(d.node_x[i] - f32(0.5) * d.node_w[i])
-            hx = f32(d.node_x[i] + f32(0.5) * d.node_w[i])
-            ly = f32(d.node_y[i] - f32(0.5) * d.node_h[i])
-            hy = f32(d.node_y[i] + f32(0.5) * d.node_h[i])
-            lo = np.searchsorted(rhy, ly, side="right")
-            hi = np.searchsorted(rly, hy, side="left")
-            if hi <= lo:
-                continue
-            dx = np.minimum(hx, rhx[lo:hi]) - np.maximum(lx, rlx[lo:hi])
-            dy = np.minimum(hy, rhy[lo:hi]) - np.maximum(ly, rly[lo:hi])
-            ov = np.where((dx > 0) & (dy > 0), dx * dy, f32(0)).astype(np.float32)
-            overlap += float(np.sum(ov.astype(np.float64)))
-    white = max(0.0, d.total_row_area - overlap)
-    area = d.node_w[:d.num_movable].astype(np.float64) * d.node_h[:d.num_movable]
-    macro = (d.node_flags[:d.num_movable] & F_MACRO) != 0
-    node_scaled = float(area[~macro].sum() + area[macro].sum() * float(target_density))
-    filler_area = white * float(target_density) - node_scaled
-    if d.num_movable == 0:
-        return 0, 0.0, 0.0, white, node_scaled, filler_area
-    s = np.sort(area)
-    lo, hi = int(0.10 * d.num_movable), int(0.90 * d.num_movable)
-    avg = float(s[lo:hi].mean()) if hi > lo else float(s[d.num_movable // 2])
-    h = d.row_height_mode or 1.0
-    w = float(np.float32(avg) / np.float32(h))
-    if not w > 0:
-        w = 1.0
-    single = float(np.float32(w)) * h
-    count = max(0, int(filler_area / max(single, 1e-9) + 0.5))
-    return count, float(np.float32(w)), float(np.float32(h)), white, node_scaled, filler_area
+    from . import hip
+    import ctypes as C
+    info = hip.SpFillerInfo()
+    da = hip._DesignArrays(dataclasses.replace(d, num_fillers=0))
+    hip.check(hip.lib().sp_filler_plan(C.byref(da.s), float(target_density), C.byref(info)))
+    return (int(info.count), float(info.width), float(info.height), float(info.whitespace_area),
+            float(info.node_area_scaled), float(info.total_filler_area))
 
 
 def add_fillers(d: Design, target_density: float, seed: int) -> Design:
-    """Append fillers (filler.cpp:122-132) placed uniformly in the core
-    (scatterFillers, filler.cpp:33-45) with the splitmix64 stream `seed`."""
+    """Append fillers (filler.cpp:122-132) with centres drawn in the core by
+    the reference's generator (scatterFillers, filler.cpp:33-45: std::mt19937
+    seeded with `seed`), through the library (sp_filler_scatter)."""
+    from . import hip
+    import ctypes as C
     count, w, h, *_ = filler_plan(d, target_density)
+    fx = np.zeros(max(count, 1), np.float32)
+    fy = np.zeros(max(count, 1), np.float32)
+    da = hip._DesignArrays(dataclasses.replace(d, num_fillers=0))
+    hip.check(hip.lib().sp_filler_scatter(C.byref(da.s), count, w, h, int(seed) & (2**64 - 1),
+                                          hip._ptr(fx), hip._ptr(fy)))
+    fx, fy = fx[:count], fy[:count]
     out = d.copy()
     if count == 0:
         return out
-    rng = Rng(seed)
-    lx, ly, hx, hy = d.core
-    lo_x, hi_x = lx + 0.5 * w, hx - 0.5 * w
-    lo_y, hi_y = ly + 0.5 * h, hy - 0.5 * h
-    fx = rng.uniform(count, lo_x, hi_x) if hi_x > lo_x else np.full(count, 0.5 * (lx + hx))
-    fy = rng.uniform(count, lo_y, hi_y) if hi_y > lo_y else np.full(count, 0.5 * (ly + hy))
     out.node_x = np.concatenate([d.node_x[:d.num_nodes], fx.astype(np.float32)])
     out.node_y = np.concatenate([d.node_y[:d.num_nodes], fy.astype(np.float32)])
     out.node_w = np.concatenate([d.node_w[:d.num_nodes], np.full(count, w, np.float32)])

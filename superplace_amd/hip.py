@@ -51,6 +51,12 @@ class NonFinite(HipError):
     """InvariantError("non_finite", ...) of the reference (src/util/invariant.h:10-33)."""
 
 
+class SpFillerInfo(C.Structure):
+    _fields_ = [("count", C.c_int32), ("width", C.c_float), ("height", C.c_float),
+                ("whitespace_area", C.c_double), ("node_area_scaled", C.c_double),
+                ("total_filler_area", C.c_double)]
+
+
 class SpHipDesign(C.Structure):
     _fields_ = [("num_movable", C.c_int32), ("num_nodes", C.c_int32),
                 ("num_fillers", C.c_int32), ("num_pins", C.c_int32),
@@ -114,7 +120,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -152,6 +158,8 @@ def lib():
     L.sp_bookshelf_read_pl.argtypes = [_P, C.c_char_p]
     L.sp_bookshelf_write_pl.argtypes = [_P, C.c_char_p, _P, _P]
     L.sp_bookshelf_destroy.argtypes = [_P]
+    L.sp_filler_plan.argtypes = [_P, C.c_float, _P]
+    L.sp_filler_scatter.argtypes = [_P, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, _P]
     _lib = L
     return L
 
