@@ -66,7 +66,9 @@ int sp_hip_poisson_solve(sp_hip_poisson* p, const float* density, float* phi, fl
  * No counterpart in the reference (its CUDA backend always crosses to the
  * host, cuda_poisson.cu:194-210); it is the data path of the engine below. */
 int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* out3_dev);
-/* PoissonBackend::name(): "hip-mfma" or "hip-mfma-fp64". */
+/* PoissonBackend::name(): "hip-fft" (power-of-two grids: LDS FFT passes) or
+ * "hip-mfma" (other sizes: dense transforms on the matrix cores), with
+ * "-fp64" appended on the FP64 path. */
 const char* sp_hip_poisson_name(const sp_hip_poisson* p);
 void sp_hip_poisson_destroy(sp_hip_poisson* p);
 

@@ -765,7 +765,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts) + stageBytes;
     // finalize: atomic grids read and cleared, windows read, static terrain, rho
     bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + 8 * 2 + ts) + stageBytes;
-    bytesPer_[SP_HIP_K_POISSON] = 0;
+    bytesPer_[SP_HIP_K_POISSON] = poisson_->bytes(false);
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
     // WA net pass: per slot node id, offsets, the node's x/y, entry id and
     // the gradient write; netStart per net
@@ -1329,7 +1329,10 @@ public:
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
 
-    const char* name() const override { return sizeof(T) == 4 ? "hip-mfma" : "hip-mfma-fp64"; }
+    const char* name() const override {
+        if (plan_->usesFft()) return sizeof(T) == 4 ? "hip-fft" : "hip-fft-fp64";
+        return sizeof(T) == 4 ? "hip-mfma" : "hip-mfma-fp64";
+    }
 
 private:
     void run(const T* rho, T* out3) {

@@ -1,4 +1,5 @@
-// DCT Poisson solve as batched dense transforms on MFMA (host-side plan).
+// DCT Poisson solve (host-side plan): LDS-resident FFTs for power-of-two
+// grids, batched dense transforms on MFMA otherwise.
 //
 // Solves lap(phi) = -rho with Neumann boundaries and returns the field
 // xi = -grad(phi) with the conventions of the reference's CPU backend
@@ -7,10 +8,11 @@
 //   phi  = REDFT01_x REDFT01_y P
 //   xi_x = RODFT01_x REDFT01_y (P shifted down one row, rows scaled by u)
 //   xi_y = REDFT01_x RODFT01_y (P shifted left one column, columns scaled by v)
-// Each 1-D transform along an axis is a product with a dense nx x nx (or
-// ny x ny) matrix; the index shift and the u/v scaling are folded into the
-// sine matrices, and the spectral divide is the epilogue of the second
-// forward product.  Layout: row-major, x slow, y fast (data[x * ld + y]),
+// Power-of-two grids run three FFT passes (rows, columns, rows).  Other
+// sizes run each 1-D transform along an axis as a product with a dense
+// nx x nx (or ny x ny) matrix; the index shift and the u/v scaling are folded
+// into the sine matrices, and the spectral divide is the epilogue of the
+// second forward product.  Layout: row-major, x slow, y fast (data[x * ld + y]),
 // padded to a multiple of 64 on both axes with zero rows/columns.
 #pragma once
 
@@ -44,8 +46,15 @@ public:
     T* fieldY() { return ey_; }
     T* fieldXY() { return exy_; }
     hipStream_t stream() const { return stream_; }
-    // Floating-point operations of one solve (dense products actually executed).
+    // Floating-point operations of one solve (dense products actually
+    // executed; 0 on the FFT path).
     double flops(bool withPhi) const;
+    // Grid bytes one solve moves through memory on the FFT path (0 on the
+    // dense path).
+    double bytes(bool withPhi) const;
+    // Power-of-two grids (4 .. 1024 per side) take the FFT path.
+    bool usesFft() const { return fft_; }
+    static bool fftShape(int n);
 
 private:
     int nx_, ny_, mx_, my_;
@@ -57,6 +66,14 @@ private:
       *ax_ = nullptr;
     T *u_ = nullptr, *v_ = nullptr;
     T invNorm_;
+    // FFT path: per axis a table of L twiddles then L half-angle factors
+    // (complex, interleaved); LDS bytes of the three passes
+    bool fft_ = false;
+    T *tabX_ = nullptr, *tabY_ = nullptr;
+    size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0;
+    int colPairs_ = 1;   // column pairs per block of the column pass
+    void setFftLds();
+    void solveFft(bool withPhi, const int* skip);
 };
 
 // Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on

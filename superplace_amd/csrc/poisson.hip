@@ -192,6 +192,476 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
     }
 }
 
+// ------------------------------------------------------------ FFT path
+// For power-of-two grids the three axis passes run as complex FFTs in LDS
+// (Makhoul's mapping of the cosine/sine transforms onto a length-L complex
+// FFT; J. Makhoul, "A fast cosine transform in one and two dimensions",
+// IEEE TASSP 28(1), 1980), so a solve moves each grid a few times through
+// HBM instead of running dense N^3 products:
+//   pass A (rows, along y):    F1 = REDFT10_y(rho)
+//   pass B (columns, along x): P  = spectral(REDFT10_x(F1)),
+//                              Hc = REDFT01_x(P), Hs = RODFT01_x(u P shifted)
+//   pass C (rows, along y):    xi_x = REDFT01_y(Hs), xi_y = RODFT01_y(v Hc shifted),
+//                              phi = REDFT01_y(Hc)
+// Two real lines share one complex transform (one as the real part, one as
+// the imaginary part).  For a real line x of length L:
+//   REDFT10:  z[n] = x[2n], z[L-1-n] = x[2n+1]; Z = FFT(z);
+//             X[k] = 2 Re(exp(-i pi k / 2L) Z[k])
+//   REDFT01 (y[n] = a[0] + 2 sum_k a[k] cos(pi k (n+1/2)/L)):
+//             Z[k] = (a[k] - i a[L-k]) exp(i pi k / 2L) (a[L] = 0) is
+//             Hermitian, so z = IFFT(Z) is real; y[2m] = z[m], y[2m+1] = z[L-1-m]
+//   the shifted sine transform y[n] = sum_{k>=1} 2 b[k] sin(pi k (n+1/2)/L)
+//             equals (-1)^n REDFT01(a)[n] with a[m] = b[L-m], a[0] = 0.
+// One team of threads owns the transforms of its two lines (one row in the
+// last pass);
+// the FFT is a Stockham autosort (radix 8 or 4, a final radix 2 where
+// needed) between two LDS buffers, with twiddles exp(-2 pi i t / L) from an
+// LDS table.
+
+template <typename T>
+using cplx = typename Vec2<T>::type;
+
+template <typename T>
+__device__ inline cplx<T> cmk(T re, T im) {
+    cplx<T> c;
+    c.x = re;
+    c.y = im;
+    return c;
+}
+template <typename T>
+__device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return cmk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return cmk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return cmk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ inline cplx<T> cconj(cplx<T> a) { return cmk<T>(a.x, -a.y); }
+
+// Position of element i of a line in the REDFT10 reordering (and of output
+// element n of REDFT01 in z).
+__device__ inline int dctPos(int i, int L) { return (i & 1) ? L - 1 - (i >> 1) : (i >> 1); }
+
+// Radix-4 DFT in registers: forward uses exp(-2 pi i/4) = -i, inverse +i.
+template <typename T, bool INV>
+__device__ inline void dft4(cplx<T>& p0, cplx<T>& p1, cplx<T>& p2, cplx<T>& p3) {
+    const cplx<T> t0 = cadd<T>(p0, p2), t1 = csub<T>(p0, p2), t2 = cadd<T>(p1, p3);
+    const cplx<T> d = csub<T>(p1, p3);
+    const cplx<T> t3 = INV ? cmk<T>(-d.y, d.x) : cmk<T>(d.y, -d.x);
+    p0 = cadd<T>(t0, t2);
+    p1 = cadd<T>(t1, t3);
+    p2 = csub<T>(t0, t2);
+    p3 = csub<T>(t1, t3);
+}
+
+// Radix-8 DFT in registers: a radix-2 step between v[r] and v[r+4] (the
+// differences turned by exp(-+2 pi i r/8)), then two radix-4 DFTs giving the
+// even and the odd outputs.
+template <typename T, bool INV>
+__device__ inline void dft8(cplx<T>* v) {
+    constexpr T h = T(0.70710678118654752440);
+    cplx<T> u0 = cadd<T>(v[0], v[4]), u1 = cadd<T>(v[1], v[5]), u2 = cadd<T>(v[2], v[6]),
+            u3 = cadd<T>(v[3], v[7]);
+    cplx<T> d0 = csub<T>(v[0], v[4]), d1 = csub<T>(v[1], v[5]), d2 = csub<T>(v[2], v[6]),
+            d3 = csub<T>(v[3], v[7]);
+    // forward: d1 (1 - i)/sqrt2, d2 (-i), d3 (-1 - i)/sqrt2; inverse: conjugates
+    if (INV) {
+        d1 = cmk<T>(h * (d1.x - d1.y), h * (d1.x + d1.y));
+        d2 = cmk<T>(-d2.y, d2.x);
+        d3 = cmk<T>(-h * (d3.x + d3.y), h * (d3.x - d3.y));
+    } else {
+        d1 = cmk<T>(h * (d1.x + d1.y), h * (d1.y - d1.x));
+        d2 = cmk<T>(d2.y, -d2.x);
+        d3 = cmk<T>(h * (d3.y - d3.x), -h * (d3.x + d3.y));
+    }
+    dft4<T, INV>(u0, u1, u2, u3);
+    dft4<T, INV>(d0, d1, d2, d3);
+    v[0] = u0; v[1] = d0; v[2] = u1; v[3] = d1;
+    v[4] = u2; v[5] = d2; v[6] = u3; v[7] = d3;
+}
+
+// One Stockham stage of radix R for K transforms at once, butterflies spread
+// over the block's threads: butterfly j reads a[j + r L/R], turns input r by
+// tw[r (j mod ns) L/(ns R)] and writes b[(j - j mod ns) R + j mod ns + r ns].
+template <typename T, bool INV, int R, int K>
+__device__ inline void fftStage(cplx<T>* const* a, cplx<T>* const* b, const cplx<T>* tw, int L,
+                                int ns, int tid, int nt) {
+    const int nb = L / R, st = L / (ns * R);
+    for (int j = tid; j < nb; j += nt) {
+        const int jm = j & (ns - 1);
+        cplx<T> v[K][R];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k][r] = a[k][j + r * nb];
+        if (jm) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cplx<T> w = tw[r * jm * st];
+                if (INV) w = cconj<T>(w);
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[k][r] = cmul<T>(v[k][r], w);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if constexpr (R == 8) {
+                dft8<T, INV>(v[k]);
+            } else if constexpr (R == 4) {
+                dft4<T, INV>(v[k][0], v[k][1], v[k][2], v[k][3]);
+            } else {
+                const cplx<T> t = v[k][0];
+                v[k][0] = cadd<T>(t, v[k][1]);
+                v[k][1] = csub<T>(t, v[k][1]);
+            }
+        }
+        const int d = (j - jm) * R + jm;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int r = 0; r < R; ++r) b[k][d + r * ns] = v[k][r];
+    }
+}
+
+// Stockham FFT of length L (power of two, >= 2) by a team of nt threads
+// (tid = the thread's index in its team), K transforms of the same length at
+// once: a[k] is transform k's input in natural order, b[k] its scratch; on
+// return a[k] points at its result.  Each stage takes the largest radix
+// (8, 4) that divides what is left and still gives every thread a
+// butterfly, else the largest that divides (a 512-point transform on 128
+// threads: radix 4, 4, 4, 4, 2).  INV: the unnormalised inverse
+// (exp(+2 pi i nk/L)).  Stages end in block barriers, so every team of the
+// block runs the same L.
+template <typename T, bool INV, int K>
+__device__ void blockFft(cplx<T>** a, cplx<T>** b, const cplx<T>* tw, int L, int tid, int nt) {
+    for (int ns = 1; ns < L;) {
+        const int q = L / ns;
+        int R;
+        if ((q & 7) == 0 && L / 8 >= nt) R = 8;
+        else if ((q & 3) == 0 && L / 4 >= nt) R = 4;
+        else R = (q & 7) == 0 ? 8 : (q & 3) == 0 ? 4 : 2;
+        if (R == 8) fftStage<T, INV, 8, K>(a, b, tw, L, ns, tid, nt);
+        else if (R == 4) fftStage<T, INV, 4, K>(a, b, tw, L, ns, tid, nt);
+        else fftStage<T, INV, 2, K>(a, b, tw, L, ns, tid, nt);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            cplx<T>* t = a[k];
+            a[k] = b[k];
+            b[k] = t;
+        }
+        ns *= R;
+    }
+}
+
+// LDS layout of the FFT passes: the twiddle table (L entries
+// exp(-2 pi i t/L)), the half-angle table (L entries exp(-i pi k/2L)), the
+// pass's spectral scale (u or v, L reals), then the waves' buffers.
+template <typename T>
+struct FftLds {
+    cplx<T>* tw;
+    cplx<T>* hf;
+    T* sc;
+    cplx<T>* waves;
+};
+template <typename T>
+__device__ inline FftLds<T> fftLds(unsigned char* smem, int L) {
+    FftLds<T> f;
+    f.tw = reinterpret_cast<cplx<T>*>(smem);
+    f.hf = f.tw + L;
+    f.sc = reinterpret_cast<T*>(f.hf + L);
+    f.waves = f.hf + L + L / 2;
+    return f;
+}
+// Complex entries of LDS ahead of the waves' buffers.
+constexpr int kFftLdsHead(int L) { return 2 * L + L / 2; }
+
+// Copies the tables (and the scale array, when given) into LDS; the caller
+// synchronises before use.
+template <typename T>
+__device__ inline void loadTables(const FftLds<T>& f, const cplx<T>* gtw, const cplx<T>* ghf,
+                                  const T* gsc, int L) {
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+        f.tw[t] = gtw[t];
+        f.hf[t] = ghf[t];
+        if (gsc) f.sc[t] = gsc[t];
+    }
+}
+
+// REDFT10 outputs for the pair (k, L-k) from the transform Z of two packed
+// lines: X_a, X_b at k and (when 0 < k < L/2) at L-k.
+template <typename T>
+__device__ inline void dct2Pair(const cplx<T>* Z, const cplx<T>* hf, int k, int L, T* xa, T* xb,
+                                T* xaM, T* xbM) {
+    const cplx<T> zk = Z[k], zm = cconj<T>(Z[(L - k) & (L - 1)]);
+    const cplx<T> va = cmk<T>(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y + zm.y));
+    const cplx<T> dv = csub<T>(zk, zm);
+    const cplx<T> vb = cmk<T>(T(0.5) * dv.y, T(-0.5) * dv.x);   // -i (Z[k] - conj Z[L-k]) / 2
+    const cplx<T> h = hf[k];
+    *xa = T(2) * (h.x * va.x - h.y * va.y);
+    *xb = T(2) * (h.x * vb.x - h.y * vb.y);
+    // at L-k: V[L-k] = conj(V[k]) and exp(-i pi (L-k)/2L) = -i conj(h)
+    const cplx<T> hm = cmk<T>(-h.y, -h.x);
+    *xaM = T(2) * (hm.x * va.x + hm.y * va.y);
+    *xbM = T(2) * (hm.x * vb.x + hm.y * vb.y);
+}
+
+// REDFT01 input pair: Z[j] and Z[L-j] of the line a (a[L] = 0) given a[j] and
+// a[L-j] (for j = 0 pass aM = 0).
+template <typename T>
+__device__ inline void dct3InPair(T aj, T aM, cplx<T> h, cplx<T>* zj, cplx<T>* zM) {
+    // exp(+i pi j / 2L) = conj(h); exp(+i pi (L-j)/2L) = i h
+    const cplx<T> g = cconj<T>(h), gm = cmk<T>(-h.y, h.x);
+    *zj = cmul<T>(cmk<T>(aj, -aM), g);
+    *zM = cmul<T>(cmk<T>(aM, -aj), gm);
+}
+
+template <typename T>
+struct FftTab {
+    const cplx<T>* tw;   // exp(-2 pi i t / L), t < L
+    const cplx<T>* hf;   // exp(-i pi k / 2L), k < L
+};
+
+// Global loads are issued kFftBatch at a time into registers before their
+// LDS stores, so their latencies overlap.
+constexpr int kFftBatch = 8;
+
+// Threads per block of the FFT passes: one block transforms one line pair
+// (passes A, B) or one row (pass C); two waves for lines of 512 and more.
+inline int fftThreads(int L) { return L >= 512 ? 128 : 64; }
+
+// Pass A: REDFT10 along y of rows 2 blockIdx.x and 2 blockIdx.x + 1.
+template <typename T>
+__global__ __launch_bounds__(128) void k_dct_rows(const T* __restrict__ rho, T* __restrict__ out,
+                                                  int nx, int ny, int ld, FftTab<T> tab,
+                                                  const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int L = ny, nt = blockDim.x, tid = threadIdx.x;
+    const FftLds<T> f = fftLds<T>(smem, L);
+    cplx<T>* a[1] = {f.waves};
+    cplx<T>* b[1] = {f.waves + L};
+    loadTables<T>(f, tab.tw, tab.hf, nullptr, L);
+    const int r0 = 2 * blockIdx.x;
+    const T* xa = rho + static_cast<long>(r0) * ld;
+    const T* xb = xa + ld;
+    for (int i0 = tid; i0 < L; i0 += nt * kFftBatch) {
+        T va[kFftBatch], vb[kFftBatch];
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int i = i0 + nt * q;
+            va[q] = i < L ? xa[i] : T(0);
+            vb[q] = i < L ? xb[i] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int i = i0 + nt * q;
+            if (i < L) a[0][dctPos(i, L)] = cmk<T>(va[q], vb[q]);
+        }
+    }
+    __syncthreads();
+    blockFft<T, false, 1>(a, b, f.tw, L, tid, nt);
+    const cplx<T>* Z = a[0];
+    T* oa = out + static_cast<long>(r0) * ld;
+    T* ob = oa + ld;
+    for (int k = tid; k <= L / 2; k += nt) {
+        T ra, rb, raM, rbM;
+        dct2Pair<T>(Z, f.hf, k, L, &ra, &rb, &raM, &rbM);
+        oa[k] = ra;
+        ob[k] = rb;
+        if (k > 0 && k < L / 2) {
+            oa[L - k] = raM;
+            ob[L - k] = rbM;
+        }
+    }
+}
+
+// Pass B: G column pairs per block (pair g: columns c0 + 2g, c0 + 2g + 1),
+// one team of nt = blockDim.x / G threads each: REDFT10 along x, the
+// spectral divide (cpu_poisson.cpp:163-173), then both inverse transforms
+// along x at once (cosine -> Hc, shifted sine of u P -> Hs).  The block
+// loads and stores 2G adjacent values per row.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
+                                                   T* __restrict__ Hs, int G, int nx, int ny,
+                                                   int ld, FftTab<T> tab, const T* __restrict__ u,
+                                                   const T* __restrict__ v, T invNorm,
+                                                   const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int L = nx, nt = blockDim.x / G;
+    const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
+    const FftLds<T> f = fftLds<T>(smem, L);
+    const cplx<T>* hf = f.hf;
+    const T* us = f.sc;
+    cplx<T>* region = f.waves;   // team g owns region[4 L g, 4 L (g + 1))
+    cplx<T>* b0 = region + static_cast<long>(team) * 4 * L;
+    cplx<T>* b1 = b0 + L;
+    cplx<T>* b2 = b1 + L;
+    cplx<T>* b3 = b2 + L;
+    loadTables<T>(f, tab.tw, tab.hf, u, L);
+    const int C = 2 * G, c0 = blockIdx.x * C;
+    // element e: row e / C, column c0 + e % C (C adjacent values per row)
+    const int total = L * C;
+    for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kFftBatch) {
+        T vals[kFftBatch];
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int e = e0 + blockDim.x * q;
+            const int x = e / C, c = e - x * C;
+            vals[q] = e < total ? F1[static_cast<long>(x) * ld + c0 + c] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int e = e0 + blockDim.x * q;
+            if (e < total) {
+                const int x = e / C, c = e - x * C;
+                reinterpret_cast<T*>(region + static_cast<long>(c >> 1) * 4 * L + dctPos(x, L))[c & 1] =
+                    vals[q];
+            }
+        }
+    }
+    const int ka = c0 + 2 * team;
+    const T va = v[ka], vb = v[ka + 1];
+    __syncthreads();
+    cplx<T>* fa[1] = {b0};
+    cplx<T>* fb[1] = {b1};
+    blockFft<T, false, 1>(fa, fb, f.tw, L, tid, nt);
+    cplx<T>* r = fa[0];
+    cplx<T>* o = fb[0];
+    // each thread owns the pairs (j, L-j): REDFT10 outputs, spectral divide,
+    // then the inputs of the two inverse transforms (W2 in place in r, W1 in o)
+    for (int j = tid; j <= L / 2; j += nt) {
+        const int jm = (L - j) & (L - 1);
+        T ga, gb, gaM, gbM;
+        dct2Pair<T>(r, hf, j, L, &ga, &gb, &gaM, &gbM);
+        if (j == 0 || j == L / 2) {
+            gaM = ga;
+            gbM = gb;
+        }
+        const T uj = us[j], um = us[jm];
+        auto spec = [&](T g, T uu, T vv) {
+            const T den = uu * uu + vv * vv;
+            return den > T(0) ? g * invNorm / den : T(0);
+        };
+        const T pa = spec(ga, uj, va), pb = spec(gb, uj, vb);
+        const T paM = j == 0 ? T(0) : spec(gaM, um, va), pbM = j == 0 ? T(0) : spec(gbM, um, vb);
+        const cplx<T> h = hf[j];
+        cplx<T> za, zaM, zb, zbM;
+        dct3InPair<T>(pa, paM, h, &za, &zaM);
+        dct3InPair<T>(pb, pbM, h, &zb, &zbM);
+        const cplx<T> w1 = cmk<T>(za.x - zb.y, za.y + zb.x), w1M = cmk<T>(zaM.x - zbM.y, zaM.y + zbM.x);
+        // shifted sine: a2[m] = u[L-m] P[L-m] (m >= 1), a2[0] = 0
+        const T sa = j == 0 ? T(0) : um * paM, sb = j == 0 ? T(0) : um * pbM;
+        const T saM = j == 0 ? T(0) : uj * pa, sbM = j == 0 ? T(0) : uj * pb;
+        dct3InPair<T>(sa, saM, h, &za, &zaM);
+        dct3InPair<T>(sb, sbM, h, &zb, &zbM);
+        const cplx<T> w2 = cmk<T>(za.x - zb.y, za.y + zb.x), w2M = cmk<T>(zaM.x - zbM.y, zaM.y + zbM.x);
+        o[j] = w1;
+        r[j] = w2;
+        if (j > 0 && j < L / 2) {
+            o[L - j] = w1M;
+            r[L - j] = w2M;
+        }
+    }
+    __syncthreads();
+    cplx<T>* in[2] = {o, r};
+    cplx<T>* sc2[2] = {b2, b3};
+    blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
+    // the results sit at the same offsets in every team's region
+    const long off1 = in[0] - b0, off2 = in[1] - b0;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int x = e / C, c = e - x * C;
+        const cplx<T>* w = region + static_cast<long>(c >> 1) * 4 * L;
+        const int pos = dctPos(x, L);
+        const cplx<T> z1 = w[off1 + pos], z2 = w[off2 + pos];
+        const long at = static_cast<long>(x) * ld + c0 + c;
+        Hc[at] = (c & 1) ? z1.y : z1.x;
+        const T sv = (c & 1) ? z2.y : z2.x;
+        Hs[at] = (x & 1) ? -sv : sv;
+    }
+}
+
+// Pass C: row blockIdx.x: xi_x = REDFT01_y(Hs[x]) and xi_y = shifted sine of
+// v Hc[x], packed in one complex transform, and phi = REDFT01_y(Hc[x]) when
+// requested (both transforms at once).  The field goes to fx[e * cstep],
+// fy[e * cstep] (cstep 2 interleaves).
+template <typename T>
+__global__ __launch_bounds__(128) void k_idct_rows(const T* __restrict__ Hc,
+                                                   const T* __restrict__ Hs, T* __restrict__ fx,
+                                                   T* __restrict__ fy, T* __restrict__ phi,
+                                                   int cstep, int nx, int ny, int ld,
+                                                   FftTab<T> tab, const T* __restrict__ v,
+                                                   const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int L = ny, nt = blockDim.x, tid = threadIdx.x;
+    const FftLds<T> f = fftLds<T>(smem, L);
+    const cplx<T>* hf = f.hf;
+    const T* vs = f.sc;
+    cplx<T>* a = f.waves;
+    cplx<T>* b = a + L;
+    cplx<T>* c = phi ? b + L : nullptr;
+    cplx<T>* d = phi ? c + L : nullptr;
+    loadTables<T>(f, tab.tw, tab.hf, v, L);
+    const int x = blockIdx.x;
+    // the row pair (Hs[x][j], Hc[x][j]) into b
+    const T* hs = Hs + static_cast<long>(x) * ld;
+    const T* hc = Hc + static_cast<long>(x) * ld;
+    for (int i0 = tid; i0 < L; i0 += nt * kFftBatch) {
+        T vsv[kFftBatch], vcv[kFftBatch];
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int i = i0 + nt * q;
+            vsv[q] = i < L ? hs[i] : T(0);
+            vcv[q] = i < L ? hc[i] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kFftBatch; ++q) {
+            const int i = i0 + nt * q;
+            if (i < L) b[i] = cmk<T>(vsv[q], vcv[q]);
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j <= L / 2; j += nt) {
+        const int jm = (L - j) & (L - 1);
+        const cplx<T> pj = b[j], pm = j == 0 ? cmk<T>(0, 0) : b[jm];
+        const cplx<T> h = hf[j];
+        cplx<T> za, zaM, zb, zbM;
+        dct3InPair<T>(pj.x, pm.x, h, &za, &zaM);
+        // shifted sine of v Hc: a2[m] = v[L-m] Hc[L-m] (m >= 1), a2[0] = 0
+        const T bj = j == 0 ? T(0) : vs[jm] * pm.y, bm = j == 0 ? T(0) : vs[j] * pj.y;
+        dct3InPair<T>(bj, bm, h, &zb, &zbM);
+        a[j] = cmk<T>(za.x - zb.y, za.y + zb.x);
+        if (j > 0 && j < L / 2) a[L - j] = cmk<T>(zaM.x - zbM.y, zaM.y + zbM.x);
+        if (phi) {
+            dct3InPair<T>(pj.y, pm.y, h, &za, &zaM);
+            c[j] = za;
+            if (j > 0 && j < L / 2) c[L - j] = zaM;
+        }
+    }
+    __syncthreads();
+    const long row = static_cast<long>(x) * ld;
+    if (phi) {
+        cplx<T>* in[2] = {a, c};
+        cplx<T>* sc2[2] = {b, d};
+        blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
+        for (int y = tid; y < L; y += nt) phi[row + y] = in[1][dctPos(y, L)].x;
+        a = in[0];
+    } else {
+        cplx<T>* in[1] = {a};
+        cplx<T>* sc1[1] = {b};
+        blockFft<T, true, 1>(in, sc1, f.tw, L, tid, nt);
+        a = in[0];
+    }
+    for (int y = tid; y < L; y += nt) {
+        const cplx<T> w = a[dctPos(y, L)];
+        fx[(row + y) * cstep] = w.x;
+        fy[(row + y) * cstep] = (y & 1) ? -w.y : w.y;
+    }
+}
+
 template <typename T, int EPI, bool TRANS>
 void gemmDispatch(int batch, const GemmBatch<T>& gb, int M, int N, int K, int lda, int ldb,
                   int ldc, const SpectralEpi<T>& epi, hipStream_t s, const int* skip = nullptr,
@@ -250,6 +720,23 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
     if (!(stepX > 0) || !(stepY > 0)) throw std::invalid_argument("bin step must be positive");
     const double pi = 3.14159265358979323846;
     const size_t gx = static_cast<size_t>(mx_) * my_;
+    fft_ = fftShape(nx) && fftShape(ny);
+    if (fft_) {
+        // twiddles exp(-2 pi i t/L) and half-angle factors exp(-i pi k/2L)
+        auto tables = [&](int L) {
+            std::vector<double> t(4 * static_cast<size_t>(L));
+            for (int k = 0; k < L; ++k) {
+                t[2 * k] = std::cos(2 * pi * k / L);
+                t[2 * k + 1] = -std::sin(2 * pi * k / L);
+                t[2 * L + 2 * k] = std::cos(pi * k / (2.0 * L));
+                t[2 * L + 2 * k + 1] = -std::sin(pi * k / (2.0 * L));
+            }
+            return upload<T>(t, stream);
+        };
+        tabX_ = tables(nx);
+        tabY_ = tables(ny);
+        setFftLds();
+    }
     // Transform matrices in FP64, zero in the padding (cpu_poisson.cpp:14-36).
     std::vector<double> cyT(static_cast<size_t>(my_) * my_, 0.0), ayT(cyT.size(), 0.0),
         bsv(cyT.size(), 0.0);
@@ -281,12 +768,14 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
             sxu[static_cast<size_t>(x) * mx_ + j] =
                 j == 0 ? 0.0 : 2.0 * u[j] * std::sin(pi * j * (x + 0.5) / nx);
         }
-    cyT_ = upload<T>(cyT, stream);
-    ayT_ = upload<T>(ayT, stream);
-    bsv_ = upload<T>(bsv, stream);
-    cx_ = upload<T>(cx, stream);
-    sxu_ = upload<T>(sxu, stream);
-    ax_ = upload<T>(ax, stream);
+    if (!fft_) {
+        cyT_ = upload<T>(cyT, stream);
+        ayT_ = upload<T>(ayT, stream);
+        bsv_ = upload<T>(bsv, stream);
+        cx_ = upload<T>(cx, stream);
+        sxu_ = upload<T>(sxu, stream);
+        ax_ = upload<T>(ax, stream);
+    }
     // the reference computes u, v in float (cpu_poisson.cpp:88-94); the FP32
     // path keeps them as float, the FP64 path in double
     std::vector<double> uf(u), vf(v);
@@ -320,19 +809,81 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
 template <typename T>
 PoissonPlan<T>::~PoissonPlan() {
     for (T* b : {rho_, phi_, ex_, ey_, exy_, t0_, p_, t1_, t2_, cyT_, cx_, ayT_, bsv_, sxu_, ax_, u_,
-                 v_})
+                 v_, tabX_, tabY_})
         if (b) (void)hipFree(b);
 }
 
 template <typename T>
 double PoissonPlan<T>::flops(bool withPhi) const {
+    if (fft_) return 0.0;
     const double sq_y = 2.0 * mx_ * my_ * my_, sq_x = 2.0 * mx_ * mx_ * my_;
     // forward: y then x; inverse: 2 products along y, 2 (or 3) along x
     return sq_y + sq_x + 2 * sq_y + (withPhi ? 3 : 2) * sq_x;
 }
 
 template <typename T>
+double PoissonPlan<T>::bytes(bool withPhi) const {
+    if (!fft_) return 0.0;
+    // pass A: rho in, F1 out; pass B: F1 in, Hc and Hs out; pass C: Hc and
+    // Hs in, the two field components (and phi) out
+    return (9.0 + (withPhi ? 1 : 0)) * nx_ * ny_ * sizeof(T);
+}
+
+template <typename T>
+bool PoissonPlan<T>::fftShape(int n) {
+    return n >= 4 && n <= 1024 && (n & (n - 1)) == 0;
+}
+
+template <typename T>
+void PoissonPlan<T>::setFftLds() {
+    const size_t c = 2 * sizeof(T);
+    ldsA_ = (kFftLdsHead(ny_) + 2 * static_cast<size_t>(ny_)) * c;
+    // column pairs per block of the column pass: one (so that the pass runs
+    // on ny / 2 blocks; measured faster at 512^2 than gathering 4 or 8 pairs
+    // per block for longer row segments), within 150 KB of LDS
+    colPairs_ = 1;
+    while (colPairs_ > 1 && (colPairs_ > ny_ / 2 ||
+                             (kFftLdsHead(nx_) + 4 * static_cast<size_t>(colPairs_) * nx_) * c >
+                                 150 * 1024))
+        colPairs_ >>= 1;
+    ldsB_ = (kFftLdsHead(nx_) + 4 * static_cast<size_t>(colPairs_) * nx_) * c;
+    ldsC_ = (kFftLdsHead(ny_) + 4 * static_cast<size_t>(ny_)) * c;
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsB_));
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_idct_rows<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsC_));
+}
+
+template <typename T>
+void PoissonPlan<T>::solveFft(bool withPhi, const int* skip) {
+    using C2 = cplx<T>;
+    const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
+    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    const int ld = my_;
+    launchKernel(k_dct_rows<T>, dim3(nx_ / 2), dim3(fftThreads(ny_)),
+                 static_cast<uint32_t>(ldsA_), stream_, rho_, t0_, nx_, ny_, ld, ty, skip);
+    SPHIP_CHECK(hipGetLastError());
+    launchKernel(k_dct_cols<T>, dim3(ny_ / (2 * colPairs_)), dim3(colPairs_ * fftThreads(nx_)),
+                 static_cast<uint32_t>(ldsB_), stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx,
+                 u_, v_, invNorm_, skip);
+    SPHIP_CHECK(hipGetLastError());
+    T* fx = interleaved_ ? exy_ : ex_;
+    T* fy = interleaved_ ? exy_ + 1 : ey_;
+    const size_t lds = withPhi ? ldsC_ : ldsC_ - 2 * static_cast<size_t>(ny_) * 2 * sizeof(T);
+    launchKernel(k_idct_rows<T>, dim3(nx_), dim3(fftThreads(ny_)), static_cast<uint32_t>(lds),
+                 stream_, t1_, t2_, fx, fy, withPhi ? phi_ : nullptr, interleaved_ ? 2 : 1, nx_,
+                 ny_, ld, ty, v_, skip);
+    SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
 void PoissonPlan<T>::solve(bool withPhi, const int* skip) {
+    if (fft_) {
+        solveFft(withPhi, skip);
+        return;
+    }
     const int M = mx_, N = my_;
     SpectralEpi<T> none{};
     // Intermediates alternate between transposed and plain layouts so that

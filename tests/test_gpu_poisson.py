@@ -1,4 +1,5 @@
-"""HIP Poisson backend (DCT as dense transforms on MFMA) against the oracle,
+"""HIP Poisson backend (LDS FFT passes for power-of-two grids, dense
+transforms on MFMA for other sizes) against the oracle,
 the scipy golden fixtures and the analytic properties of
 test/test_poisson.cpp; field tolerance max|d|/max|ref| < 1e-5 for FP32 (the
 reference's own CPU/GPU criterion, test/test_poisson_parity.cpp:22) and
@@ -34,7 +35,8 @@ def test_matches_oracle(shape):
     rho = rng.uniform(0, 1, shape).astype(np.float32)
     sx, sy = 22.65625, 21.0
     be, (phi, ex, ey) = solve(rho, sx, sy)
-    assert be.name() == "hip-mfma"
+    pow2 = all(n >= 4 and n & (n - 1) == 0 for n in shape)
+    assert be.name() == ("hip-fft" if pow2 else "hip-mfma")
     rphi, rex, rey = oracle.poisson(rho.astype(np.float64), sx, sy)
     print(shape, rel(phi, rphi), rel(ex, rex), rel(ey, rey), flush=True)
     assert rel(phi, rphi) < 1e-5 and rel(ex, rex) < 1e-5 and rel(ey, rey) < 1e-5
@@ -44,7 +46,7 @@ def test_fp64_path_matches_oracle_tightly():
     nx = ny = 128
     rho = np.random.default_rng(5).uniform(0, 1, (nx, ny))
     be = hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 3.0, 3.0, fp64=True))
-    assert be.name() == "hip-mfma-fp64"
+    assert be.name() == "hip-fft-fp64"
     from tests.hiprt import DevArray
     d_in = DevArray(rho.astype(np.float64))
     d_out = DevArray.zeros(3 * nx * ny, np.float64)
