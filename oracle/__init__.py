@@ -364,3 +364,79 @@ def ref_hpwl(d, x=None, y=None):
             np.ascontiguousarray(d.pin2node, np.int32), np.ascontiguousarray(d.pin2net, np.int32),
             _f32(d.pin_offset_x), _f32(d.pin_offset_y)]
     return L.ref_hpwl(d.num_movable, n, d.num_pins, d.num_nets, *[_ptr(a) for a in args])
+
+
+# ---- quadratic initial placement (src/init/quadratic_placer.cpp) -----------
+class OraQpConfig(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("tol", C.c_float), ("solver_max_iter", C.c_int32),
+                ("outer_tol", C.c_float), ("min_distance", C.c_float),
+                ("ignore_net_degree", C.c_int32)]
+
+
+def qp_config(max_iter=20, tol=1e-6, solver_max_iter=100, outer_tol=1e-4, min_distance=1.0,
+              ignore_net_degree=100):
+    """The QP fields of Config (src/util/config.h:49-74, :83) with their defaults."""
+    return OraQpConfig(max_iter, tol, solver_max_iter, outer_tol, min_distance,
+                       ignore_net_degree)
+
+
+def bound_pins(d, x, y):
+    L = lib()
+    L.ora_bound_pins.argtypes = [_P] * 7
+    ds = design_struct(d)
+    x, y = _f32(x), _f32(y)
+    out = [np.zeros(d.num_nets, np.int32) for _ in range(4)]
+    L.ora_bound_pins(C.byref(ds.s), _ptr(x), _ptr(y), *[_ptr(o) for o in out])
+    return out
+
+
+def b2b_system(d, x, y, axis, net_weight=None, min_distance=1.0, ignore_net_degree=100):
+    """(diag, rhs, row_start, col, val) of one axis' Bound2Bound system."""
+    L = lib()
+    L.ora_b2b_system.restype = C.c_int64
+    L.ora_b2b_system.argtypes = [_P, _P, _P, _P, C.c_int, C.c_float, C.c_int] + [_P] * 5
+    ds = design_struct(d)
+    x, y = _f32(x), _f32(y)
+    w = None if net_weight is None else _f32(net_weight)
+    wp = None if w is None else _ptr(w)
+    nnz = L.ora_b2b_system(C.byref(ds.s), _ptr(x), _ptr(y), wp, int(axis), min_distance,
+                           int(ignore_net_degree), None, None, None, None, None)
+    n = d.num_movable
+    diag, rhs = np.zeros(n), np.zeros(n)
+    rs = np.zeros(n + 1, np.int64)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    L.ora_b2b_system(C.byref(ds.s), _ptr(x), _ptr(y), wp, int(axis), min_distance,
+                     int(ignore_net_degree), _ptr(diag), _ptr(rhs), _ptr(rs), _ptr(col), _ptr(val))
+    return diag, rhs, rs, col[:nnz], val[:nnz]
+
+
+def bicgstab(diag, rhs, rs, col, val, x0, tol=1e-6, max_iter=100):
+    L = lib()
+    L.ora_bicgstab.restype = C.c_double
+    L.ora_bicgstab.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int, _P]
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    its = C.c_int(0)
+    c = np.ascontiguousarray(col, np.int32) if len(col) else np.zeros(1, np.int32)
+    v = np.ascontiguousarray(val, np.float64) if len(val) else np.zeros(1)
+    err = L.ora_bicgstab(len(diag), _ptr(np.ascontiguousarray(diag, np.float64)),
+                         _ptr(np.ascontiguousarray(rs, np.int64)), _ptr(c), _ptr(v),
+                         _ptr(np.ascontiguousarray(rhs, np.float64)), _ptr(x), float(tol),
+                         int(max_iter), C.byref(its))
+    return x, err, its.value
+
+
+def qp_place(d, cfg=None, net_weight=None, max_rows=64):
+    """QuadraticPlacer::place: (x, y, hpwl per iteration, err per iteration)."""
+    L = lib()
+    L.ora_qp_place.restype = C.c_int
+    L.ora_qp_place.argtypes = [_P, _P, _P, _P, _P, _P, _P, C.c_int]
+    cfg = cfg or qp_config()
+    ds = design_struct(d)
+    x = _f32(d.node_x[:d.num_nodes]).copy()
+    y = _f32(d.node_y[:d.num_nodes]).copy()
+    hp, er = np.zeros(max_rows), np.zeros(max_rows)
+    w = None if net_weight is None else _f32(net_weight)
+    n = L.ora_qp_place(C.byref(ds.s), None if w is None else _ptr(w), C.byref(cfg), _ptr(x),
+                       _ptr(y), _ptr(hp), _ptr(er), max_rows)
+    return x, y, hp[:min(n, max_rows)], er[:min(n, max_rows)]

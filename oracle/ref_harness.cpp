@@ -1,7 +1,7 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
 // src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp,
-// src/gp/filler.cpp),
+// src/gp/filler.cpp, and the header-only src/init/b2b_model.h),
 // compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
 //
@@ -15,6 +15,7 @@
 #include "db/hpwl.h"
 #include "db/place_db.h"
 #include "gp/filler.h"
+#include "init/b2b_model.h"
 #include "io/bookshelf_reader.h"
 #include "io/bookshelf_writer.h"
 #include "gp/wa_wirelength.h"
@@ -194,6 +195,48 @@ int ref_bookshelf_write_pl(void* h, const char* path, const float* x, const floa
 }
 
 void ref_bookshelf_free(void* h) { delete static_cast<sp::PlaceDB*>(h); }
+
+// computeNetBoundPins (src/db/hpwl.cpp:70-102) and forEachB2BPair
+// (src/init/b2b_model.h:56-82) on a database built from arrays: the bound
+// pins of every net into bounds (minX, maxX, minY, maxY; 4 x numNets) and,
+// for one axis, every pair (p, q, w) in enumeration order over the nets of
+// degree <= ignoreDeg (the filter assemble applies, quadratic_placer.cpp:71).
+// Returns the pair count; at most cap pairs are written.
+int64_t ref_b2b_pairs(int numMovable, int numNodes, int numPins, int numNets, const float* x,
+                      const float* y, const float* w, const float* h, const uint8_t* flags,
+                      const int32_t* pin2node, const int32_t* pin2net, const float* offx,
+                      const float* offy, const float* netWeight, int axis, float minDist,
+                      int ignoreDeg, int32_t* bounds, int32_t* pp, int32_t* qq, float* ww,
+                      int64_t cap) {
+    sp::PlaceDB db = buildDb(numMovable, numNodes, numPins, numNets, x, y, w, h, flags, pin2node,
+                             pin2net, offx, offy);
+    if (netWeight) db.net_weight.assign(netWeight, netWeight + numNets);
+    std::vector<int> a, b, c, d;
+    sp::computeNetBoundPins(db, a, b, c, d);
+    for (int k = 0; k < numNets; ++k) {
+        bounds[k] = a[k];
+        bounds[numNets + k] = b[k];
+        bounds[2 * numNets + k] = c[k];
+        bounds[3 * numNets + k] = d[k];
+    }
+    int64_t n = 0;
+    auto put = [&](int p, int q, float wt) {
+        if (n < cap) {
+            pp[n] = p;
+            qq[n] = q;
+            ww[n] = wt;
+        }
+        ++n;
+    };
+    for (int k = 0; k < numNets; ++k) {
+        if (db.netDegree(k) > ignoreDeg) continue;
+        if (axis == 0)
+            sp::forEachB2BPair<sp::Axis::X>(db, k, a[k], b[k], minDist, put);
+        else
+            sp::forEachB2BPair<sp::Axis::Y>(db, k, c[k], d[k], minDist, put);
+    }
+    return n;
+}
 
 // initializeFillers (filler.cpp:47-139) on a database read by
 // ref_bookshelf_read, with Config{target_density, random_seed}.  info:

@@ -1029,6 +1029,33 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
 
 // ===========================================================================
 // C API
+namespace {
+
+// Partners of pin p on axis positions pos (b2b_model.h:56-82 seen from one
+// pin): calls f(q, w) in net order.
+template <typename F>
+void b2bPartners(const ora_design* d, const float* pos, const float* off, const float* netW, int p,
+                 int bmin, int bmax, float minDist, int ignoreDeg, F&& f) {
+    const int k = d->pin2net[p];
+    const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
+    const int deg = e - b;
+    if (deg < 2 || deg > ignoreDeg) return;
+    const float coef = (netW ? netW[k] : 1.f) / static_cast<float>(deg - 1);
+    const float minD = std::max(1e-6f, minDist);
+    const int np = d->pin2node[p];
+    const float pp = pos[np] + off[p];
+    const bool pBound = p == bmin || p == bmax;
+    for (int t = b; t < e; ++t) {
+        const int q = d->flat_net2pin[t];
+        if (q == p || d->pin2node[q] == np) continue;
+        if (!pBound && q != bmin && q != bmax) continue;
+        const float dist = std::fabs(pp - (pos[d->pin2node[q]] + off[q]));
+        f(q, coef / std::max(dist, minD));
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 ora_grid* ora_grid_create(const ora_design* d, const float* x, const float* y,
@@ -1219,6 +1246,211 @@ int ora_eplace_run(const ora_design* d, float* x, float* y, const ora_gp_config*
                    ora_gp_result* res, ora_iter_metrics* rows, int maxRows) {
     if (cfg->precision == 32) return eplaceRun<float>(d, x, y, cfg, res, rows, maxRows);
     return eplaceRun<double>(d, x, y, cfg, res, rows, maxRows);
+}
+
+// ---- quadratic initial placement --------------------------------------------
+
+// computeNetBoundPins, src/db/hpwl.cpp:70-102.
+void ora_bound_pins(const ora_design* d, const float* x, const float* y, int32_t* minX,
+                    int32_t* maxX, int32_t* minY, int32_t* maxY) {
+    for (int k = 0; k < d->num_nets; ++k) {
+        const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
+        if (b == e) {
+            minX[k] = maxX[k] = minY[k] = maxY[k] = -1;
+            continue;
+        }
+        auto px = [&](int p) { return x[d->pin2node[p]] + d->pin_offset_x[p]; };
+        auto py = [&](int p) { return y[d->pin2node[p]] + d->pin_offset_y[p]; };
+        int p0 = d->flat_net2pin[b];
+        int a = p0, bb = p0, c = p0, dd = p0;
+        float lox = px(p0), hix = lox, loy = py(p0), hiy = loy;
+        for (int t = b + 1; t < e; ++t) {
+            const int p = d->flat_net2pin[t];
+            const float vx = px(p), vy = py(p);
+            if (vx < lox) { lox = vx; a = p; }
+            if (vx > hix) { hix = vx; bb = p; }
+            if (vy < loy) { loy = vy; c = p; }
+            if (vy > hiy) { hiy = vy; dd = p; }
+        }
+        minX[k] = a; maxX[k] = bb; minY[k] = c; maxY[k] = dd;
+    }
+}
+
+
+// assemble, src/init/quadratic_placer.cpp:43-113 (double sums, row order).
+int64_t ora_b2b_system(const ora_design* d, const float* x, const float* y, const float* netW,
+                       int axis, float minDist, int ignoreDeg, double* diag, double* rhs,
+                       int64_t* rowStart, int32_t* col, double* val) {
+    const int nMov = d->num_movable;
+    std::vector<int32_t> bmin(d->num_nets), bmax(d->num_nets), tmin(d->num_nets),
+        tmax(d->num_nets);
+    if (axis == 0) ora_bound_pins(d, x, y, bmin.data(), bmax.data(), tmin.data(), tmax.data());
+    else ora_bound_pins(d, x, y, tmin.data(), tmax.data(), bmin.data(), bmax.data());
+    const float* pos = axis == 0 ? x : y;
+    const float* off = axis == 0 ? d->pin_offset_x : d->pin_offset_y;
+    int64_t nnz = 0;
+    for (int i = 0; i < nMov; ++i) {
+        if (rowStart) rowStart[i] = nnz;
+        double dg = 0.0, r = 0.0;
+        for (int s = d->node2pin_start[i]; s < d->node2pin_start[i + 1]; ++s) {
+            const int p = d->flat_node2pin[s];
+            const int k = d->pin2net[p];
+            b2bPartners(d, pos, off, netW, p, bmin[k], bmax[k], minDist, ignoreDeg,
+                        [&](int q, float w) {
+                            const int j = d->pin2node[q];
+                            dg += static_cast<double>(w);
+                            if (j < nMov) {
+                                if (col) {
+                                    col[nnz] = j;
+                                    val[nnz] = -static_cast<double>(w);
+                                }
+                                ++nnz;
+                                r += -static_cast<double>(w) *
+                                     (static_cast<double>(off[p]) - static_cast<double>(off[q]));
+                            } else {
+                                r += static_cast<double>(w) *
+                                     (static_cast<double>(pos[j] + off[q]) -
+                                      static_cast<double>(off[p]));
+                            }
+                        });
+        }
+        if (dg == 0.0) {
+            dg = 1e-6;
+            r += 1e-6 * static_cast<double>(pos[i]);
+        }
+        if (diag) {
+            diag[i] = dg;
+            rhs[i] = r;
+        }
+    }
+    if (rowStart) rowStart[nMov] = nnz;
+    return nnz;
+}
+
+// Right-preconditioned BiCGSTAB with a Jacobi preconditioner (H. A. van der
+// Vorst, SIAM J. Sci. Stat. Comput. 13(2), 1992).
+double ora_bicgstab(int n, const double* diag, const int64_t* rowStart, const int32_t* col,
+                    const double* val, const double* b, double* x, double tol, int maxIter,
+                    int* iters) {
+    auto spmv = [&](const std::vector<double>& in, std::vector<double>& out) {
+        for (int i = 0; i < n; ++i) {
+            double acc = diag[i] * in[i];
+            for (int64_t e = rowStart[i]; e < rowStart[i + 1]; ++e) acc += val[e] * in[col[e]];
+            out[i] = acc;
+        }
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
+        double acc = 0.0;
+        for (int i = 0; i < n; ++i) acc += a[i] * c[i];
+        return acc;
+    };
+    std::vector<double> xv(x, x + n), r(n), r0(n), p(n, 0.0), v(n, 0.0), y(n), s(n), z(n), t(n);
+    spmv(xv, r);
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    r0 = r;
+    double bnorm2 = 0.0;
+    for (int i = 0; i < n; ++i) bnorm2 += b[i] * b[i];
+    int it = 0;
+    double rr = dot(r, r);
+    if (bnorm2 == 0.0) {
+        for (int i = 0; i < n; ++i) x[i] = 0.0;
+        if (iters) *iters = 0;
+        return 0.0;
+    }
+    const double tol2 = static_cast<double>(tol) * tol * bnorm2;
+    double rho = 1.0, alpha = 1.0, w = 1.0, rhoNew = rr;
+    while (rr > tol2 && it < maxIter) {
+        const double rhoOld = rho;
+        rho = rhoNew;
+        if (rhoOld == 0.0 || w == 0.0) break;
+        const double beta = (rho / rhoOld) * (alpha / w);
+        for (int i = 0; i < n; ++i) {
+            p[i] = r[i] + beta * (p[i] - w * v[i]);
+            y[i] = p[i] / diag[i];
+        }
+        spmv(y, v);
+        const double r0v = dot(r0, v);
+        if (r0v == 0.0) break;
+        alpha = rho / r0v;
+        for (int i = 0; i < n; ++i) {
+            s[i] = r[i] - alpha * v[i];
+            z[i] = s[i] / diag[i];
+        }
+        spmv(z, t);
+        const double tt = dot(t, t), ts = dot(t, s);
+        w = tt > 0.0 ? ts / tt : 0.0;
+        for (int i = 0; i < n; ++i) {
+            xv[i] += alpha * y[i] + w * z[i];
+            r[i] = s[i] - w * t[i];
+        }
+        rr = dot(r, r);
+        rhoNew = dot(r0, r);
+        ++it;
+    }
+    for (int i = 0; i < n; ++i) x[i] = xv[i];
+    if (iters) *iters = it;
+    return std::sqrt(rr / bnorm2);
+}
+
+// QuadraticPlacer::place, src/init/quadratic_placer.cpp:149-228.
+int ora_qp_place(const ora_design* d, const float* netW, const ora_qp_config* cfg, float* x,
+                 float* y, double* hpwlRows, double* errRows, int maxRows) {
+    const int nMov = d->num_movable;
+    if (nMov == 0) return 0;
+    const float cx = 0.5f * (d->core_lx + d->core_hx), cy = 0.5f * (d->core_ly + d->core_hy);
+    for (int i = 0; i < nMov; ++i) {
+        x[i] = cx;
+        y[i] = cy;
+    }
+    std::vector<double> solX(nMov, cx), solY(nMov, cy), diag(nMov), rhs(nMov);
+    std::vector<int64_t> rs(nMov + 1);
+    double last = 0.0;
+    int iter = 0;
+    for (;; ++iter) {
+        double err[2] = {0, 0};
+        // both axes take the bound pins of the positions at the start of the
+        // iteration (computeNetBoundPins runs once per iteration)
+        const std::vector<float> x0(x, x + d->num_nodes), y0(y, y + d->num_nodes);
+        for (int axis = 0; axis < 2; ++axis) {
+            const int64_t nnz = ora_b2b_system(d, x0.data(), y0.data(), netW, axis,
+                                               cfg->min_distance, cfg->ignore_net_degree,
+                                               nullptr, nullptr, nullptr, nullptr, nullptr);
+            std::vector<int32_t> col(nnz);
+            std::vector<double> val(nnz);
+            ora_b2b_system(d, x0.data(), y0.data(), netW, axis, cfg->min_distance,
+                           cfg->ignore_net_degree, diag.data(), rhs.data(), rs.data(), col.data(),
+                           val.data());
+            std::vector<double>& sol = axis == 0 ? solX : solY;
+            float* pos = axis == 0 ? x : y;
+            const float* half = axis == 0 ? d->node_w : d->node_h;
+            const float lo0 = axis == 0 ? d->core_lx : d->core_ly;
+            const float hi0 = axis == 0 ? d->core_hx : d->core_hy;
+            int its = 0;
+            const double e = ora_bicgstab(nMov, diag.data(), rs.data(), col.data(), val.data(),
+                                          rhs.data(), sol.data(), cfg->tol, cfg->solver_max_iter,
+                                          &its);
+            err[axis] = std::isfinite(e) ? e : 0.0;
+            for (int i = 0; i < nMov; ++i) {
+                float v = static_cast<float>(sol[i]);
+                if (!std::isfinite(v)) v = pos[i];
+                const float h = 0.5f * half[i];
+                const float lo = lo0 + h, hi = hi0 - h;
+                v = lo <= hi ? std::min(std::max(v, lo), hi) : 0.5f * (lo + hi);
+                pos[i] = v;
+                sol[i] = v;
+            }
+        }
+        const double hp = hpwl<float>(d, x, y);
+        if (iter < maxRows) {
+            if (hpwlRows) hpwlRows[iter] = hp;
+            if (errRows) errRows[iter] = std::max(err[0], err[1]);
+        }
+        const bool stalled = iter > 4 && last > 0.0 &&
+                             std::fabs(hp - last) < static_cast<double>(cfg->outer_tol) * last;
+        last = hp;
+        if (stalled || iter + 1 >= cfg->max_iter) break;
+    }
+    return iter + 1;
 }
 
 void ora_set_threads(int n) {

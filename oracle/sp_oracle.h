@@ -166,6 +166,59 @@ typedef struct ora_gp_result {
 int ora_eplace_run(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
                    ora_gp_result* res, ora_iter_metrics* metrics, int max_rows);
 
+/* ---- quadratic initial placement (src/init/quadratic_placer.cpp,
+ * src/init/b2b_model.h) ------------------------------------------------ */
+
+/* computeNetBoundPins (src/db/hpwl.cpp:70-102): per net the first pin (in
+ * net order) holding the smallest / largest x and y, pins at node centre +
+ * offset in float; -1 for empty nets. */
+void ora_bound_pins(const ora_design* d, const float* x, const float* y, int32_t* min_x,
+                    int32_t* max_x, int32_t* min_y, int32_t* max_y);
+
+/* The Bound2Bound system of one axis (axis 0 = x, 1 = y) over the movable
+ * nodes (assemble, quadratic_placer.cpp:43-113, pairs of forEachB2BPair,
+ * b2b_model.h:56-82): nets of degree 2 .. ignore_deg, weight
+ * net_weight / (deg-1) / max(|dpos|, max(1e-6, min_dist)) in float.
+ * Row i lists, for each pin p of node i (node->pin order) and each partner q
+ * of p (net order; partners of a bound pin: every pin on another node; of
+ * any other pin: the bound pins on other nodes) with node(q) movable, the
+ * entry (node(q), -w).  diag and rhs are summed in double in the same order;
+ * a zero diagonal becomes 1e-6 with rhs += 1e-6 * position.  net_weight may
+ * be NULL (all 1).  Returns the entry count; row_start (num_movable + 1),
+ * col and val may be NULL to ask for it. */
+int64_t ora_b2b_system(const ora_design* d, const float* x, const float* y,
+                       const float* net_weight, int axis, float min_dist, int ignore_deg,
+                       double* diag, double* rhs, int64_t* row_start, int32_t* col, double* val);
+
+/* Right-preconditioned BiCGSTAB (van der Vorst 1992) with the Jacobi
+ * preconditioner on the system above, from the initial guess x (in/out),
+ * until ||r|| <= tol ||b|| or max_iter iterations; returns ||r|| / ||b||
+ * (the solver error QuadraticPlacer records, quadratic_placer.cpp:130-134).
+ * *iters receives the iterations run. */
+double ora_bicgstab(int n, const double* diag, const int64_t* row_start, const int32_t* col,
+                    const double* val, const double* rhs, double* x, double tol, int max_iter,
+                    int* iters);
+
+typedef struct ora_qp_config {
+    int32_t max_iter;          /* qp_max_iter (outer) */
+    float tol;                 /* qp_tol (BiCGSTAB relative residual) */
+    int32_t solver_max_iter;   /* qp_solver_max_iter */
+    float outer_tol;           /* qp_outer_tol (relative HPWL change) */
+    float min_distance;        /* qp_min_distance */
+    int32_t ignore_net_degree;
+} ora_qp_config;
+
+/* QuadraticPlacer::place (quadratic_placer.cpp:149-228): movable nodes start
+ * at the core centre; each outer iteration takes the bound pins of the
+ * current positions, solves x then y (warm start = previous solution),
+ * clamps every node inside the core and records HPWL and max(err_x, err_y);
+ * it stops after iteration 5 once HPWL changes by less than outer_tol
+ * relative, or after max_iter iterations.  x, y hold all nodes (fixed ones
+ * are read); rows (may be NULL) get hpwl and err per iteration, at most
+ * max_rows.  Returns the iterations run. */
+int ora_qp_place(const ora_design* d, const float* net_weight, const ora_qp_config* cfg,
+                 float* x, float* y, double* hpwl_rows, double* err_rows, int max_rows);
+
 /* OpenMP threads the oracle uses (0 = leave runtime default). */
 void ora_set_threads(int n);
 int ora_get_threads(void);
