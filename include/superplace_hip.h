@@ -242,6 +242,41 @@ int sp_bookshelf_write_pl(const sp_bookshelf* b, const char* path, const float* 
                           const float* y);
 void sp_bookshelf_destroy(sp_bookshelf* b);
 
+/* ---------------------------------------------------- Quadratic placement
+ * Replaces QuadraticPlacer::place (src/init/quadratic_placer.h:18,
+ * src/init/quadratic_placer.cpp:149-228), the Kraftwerk2 Bound2Bound initial
+ * placement: movable nodes start at the core centre; each outer iteration
+ * takes the bound pins of the current positions (computeNetBoundPins,
+ * src/db/hpwl.cpp:70-102), builds both axes' B2B systems (pairs of
+ * forEachB2BPair, src/init/b2b_model.h:56-82), solves them with
+ * Jacobi-preconditioned BiCGSTAB from the previous solution, clamps every
+ * node into the core and records HPWL; it stops after iteration 5 once HPWL
+ * moves by less than outer_tol relative, or after max_iter iterations.  On
+ * the GPU, FP64 solver.  Needs a GPU (SP_HIP_ENODEV without one). */
+typedef struct sp_qp_config {     /* the qp_* fields of Config, src/util/config.h:49-74 */
+    int32_t max_iter;             /* qp_max_iter (20) */
+    float tol;                    /* qp_tol (1e-6): BiCGSTAB relative residual */
+    int32_t solver_max_iter;      /* qp_solver_max_iter (100) */
+    float outer_tol;              /* qp_outer_tol (1e-4) */
+    float min_distance;           /* qp_min_distance (1.0) */
+    int32_t ignore_net_degree;    /* ignore_net_degree (100): larger nets are left out */
+} sp_qp_config;
+/* Positions in: the design's node_x / node_y (fixed nodes); out: x, y with
+ * num_nodes entries.  net_weight (num_nets) may be NULL (all 1).  rows (may
+ * be NULL) receive iter, hpwl, grad_norm_wl = max(err_x, err_y) and
+ * elapsed_ms per outer iteration, at most max_rows. */
+int sp_qp_place(const sp_hip_design* d, const float* net_weight, const sp_qp_config* cfg,
+                float* x, float* y, sp_hip_iter_metrics* rows, int max_rows,
+                int32_t* iterations);
+/* The B2B system of one axis (0 = x, 1 = y) at the design's positions, as
+ * the GPU assembles it (assemble, quadratic_placer.cpp:43-113): per movable
+ * node i a row of entries (movable node, -w) in row_start[i] ..
+ * row_start[i+1], FP64 diagonal and right-hand side.  Call with diag NULL to
+ * get *nnz only; cap is the capacity of col / val. */
+int sp_qp_b2b_system(const sp_hip_design* d, const float* net_weight, int axis,
+                     float min_distance, int32_t ignore_net_degree, double* diag, double* rhs,
+                     int64_t* row_start, int32_t* col, double* val, int64_t cap, int64_t* nnz);
+
 /* ------------------------------------------------------------------ Fillers
  * Replaces initializeFillers (src/gp/filler.h:29, filler.cpp:47-139) and
  * scatterFillers (filler.h:32, filler.cpp:33-45) for a caller that holds the

@@ -1031,6 +1031,40 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
 // C API
 namespace {
 
+// A sum of n terms f(i) taken in the order of the GPU's block reduction
+// (superplace_amd/csrc/qp.hip: G = min(1024, ceil(n/256)) blocks of 256
+// threads; thread t of block b adds terms b*256 + t + k*G*256 in k order;
+// each 64-lane wave folds lane l + o into lane l for o = 32, 16, .., 1; the
+// block adds its 4 wave sums in order; the blocks' sums are added in block
+// order).  The oracle's quadratic placer sums in this order so that its
+// BiCGSTAB iterates can be compared bit for bit with the GPU's.
+template <typename F>
+double gpuOrderSum(int64_t n, F&& f) {
+    const int64_t G = std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256));
+    double total = 0.0;
+    std::vector<double> lane(256);
+    for (int64_t b = 0; b < G; ++b) {
+        for (int t = 0; t < 256; ++t) {
+            double acc = 0.0;
+            for (int64_t i = b * 256 + t; i < n; i += G * 256) acc += f(i);
+            lane[t] = acc;
+        }
+        double blk = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            double* v = lane.data() + 64 * w;
+            for (int o = 32; o > 0; o >>= 1)
+                for (int l = 0; l < o; ++l) v[l] += v[l + o];
+            blk += v[0];
+        }
+        total += blk;
+    }
+    return total;
+}
+
+}  // namespace
+
+namespace {
+
 // Partners of pin p on axis positions pos (b2b_model.h:56-82 seen from one
 // pin): calls f(q, w) in net order.
 template <typename F>
@@ -1340,16 +1374,13 @@ double ora_bicgstab(int n, const double* diag, const int64_t* rowStart, const in
         }
     };
     auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
-        double acc = 0.0;
-        for (int i = 0; i < n; ++i) acc += a[i] * c[i];
-        return acc;
+        return gpuOrderSum(n, [&](int64_t i) { return a[i] * c[i]; });
     };
     std::vector<double> xv(x, x + n), r(n), r0(n), p(n, 0.0), v(n, 0.0), y(n), s(n), z(n), t(n);
     spmv(xv, r);
     for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
     r0 = r;
-    double bnorm2 = 0.0;
-    for (int i = 0; i < n; ++i) bnorm2 += b[i] * b[i];
+    const double bnorm2 = gpuOrderSum(n, [&](int64_t i) { return b[i] * b[i]; });
     int it = 0;
     double rr = dot(r, r);
     if (bnorm2 == 0.0) {
@@ -1440,7 +1471,22 @@ int ora_qp_place(const ora_design* d, const float* netW, const ora_qp_config* cf
                 sol[i] = v;
             }
         }
-        const double hp = hpwl<float>(d, x, y);
+        // HPWL as computeHPWL (hpwl.cpp:13-39), summed over the nets in the
+        // GPU's reduction order
+        const double hp = gpuOrderSum(d->num_nets, [&](int64_t kk) -> double {
+            const int k = static_cast<int>(kk);
+            const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
+            if (e - b < 2) return 0.0;
+            float lox = std::numeric_limits<float>::max(), hix = -lox, loy = lox, hiy = -lox;
+            for (int t = b; t < e; ++t) {
+                const int p = d->flat_net2pin[t];
+                const float px = x[d->pin2node[p]] + d->pin_offset_x[p];
+                const float py = y[d->pin2node[p]] + d->pin_offset_y[p];
+                lox = std::min(lox, px); hix = std::max(hix, px);
+                loy = std::min(loy, py); hiy = std::max(hiy, py);
+            }
+            return static_cast<double>(hix - lox) + static_cast<double>(hiy - loy);
+        });
         if (iter < maxRows) {
             if (hpwlRows) hpwlRows[iter] = hp;
             if (errRows) errRows[iter] = std::max(err[0], err[1]);

@@ -194,7 +194,8 @@ int64_t ora_b2b_system(const ora_design* d, const float* x, const float* y,
  * preconditioner on the system above, from the initial guess x (in/out),
  * until ||r|| <= tol ||b|| or max_iter iterations; returns ||r|| / ||b||
  * (the solver error QuadraticPlacer records, quadratic_placer.cpp:130-134).
- * *iters receives the iterations run. */
+ * *iters receives the iterations run.  Dot products are summed in the fixed
+ * order of the GPU's block reduction, so iterates compare bit for bit. */
 double ora_bicgstab(int n, const double* diag, const int64_t* row_start, const int32_t* col,
                     const double* val, const double* rhs, double* x, double tol, int max_iter,
                     int* iters);

@@ -51,6 +51,12 @@ class NonFinite(HipError):
     """InvariantError("non_finite", ...) of the reference (src/util/invariant.h:10-33)."""
 
 
+class SpQpConfig(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("tol", C.c_float), ("solver_max_iter", C.c_int32),
+                ("outer_tol", C.c_float), ("min_distance", C.c_float),
+                ("ignore_net_degree", C.c_int32)]
+
+
 class SpFillerInfo(C.Structure):
     _fields_ = [("count", C.c_int32), ("width", C.c_float), ("height", C.c_float),
                 ("whitespace_area", C.c_double), ("node_area_scaled", C.c_double),
@@ -120,7 +126,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -158,6 +164,9 @@ def lib():
     L.sp_bookshelf_read_pl.argtypes = [_P, C.c_char_p]
     L.sp_bookshelf_write_pl.argtypes = [_P, C.c_char_p, _P, _P]
     L.sp_bookshelf_destroy.argtypes = [_P]
+    L.sp_qp_place.argtypes = [_P, _P, _P, _P, _P, _P, C.c_int, _P]
+    L.sp_qp_b2b_system.argtypes = [_P, _P, C.c_int, C.c_float, C.c_int32, _P, _P, _P, _P, _P,
+                                   C.c_int64, _P]
     L.sp_filler_plan.argtypes = [_P, C.c_float, _P]
     L.sp_filler_scatter.argtypes = [_P, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, _P]
     _lib = L

@@ -53,6 +53,9 @@ def test_no_device_paths_report_enodev():
     from superplace_amd import design as D
     with pytest.raises(hip.NoDevice):
         hip.GpEngine(D.fixture_eplace(20, 0))
+    from superplace_amd import qp
+    with pytest.raises(hip.NoDevice):
+        qp.qp_place(D.fixture_eplace(20, 0))
 
 
 def test_invalid_arguments_rejected_before_device():
