@@ -277,6 +277,36 @@ int sp_qp_b2b_system(const sp_hip_design* d, const float* net_weight, int axis,
                      float min_distance, int32_t ignore_net_degree, double* diag, double* rhs,
                      int64_t* row_start, int32_t* col, double* val, int64_t cap, int64_t* nnz);
 
+/* ---------------------------------------------------------------- Metrics
+ * Replaces MetricsSink (src/util/metrics.h:40-90, src/util/metrics.cpp:19-140):
+ * per-iteration rows to <out_dir>/metrics.csv (rewritten whenever a stage
+ * ends) and a summary.json of string keys, number keys and durations_ms,
+ * byte for byte in the reference's formats.  Host code; no GPU is needed.
+ * Stages nest (mGP inside gp): rows go to the innermost open stage.
+ * Non-finite rows or numbers return SP_HIP_ENONFINITE (InvariantError
+ * "non_finite" in the reference); a file that cannot be written returns
+ * SP_HIP_EINVAL (the reference only warns, so a caller may ignore it). */
+typedef struct sp_metrics sp_metrics;
+/* out_dir may be NULL or "" (no metrics.csv); created if missing. */
+int sp_metrics_create(const char* out_dir, sp_metrics** out);
+int sp_metrics_begin_stage(sp_metrics* m, const char* stage);
+/* MetricsSink::push for n rows; max_displacement (n values) may be NULL (0). */
+int sp_metrics_push(sp_metrics* m, const sp_hip_iter_metrics* rows, int n,
+                    const double* max_displacement);
+/* MetricsSink::endStage: records the stage's duration, rewrites metrics.csv. */
+int sp_metrics_end_stage(sp_metrics* m, double elapsed_ms);
+int sp_metrics_record_duration(sp_metrics* m, const char* label, double ms);
+int sp_metrics_set_number(sp_metrics* m, const char* key, double value);
+int sp_metrics_set_string(sp_metrics* m, const char* key, const char* value);
+/* The summary keys EPlace::run writes for a stage (src/gp/eplace.cpp:266-270,
+ * :509-513): <stage>_initial_hpwl, _initial_overflow, _initial_lambda,
+ * _hpwl_reference, _restarts (when restarts >= 0), _evaluations, _converged,
+ * _stagnated, _rollback_iter. */
+int sp_metrics_gp_result(sp_metrics* m, const char* stage, const sp_hip_gp_result* r,
+                         double hpwl_reference, int restarts);
+int sp_metrics_write_summary(const sp_metrics* m, const char* path);
+void sp_metrics_destroy(sp_metrics* m);
+
 /* ------------------------------------------------------------------ Fillers
  * Replaces initializeFillers (src/gp/filler.h:29, filler.cpp:47-139) and
  * scatterFillers (filler.h:32, filler.cpp:33-45) for a caller that holds the

@@ -1,7 +1,8 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
 // src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp,
-// src/gp/filler.cpp, and the header-only src/init/b2b_model.h),
+// src/gp/filler.cpp, src/util/metrics.cpp, and the header-only
+// src/init/b2b_model.h),
 // compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
 //
@@ -21,6 +22,8 @@
 #include "gp/wa_wirelength.h"
 #include "numeric/optimizer.h"
 #include "util/config.h"
+#include "util/invariant.h"
+#include "util/metrics.h"
 
 namespace {
 
@@ -47,6 +50,16 @@ sp::PlaceDB buildDb(int numMovable, int numNodes, int numPins, int numNets, cons
     db.net_name.assign(static_cast<size_t>(numNets), "n");
     db.finalizeCSR(pinNet);
     return db;
+}
+
+template <typename F>
+int refGuard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (...) {
+        return 1;
+    }
 }
 
 }  // namespace
@@ -195,6 +208,56 @@ int ref_bookshelf_write_pl(void* h, const char* path, const float* x, const floa
 }
 
 void ref_bookshelf_free(void* h) { delete static_cast<sp::PlaceDB*>(h); }
+
+// MetricsSink (src/util/metrics.cpp:19-140) behind plain calls; every call
+// returns 0, or 1 when the sink threw.
+void* ref_metrics_create(const char* dir) {
+    auto* m = new sp::MetricsSink();
+    if (dir && *dir) m->setOutputDir(dir);
+    return m;
+}
+int ref_metrics_begin(void* h, const char* stage) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->beginStage(stage); });
+}
+int ref_metrics_push(void* h, int iter, double hpwl, float overflow, float lambda, float gamma,
+                     float step, float gwl, float gden, float maxDisp, double elapsed, float evl,
+                     float evg, double dh, float mu, long evals) {
+    return refGuard([&] {
+        sp::IterMetrics m;
+        m.iter = iter;
+        m.hpwl = hpwl;
+        m.overflow = overflow;
+        m.lambda = lambda;
+        m.gamma = gamma;
+        m.step_size = step;
+        m.grad_norm_wl = gwl;
+        m.grad_norm_den = gden;
+        m.max_displacement = maxDisp;
+        m.elapsed_ms = elapsed;
+        m.eval_lambda = evl;
+        m.eval_gamma = evg;
+        m.delta_hpwl = dh;
+        m.lambda_mu = mu;
+        m.evaluations = evals;
+        static_cast<sp::MetricsSink*>(h)->push(m);
+    });
+}
+int ref_metrics_end(void* h, double ms) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->endStage(ms); });
+}
+int ref_metrics_duration(void* h, const char* label, double ms) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->recordDuration(label, ms); });
+}
+int ref_metrics_number(void* h, const char* key, double v) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->setSummary(key, v); });
+}
+int ref_metrics_string(void* h, const char* key, const char* v) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->setSummary(key, std::string(v)); });
+}
+int ref_metrics_write(void* h, const char* path) {
+    return refGuard([&] { static_cast<sp::MetricsSink*>(h)->writeSummary(path); });
+}
+void ref_metrics_free(void* h) { delete static_cast<sp::MetricsSink*>(h); }
 
 // computeNetBoundPins (src/db/hpwl.cpp:70-102) and forEachB2BPair
 // (src/init/b2b_model.h:56-82) on a database built from arrays: the bound

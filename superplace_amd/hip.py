@@ -126,7 +126,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -167,6 +167,16 @@ def lib():
     L.sp_qp_place.argtypes = [_P, _P, _P, _P, _P, _P, C.c_int, _P]
     L.sp_qp_b2b_system.argtypes = [_P, _P, C.c_int, C.c_float, C.c_int32, _P, _P, _P, _P, _P,
                                    C.c_int64, _P]
+    L.sp_metrics_create.argtypes = [C.c_char_p, C.POINTER(_P)]
+    L.sp_metrics_begin_stage.argtypes = [_P, C.c_char_p]
+    L.sp_metrics_push.argtypes = [_P, _P, C.c_int, _P]
+    L.sp_metrics_end_stage.argtypes = [_P, C.c_double]
+    L.sp_metrics_record_duration.argtypes = [_P, C.c_char_p, C.c_double]
+    L.sp_metrics_set_number.argtypes = [_P, C.c_char_p, C.c_double]
+    L.sp_metrics_set_string.argtypes = [_P, C.c_char_p, C.c_char_p]
+    L.sp_metrics_gp_result.argtypes = [_P, C.c_char_p, _P, C.c_double, C.c_int]
+    L.sp_metrics_write_summary.argtypes = [_P, C.c_char_p]
+    L.sp_metrics_destroy.argtypes = [_P]
     L.sp_filler_plan.argtypes = [_P, C.c_float, _P]
     L.sp_filler_scatter.argtypes = [_P, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, _P]
     _lib = L
@@ -210,8 +220,10 @@ class BackendConfig:
 
 
 class HipPoissonBackend:
-    """PoissonBackend on the matrix cores: solve() reads density and writes
-    phi / field_x / field_y (the field xi = -grad phi, not a gradient)."""
+    """PoissonBackend on the GPU (LDS FFT passes, or dense transforms on the
+    matrix cores for grids that are not a power of two): solve() reads
+    density and writes phi / field_x / field_y (the field xi = -grad phi,
+    not a gradient)."""
 
     def __init__(self, cfg: BackendConfig):
         self.nx, self.ny = int(cfg.nx), int(cfg.ny)

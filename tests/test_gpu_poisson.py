@@ -148,3 +148,16 @@ def test_gemm_matches_numpy(fp64, mnk):
         err = np.abs(c.get().astype(np.float64) - ref).max() / np.abs(ref).max()
         # FP32: k products accumulated in FP32, about sqrt(k) * 2^-24 relative
         assert err < (1e-13 if fp64 else 1e-5), err
+
+
+def test_poisson_bench_tool():
+    """superplace-poisson-bench's JSON (tools/poisson_bench/main.cpp) for the
+    HIP backend; the checksum (sum of phi) matches the oracle's."""
+    from superplace_amd import poisson_bench
+    r = poisson_bench.run(64, 4, 5, 1)
+    for k in ("grid", "fftw_threads", "fftw_plan_threads", "omp_threads", "iterations",
+              "plan_ms", "solve_ms", "checksum"):
+        assert k in r
+    assert r["grid"] == 64 and r["iterations"] == 5 and r["fftw_plan_threads"] == 0
+    phi, _, _ = oracle.poisson(poisson_bench.density(64).astype(np.float64), 1.25, 0.75)
+    assert abs(r["checksum"] - phi.sum()) <= 1e-4 * np.abs(phi).sum()
