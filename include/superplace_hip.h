@@ -277,6 +277,29 @@ int sp_qp_b2b_system(const sp_hip_design* d, const float* net_weight, int axis,
                      float min_distance, int32_t ignore_net_degree, double* diag, double* rhs,
                      int64_t* row_start, int32_t* col, double* val, int64_t cap, int64_t* nnz);
 
+/* -------------------------------------------------------------- Legalizer
+ * Replaces legalizeCells and buildAbacusSegments (src/lg/abacus.h:34-35,
+ * src/lg/abacus.cpp:108-286): Abacus standard-cell legalization over the
+ * site rows, bit for bit the reference's result.  Host code; no GPU is
+ * needed (the algorithm is one sequential greedy pass).  row_step holds the
+ * site width of every row (sp_bookshelf_extra gives it); the rows' hx is
+ * lx + step * sites.  x, y: the centres of all num_nodes nodes, in and out
+ * (only standard cells move).  Invalid rows or non-finite geometry return
+ * SP_HIP_EINVAL (std::invalid_argument there). */
+enum { SP_ABACUS_FULL_TRIALS = 1,   /* AbacusOptions::referenceTrials */
+       SP_ABACUS_AREA_WEIGHT = 2 }; /* AbacusOptions::areaWeight */
+typedef struct sp_abacus_result {   /* AbacusResult, src/lg/abacus.h:21-32 */
+    int32_t precondition_ok, cells, failed, rows, segments, rounded_widths;
+    int32_t selfcheck_overlaps, selfcheck_offsite;
+    double disp_mean, disp_max;
+} sp_abacus_result;
+int sp_abacus_legalize(const sp_hip_design* d, const float* row_step, float* x, float* y,
+                       int flags, sp_abacus_result* result);
+/* The free row segments at the design's positions; call with seg_row NULL
+ * to get *count only. */
+int sp_abacus_segments(const sp_hip_design* d, const float* row_step, int32_t* seg_row,
+                       float* seg_lx, float* seg_hx, int32_t cap, int32_t* count);
+
 /* ---------------------------------------------------------------- Metrics
  * Replaces MetricsSink (src/util/metrics.h:40-90, src/util/metrics.cpp:19-140):
  * per-iteration rows to <out_dir>/metrics.csv (rewritten whenever a stage

@@ -1,7 +1,7 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
 // src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp,
-// src/gp/filler.cpp, src/util/metrics.cpp, and the header-only
+// src/gp/filler.cpp, src/util/metrics.cpp, src/lg/abacus.cpp, and the header-only
 // src/init/b2b_model.h),
 // compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
@@ -17,6 +17,7 @@
 #include "db/place_db.h"
 #include "gp/filler.h"
 #include "init/b2b_model.h"
+#include "lg/abacus.h"
 #include "io/bookshelf_reader.h"
 #include "io/bookshelf_writer.h"
 #include "gp/wa_wirelength.h"
@@ -258,6 +259,47 @@ int ref_metrics_write(void* h, const char* path) {
     return refGuard([&] { static_cast<sp::MetricsSink*>(h)->writeSummary(path); });
 }
 void ref_metrics_free(void* h) { delete static_cast<sp::MetricsSink*>(h); }
+
+// legalizeCells (src/lg/abacus.cpp:172-284) on a database of nodes (no pins)
+// and site rows {ly, height, lx, step, numSites}, its regions from
+// PlaceDB::computeRegions.  res: preconditionOk, cells, failed, rows,
+// segments, roundedWidths, selfcheckOverlaps, selfcheckOffSite, dispMean,
+// dispMax; x, y (centres) are updated.  Returns 0, or 1 when it threw.
+int ref_abacus(int numMovable, int numNodes, float* x, float* y, const float* w, const float* h,
+               const uint8_t* flags, int numRows, const float* rowLy, const float* rowH,
+               const float* rowLx, const float* rowStep, const int32_t* rowSites, int options,
+               double* res) {
+    sp::PlaceDB db;
+    for (int i = 0; i < numNodes; ++i) {
+        db.addNode("n" + std::to_string(i), w[i], h[i], flags[i]);
+        db.node_x[i] = x[i];
+        db.node_y[i] = y[i];
+    }
+    db.numMovable = numMovable;
+    db.numNodes = numNodes;
+    db.finalizeCSR({});
+    for (int r = 0; r < numRows; ++r)
+        db.rows.push_back(sp::PlaceDB::SiteRow{rowLy[r], rowH[r], rowLx[r], rowStep[r], rowSites[r]});
+    db.computeRegions();
+    sp::AbacusOptions opt;
+    opt.referenceTrials = (options & 1) != 0;
+    opt.areaWeight = (options & 2) != 0;
+    try {
+        const sp::AbacusResult r = sp::legalizeCells(db, opt);
+        const double v[10] = {double(r.preconditionOk), double(r.cells), double(r.failed),
+                              double(r.rows), double(r.segments), double(r.roundedWidths),
+                              double(r.selfcheckOverlaps), double(r.selfcheckOffSite), r.dispMean,
+                              r.dispMax};
+        for (int k = 0; k < 10; ++k) res[k] = v[k];
+    } catch (...) {
+        return 1;
+    }
+    for (int i = 0; i < numNodes; ++i) {
+        x[i] = db.node_x[i];
+        y[i] = db.node_y[i];
+    }
+    return 0;
+}
 
 // computeNetBoundPins (src/db/hpwl.cpp:70-102) and forEachB2BPair
 // (src/init/b2b_model.h:56-82) on a database built from arrays: the bound

@@ -51,6 +51,13 @@ class NonFinite(HipError):
     """InvariantError("non_finite", ...) of the reference (src/util/invariant.h:10-33)."""
 
 
+class SpAbacusResult(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("precondition_ok", "cells", "failed", "rows", "segments",
+                                         "rounded_widths", "selfcheck_overlaps",
+                                         "selfcheck_offsite")] + \
+        [("disp_mean", C.c_double), ("disp_max", C.c_double)]
+
+
 class SpQpConfig(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("tol", C.c_float), ("solver_max_iter", C.c_int32),
                 ("outer_tol", C.c_float), ("min_distance", C.c_float),
@@ -126,7 +133,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics|abacus)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -167,6 +174,8 @@ def lib():
     L.sp_qp_place.argtypes = [_P, _P, _P, _P, _P, _P, C.c_int, _P]
     L.sp_qp_b2b_system.argtypes = [_P, _P, C.c_int, C.c_float, C.c_int32, _P, _P, _P, _P, _P,
                                    C.c_int64, _P]
+    L.sp_abacus_legalize.argtypes = [_P, _P, _P, _P, C.c_int, _P]
+    L.sp_abacus_segments.argtypes = [_P, _P, _P, _P, _P, C.c_int32, _P]
     L.sp_metrics_create.argtypes = [C.c_char_p, C.POINTER(_P)]
     L.sp_metrics_begin_stage.argtypes = [_P, C.c_char_p]
     L.sp_metrics_push.argtypes = [_P, _P, C.c_int, _P]
