@@ -300,6 +300,41 @@ int sp_abacus_legalize(const sp_hip_design* d, const float* row_step, float* x, 
 int sp_abacus_segments(const sp_hip_design* d, const float* row_step, int32_t* seg_row,
                        float* seg_lx, float* seg_hx, int32_t cap, int32_t* count);
 
+/* Replaces legalizeMacros (src/lg/macro_sa.h:40, src/lg/macro_sa.cpp:493-1272):
+ * simulated annealing over the movable macros (flag macro, not fixed, below
+ * num_movable) followed by greedy repairs, bit for bit the reference's
+ * result on the same design and seed.  Host code; no GPU is needed (one
+ * sequential Markov chain).  site_width is the first row's site step,
+ * row_height the design's row height (<= 0 means 1).  In/out: x, y (centres
+ * of all num_nodes nodes; only macros move), node_flags (macros gain flag
+ * 2, fixed), pin_offset_x (negated on the pins of horizontally flipped
+ * macros), orient (BookShelf orientation codes, N<->FN and S<->FS on flip;
+ * may be NULL).  net_weight may be NULL (all 1).  sink (may be NULL)
+ * receives the summaries mlg_<step>_{overlap, disp_mean, disp_max,
+ * worst_macro, original_x, original_y, x, y} for the steps sa, separate1,
+ * chain1, local_grid, separate2, chain2, unlimited_grid (when run),
+ * postlegal and final, mlg_postlegal_accepted, and the mLG duration;
+ * worst_macro is node_names[node] (the node index when node_names is
+ * NULL).  A spatial index that disagrees with a full scan (checked up to
+ * 8000 macros) returns SP_HIP_EINVAL (InvariantError there). */
+typedef struct sp_mlg_config {   /* the macro_sa_* fields of Config, src/util/config.h */
+    uint64_t random_seed;        /* Config::random_seed (1002) */
+    int32_t moves;               /* macro_sa_moves; 0 = max(20000, 300 x overlapping) */
+    float init_accept;           /* macro_sa_init_accept (0.9) */
+    float temp_ratio;            /* macro_sa_temp_ratio (1e-4) */
+    float disp_weight;           /* macro_sa_disp_weight (1.0) */
+} sp_mlg_config;
+typedef struct sp_mlg_result {   /* MacroSaResult, src/lg/macro_sa.h:20-35 */
+    int32_t num_macros, moves, accepted, repaired, chained, hot_macros;
+    double overlap_before, overlap_after_sa, overlap_after, sa_ms;
+    double t0, typical_worsen, hpwl_before, hpwl_after;
+} sp_mlg_result;
+int sp_macro_legalize(const sp_hip_design* d, const float* net_weight, float site_width,
+                      float row_height, const sp_mlg_config* cfg, float* x, float* y,
+                      uint8_t* node_flags, float* pin_offset_x, uint8_t* orient,
+                      struct sp_metrics* sink, const char* const* node_names,
+                      sp_mlg_result* result);
+
 /* ---------------------------------------------------------------- Metrics
  * Replaces MetricsSink (src/util/metrics.h:40-90, src/util/metrics.cpp:19-140):
  * per-iteration rows to <out_dir>/metrics.csv (rewritten whenever a stage

@@ -1,8 +1,8 @@
 // C-ABI harness around the reference's own self-contained sources
 // (src/gp/wa_wirelength.cpp, src/db/hpwl.cpp, src/numeric/cpu/nesterov.cpp,
 // src/db/place_db.cpp, src/io/bookshelf_reader.cpp, src/io/bookshelf_writer.cpp,
-// src/gp/filler.cpp, src/util/metrics.cpp, src/lg/abacus.cpp, and the header-only
-// src/init/b2b_model.h),
+// src/gp/filler.cpp, src/util/metrics.cpp, src/lg/abacus.cpp, src/lg/macro_sa.cpp,
+// and the header-only src/init/b2b_model.h),
 // compiled from where they lie by oracle/Makefile into oracle/_ref/libsp_ref.so.  TEST INFRASTRUCTURE ONLY: used by tests to pin
 // the oracle's restatement against the reference's real code.
 //
@@ -18,12 +18,14 @@
 #include "gp/filler.h"
 #include "init/b2b_model.h"
 #include "lg/abacus.h"
+#include "lg/macro_sa.h"
 #include "io/bookshelf_reader.h"
 #include "io/bookshelf_writer.h"
 #include "gp/wa_wirelength.h"
 #include "numeric/optimizer.h"
 #include "util/config.h"
 #include "util/invariant.h"
+#include "util/logger.h"
 #include "util/metrics.h"
 
 namespace {
@@ -377,6 +379,54 @@ void ref_fillers_rescatter(void* h, uint64_t seed, uint64_t offset, float* x, fl
         x[k] = db->node_x[db->numNodes + k];
         y[k] = db->node_y[db->numNodes + k];
     }
+}
+
+// legalizeMacros (src/lg/macro_sa.cpp:493-1272) on a database built from
+// arrays plus site rows {ly, height, lx, step, numSites} (coreRegion and
+// rowHeight from PlaceDB::computeRegions), node names "n<i>".  cfgf:
+// init_accept, temp_ratio, disp_weight.  sink: a ref_metrics_create handle
+// or NULL.  x, y, offx, flags, orient are updated; res: numMacros, moves,
+// accepted, repaired, chained, hotMacros, overlapBefore, overlapAfterSa,
+// overlapAfter, saMs, t0, typicalWorsen, hpwlBefore, hpwlAfter.  Returns 0,
+// or 1 when it threw.
+int ref_macro_sa(int numMovable, int numNodes, int numPins, int numNets, float* x, float* y,
+                 const float* w, const float* h, uint8_t* flags, const int32_t* pin2node,
+                 const int32_t* pin2net, float* offx, const float* offy, const float* netWeight,
+                 uint8_t* orient, int numRows, const float* rowLy, const float* rowH,
+                 const float* rowLx, const float* rowStep, const int32_t* rowSites,
+                 uint64_t seed, int moves, const float* cfgf, void* sink, double* res) {
+    sp::PlaceDB db = buildDb(numMovable, numNodes, numPins, numNets, x, y, w, h, flags, pin2node,
+                             pin2net, offx, offy);
+    if (netWeight) db.net_weight.assign(netWeight, netWeight + numNets);
+    if (orient) db.node_orient.assign(orient, orient + numNodes);
+    for (int r = 0; r < numRows; ++r)
+        db.rows.push_back(sp::PlaceDB::SiteRow{rowLy[r], rowH[r], rowLx[r], rowStep[r], rowSites[r]});
+    db.computeRegions();
+    sp::Config cfg;
+    cfg.random_seed = seed;
+    cfg.macro_sa_moves = moves;
+    cfg.macro_sa_init_accept = cfgf[0];
+    cfg.macro_sa_temp_ratio = cfgf[1];
+    cfg.macro_sa_disp_weight = cfgf[2];
+    sp::Logger::get().setLevel(sp::LogLevel::kError);
+    sp::MetricsSink quiet;
+    sp::MetricsSink& out = sink ? *static_cast<sp::MetricsSink*>(sink) : quiet;
+    sp::MacroSaResult r;
+    if (refGuard([&] { r = sp::legalizeMacros(db, cfg, out); })) return 1;
+    const double v[14] = {double(r.numMacros), double(r.moves),  double(r.accepted),
+                          double(r.repaired),  double(r.chained), double(r.hotMacros),
+                          r.overlapBefore,     r.overlapAfterSa,  r.overlapAfter,
+                          r.saMs,              r.t0,              r.typicalWorsen,
+                          r.hpwlBefore,        r.hpwlAfter};
+    for (int k = 0; k < 14; ++k) res[k] = v[k];
+    for (int i = 0; i < numNodes; ++i) {
+        x[i] = db.node_x[i];
+        y[i] = db.node_y[i];
+        flags[i] = db.node_flags[i];
+        if (orient) orient[i] = db.node_orient[i];
+    }
+    for (int p = 0; p < numPins; ++p) offx[p] = db.pin_offset_x[p];
+    return 0;
 }
 
 }  // extern "C"

@@ -58,6 +58,18 @@ class SpAbacusResult(C.Structure):
         [("disp_mean", C.c_double), ("disp_max", C.c_double)]
 
 
+class SpMlgConfig(C.Structure):
+    _fields_ = [("random_seed", C.c_uint64), ("moves", C.c_int32), ("init_accept", C.c_float),
+                ("temp_ratio", C.c_float), ("disp_weight", C.c_float)]
+
+
+class SpMlgResult(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("num_macros", "moves", "accepted", "repaired",
+                                         "chained", "hot_macros")] + \
+        [(k, C.c_double) for k in ("overlap_before", "overlap_after_sa", "overlap_after", "sa_ms",
+                                   "t0", "typical_worsen", "hpwl_before", "hpwl_after")]
+
+
 class SpQpConfig(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("tol", C.c_float), ("solver_max_iter", C.c_int32),
                 ("outer_tol", C.c_float), ("min_distance", C.c_float),
@@ -133,7 +145,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics|abacus)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics|abacus|macro)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -176,6 +188,8 @@ def lib():
                                    C.c_int64, _P]
     L.sp_abacus_legalize.argtypes = [_P, _P, _P, _P, C.c_int, _P]
     L.sp_abacus_segments.argtypes = [_P, _P, _P, _P, _P, C.c_int32, _P]
+    L.sp_macro_legalize.argtypes = [_P, _P, C.c_float, C.c_float, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _P]
     L.sp_metrics_create.argtypes = [C.c_char_p, C.POINTER(_P)]
     L.sp_metrics_begin_stage.argtypes = [_P, C.c_char_p]
     L.sp_metrics_push.argtypes = [_P, _P, C.c_int, _P]
