@@ -13,7 +13,14 @@ events on the engine's stream over the timed region, and a CPU baseline (the
 oracle's FP32 restatement of the reference CPU path, OpenMP on this host's
 cores) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+Many instances: a batch of 64 uneven synthetic designs (5k-80k cells, bins
+128-512) advanced together by the library's scheduler against the same
+designs run one after another, and a serving stream (Poisson arrivals,
+10k-640k cells, bins 256-1024, mGP to overflow 0.10) with its p50/p99
+time-to-converge and jobs/h, against the same arrivals served one at a time
+on the FP64 path.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-serve]
 """
 from __future__ import annotations
 
@@ -54,6 +61,103 @@ def timed_run(design, prec, steps, warmup, profile_mask=0, max_iter=None):
     eng.close()
     return dict(iters=k, ms=t_ms, iters_per_s=1000.0 * k / t_ms, evals=evals, setup_s=setup,
                 result=res, last=rows[-1], stats=stats)
+
+
+def auto_bin_dim(d, lo, hi):
+    """The reference's automatic bin dimension (bin_grid.cpp:172-189: the
+    largest power of two whose bins hold one average movable node at target
+    density 1), clamped to [lo, hi]."""
+    import numpy as np
+    mov = np.arange(d.num_movable)
+    keep = (d.node_flags[mov] & (2 | 8)) == 0
+    area = float(np.sum(d.node_w[mov][keep].astype(np.float64) * d.node_h[mov][keep]))
+    ideal = (d.core[2] - d.core[0]) * (d.core[3] - d.core[1]) / max(area / max(d.num_movable, 1), 1e-9)
+    dim = 4
+    while dim * 2 <= 1024 and (dim * 2) ** 2 <= ideal:
+        dim *= 2
+    return min(max(dim, lo), hi)
+
+
+def job_designs(count, lo, hi, seed0, dim_lo, dim_hi):
+    """Seeded synthetic designs, cell counts log-uniform in [lo, hi] (drawn
+    from the splitmix64 stream of seed 1002), config-1 degree distribution,
+    1.05 nets per cell, 57% utilisation at adaptec1's cell density."""
+    import numpy as np
+    from superplace_amd import design as D
+    u = D.Rng(1002 + seed0).uniform(count)
+    out = []
+    for k in range(count):
+        cells = int(round(lo * (hi / lo) ** float(u[k])))
+        d = D.synth_design(cells, max(8, cells // 400), int(cells * 1.05), 25.25 * np.sqrt(cells),
+                           0.57, seed=seed0 + k + 1, name=f"job{k}")
+        out.append((d, auto_bin_dim(d, dim_lo, dim_hi)))
+    return out
+
+
+def batch_bench(count):
+    """Uneven instances advanced together by the scheduler, against the same
+    instances run one after another."""
+    from superplace_amd import hip, serve
+    jobs = job_designs(count, 5000, 80000, 0, 128, 512)
+    cfgs = [hip.GpConfig(gp_bin_dim=dim) for _, dim in jobs]
+    t0 = time.time()
+    iters_seq = 0
+    for (d, _), cfg in zip(jobs, cfgs):
+        eng = hip.GpEngine(d, cfg)
+        res, _ = eng.run(max_rows=1)
+        eng.positions()
+        eng.close()
+        iters_seq += res["iterations"]
+    seq_s = time.time() - t0
+    srv = serve.Server(hbm_budget=200 << 30, max_active=16)
+    for (d, _), cfg in zip(jobs, cfgs):
+        srv.submit(d, cfg, 0.0)
+    t0 = time.time()
+    srv.drain(timeout_s=600)
+    batch_s = time.time() - t0
+    infos = [srv.job(i, positions=False) for i in range(count)]
+    srv.close()
+    iters = sum(i["result"]["iterations"] for i in infos)
+    if iters != iters_seq or any(i["state"] != serve.DONE for i in infos):
+        raise RuntimeError("batched jobs differ from the same jobs run alone")
+    cells = [d.num_movable for d, _ in jobs]
+    return dict(jobs=count, cells_min=min(cells), cells_max=max(cells), cells_total=sum(cells),
+                iterations=iters, batch_s=batch_s, serial_s=seq_s,
+                batch_iters_per_s=iters / batch_s, serial_iters_per_s=iters / seq_s,
+                speedup=seq_s / batch_s)
+
+
+def serve_bench(count, rate):
+    """Poisson arrivals at `rate` jobs/s served together (FP32), against the
+    same arrivals served one at a time on the FP64 path; latency = finish -
+    arrival (time to overflow 0.10, queueing included)."""
+    import numpy as np
+    from superplace_amd import design as D
+    from superplace_amd import hip, serve
+    jobs = job_designs(count, 10000, 640000, 100, 256, 1024)
+    gaps = -np.log(1.0 - D.Rng(1002).uniform(count)) / rate
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def run(prec, slots):
+        srv = serve.Server(hbm_budget=200 << 30, max_active=slots)
+        for (d, dim), t in zip(jobs, arrivals):
+            srv.submit(d, hip.GpConfig(gp_bin_dim=dim, precision=prec), float(t))
+        srv.drain(timeout_s=900)
+        infos = [srv.job(i, positions=False) for i in range(count)]
+        srv.close()
+        if any(i["state"] != serve.DONE for i in infos):
+            raise RuntimeError("a served job failed")
+        lat = np.array([i["finish_s"] - i["arrival_s"] for i in infos])
+        span = max(i["finish_s"] for i in infos) - float(arrivals[0])
+        return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                    jobs_per_hour=3600.0 * count / span,
+                    converged=sum(i["result"]["converged"] for i in infos))
+
+    mix = run(32, 16)
+    serial = run(64, 1)
+    cells = [d.num_movable for d, _ in jobs]
+    return dict(jobs=count, arrival_rate_per_s=rate, cells_min=min(cells), cells_max=max(cells),
+                pins_max=max(d.num_pins for d, _ in jobs), served_fp32=mix, serial_fp64=serial)
 
 
 def measured_traffic(family):
@@ -98,6 +202,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=200)
+    ap.add_argument("--no-serve", action="store_true", help="skip the batch and serving runs")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -156,7 +264,20 @@ def main():
         log(f"FP64: {r64['iters_per_s']:.1f} iters/s; tau {r64['last']['overflow']:.4f} "
             f"HPWL {r64['last']['hpwl']:.6g}")
 
-    # 4) CPU baseline (rank 0, N=1 only), bounded sample
+    # 4) many instances: a batch advanced together, and a serving stream
+    batch = served = None
+    if not args.no_serve:
+        batch = batch_bench(args.batch_jobs)
+        log(f"batch: {batch['jobs']} jobs ({batch['cells_min']}-{batch['cells_max']} cells), "
+            f"{batch['iterations']} iterations in {batch['batch_s']:.2f} s together vs "
+            f"{batch['serial_s']:.2f} s one at a time ({batch['speedup']:.2f}x)")
+        served = serve_bench(args.serve_jobs, args.serve_rate)
+        m, f = served["served_fp32"], served["serial_fp64"]
+        log(f"serving: {served['jobs']} jobs at {args.serve_rate}/s: p50 {m['p50_s']:.3f} s, "
+            f"p99 {m['p99_s']:.3f} s, {m['jobs_per_hour']:.0f} jobs/h; one at a time FP64: "
+            f"p50 {f['p50_s']:.3f} s, p99 {f['p99_s']:.3f} s, {f['jobs_per_hour']:.0f} jobs/h")
+
+    # 5) CPU baseline (rank 0, N=1 only), bounded sample
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_iters} oracle FP32 iterations on {os.cpu_count()} cores ...")
@@ -186,6 +307,8 @@ def main():
         "fp32_over_fp64": (r32["iters_per_s"] / r64["iters_per_s"]) if r64 else None,
         "evals_per_iter": r32["evals"] / r32["iters"],
         "kernel_us_per_launch": {f: round(v, 2) for f, v in fam_us.items()},
+        "batch": batch,
+        "serving": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -178,6 +178,7 @@ typedef struct sp_hip_gp_info {
     int32_t precision;
     double step_x, step_y, origin_x, origin_y;
     double movable_area_scaled;   /* tau denominator (bin_grid.cpp:182-188) */
+    int64_t device_bytes;         /* device memory the instance allocated so far */
 } sp_hip_gp_info;
 
 /* Kernel families timed with HIP events on the engine's stream in the
@@ -276,6 +277,49 @@ int sp_qp_place(const sp_hip_design* d, const float* net_weight, const sp_qp_con
 int sp_qp_b2b_system(const sp_hip_design* d, const float* net_weight, int axis,
                      float min_distance, int32_t ignore_net_degree, double* diag, double* rhs,
                      int64_t* row_start, int32_t* col, double* val, int64_t cap, int64_t* nnz);
+
+/* ---------------------------------------------------------------- Serving
+ * Many placement jobs on one GPU.  No counterpart in the reference (it
+ * places one design per process): a scheduler for callers that place many
+ * designs.  Each job is one sp_hip_gp-style instance (one stage: the
+ * config's stage, from the design's positions) with its design and
+ * optimizer state resident on the device; running jobs advance together,
+ * each on its own stream with its device-driven loop (loop_mode 1 or 2)
+ * kept two launches deep, so their kernels overlap.  A job is admitted, in
+ * arrival order, once its arrival time has passed, fewer than max_active
+ * jobs run and its footprint fits in what the budget has left; the host
+ * set-up of the next job overlaps the running ones.  Every job's result
+ * equals that of sp_hip_gp_run on the same design and configuration, bit
+ * for bit.  One host thread drives a server; times are seconds on the
+ * server's clock (0 at create). */
+enum { SP_HIP_JOB_QUEUED = 0, SP_HIP_JOB_RUNNING = 1, SP_HIP_JOB_DONE = 2, SP_HIP_JOB_FAILED = 3 };
+typedef struct sp_hip_job_info {
+    int32_t state;          /* SP_HIP_JOB_* */
+    int32_t status;         /* sp_hip_status of a failed job (SP_HIP_EOOM: footprint above the budget) */
+    int64_t device_bytes;   /* footprint: the estimate until set up, then the measured bytes */
+    double arrival_s, admit_s, finish_s;   /* -1 until reached */
+} sp_hip_job_info;
+typedef struct sp_hip_server sp_hip_server;
+/* Device bytes an instance of this design and configuration allocates (an
+ * upper bound of what it measures once set up). */
+int sp_hip_gp_footprint(const sp_hip_design* d, const sp_hip_gp_config* cfg, int64_t* bytes);
+int sp_hip_server_create(int64_t hbm_budget, int32_t max_active, sp_hip_server** out);
+/* Copies the design; the job becomes eligible at arrival_s. */
+int sp_hip_server_submit(sp_hip_server* s, const sp_hip_design* d, const sp_hip_gp_config* cfg,
+                         double arrival_s, int32_t* job);
+/* One scheduling round (never waits for the device); any pointer may be NULL. */
+int sp_hip_server_pump(sp_hip_server* s, double* now_s, int32_t* active, int32_t* queued,
+                       int32_t* finished);
+/* Pumps until every submitted job is done or failed (SP_HIP_EINVAL after timeout_s). */
+int sp_hip_server_drain(sp_hip_server* s, double timeout_s);
+int sp_hip_server_now(const sp_hip_server* s, double* now_s);
+/* A job's state, result and (once done) the positions of all its nodes;
+ * res, x, y may be NULL. */
+int sp_hip_server_job(const sp_hip_server* s, int32_t job, sp_hip_job_info* info,
+                      sp_hip_gp_result* res, float* x, float* y);
+/* Frees a finished job's host copy of its design and positions. */
+int sp_hip_server_release(sp_hip_server* s, int32_t job);
+void sp_hip_server_destroy(sp_hip_server* s);
 
 /* -------------------------------------------------------------- Legalizer
  * Replaces legalizeCells and buildAbacusSegments (src/lg/abacus.h:34-35,

@@ -18,7 +18,8 @@ INCLUDE = HERE.parent / "include"
 LIB = HERE / "libsuperplace_hip.so"
 OBJ_DIR = HERE / "_build"
 SOURCES = ["poisson.hip", "gp_kernels.hip", "engine.hip", "qp.hip", "bookshelf.cpp", "setup.cpp",
-           "metrics.cpp", "legalize.cpp", "macro_lg.cpp"]
+           "metrics.cpp", "legalize.cpp", "macro_lg.cpp",
+           "serve.cpp"]
 ARCH = os.environ.get("SP_HIP_ARCH", "gfx950")
 
 

@@ -23,6 +23,10 @@ struct HipError : std::runtime_error {
 
 constexpr int kBlock = 256;   // threads per block for the elementwise/gather kernels
 
+// Device bytes allocated by this thread through the library's allocation
+// helpers: an instance's footprint is the difference across its set-up.
+inline thread_local size_t g_devBytes = 0;
+
 inline int ceilDiv(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 inline int roundUp(int a, int b) { return (a + b - 1) / b * b; }
 

@@ -1,0 +1,47 @@
+// The placement instance behind sp_hip_gp, as seen by the other host code
+// of the library (the scheduler of concurrent instances).
+#pragma once
+
+#include <cstddef>
+#include <memory>
+
+#include "../../include/superplace_hip.h"
+
+namespace sphip {
+
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual void info(sp_hip_gp_info* out) const = 0;
+    virtual void optIndex(int32_t* out) const = 0;
+    virtual void evalHost(const float* x, const float* y, float lambda, float gamma,
+                          sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) = 0;
+    virtual void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) = 0;
+    virtual void positions(float* x, float* y) const = 0;
+    virtual void setProfiling(int mask) = 0;
+    virtual void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const = 0;
+    // The stage loop in three parts, for a caller that interleaves several
+    // instances (device-driven loop modes only): begin() runs the host part
+    // before the loop and returns true when the stage ended there; then
+    // launchAsync() enqueues one launch of the loop (at most kAsyncDepth in
+    // flight) and pollAsync() reports the oldest one (-1 still running, 0
+    // finished with the stage still going, 1 stage done); finishAsync()
+    // completes the result once the stage is done.
+    virtual bool begin(sp_hip_gp_result* res) = 0;
+    virtual void launchAsync() = 0;
+    virtual int pollAsync(bool wait) = 0;
+    virtual int inFlight() const = 0;
+    virtual void finishAsync(sp_hip_gp_result* res) = 0;
+    virtual size_t deviceBytes() const = 0;
+};
+
+constexpr int kAsyncDepth = 2;   // loop launches in flight per instance
+
+// A new instance on its own stream (precision from cfg.precision).
+std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg);
+
+// Device bytes an instance of this design and configuration allocates, an
+// upper bound of EngineBase::deviceBytes() once its stage loop has run.
+size_t footprintEstimate(const sp_hip_design& d, const sp_hip_gp_config& cfg);
+
+}  // namespace sphip

@@ -19,6 +19,7 @@
 
 #include "../../include/superplace_hip.h"
 #include "abi.h"
+#include "engine.h"
 #include "gp_kernels.h"
 #include "poisson.h"
 
@@ -29,6 +30,7 @@ template <typename T>
 T* dalloc(size_t n) {
     void* p = nullptr;
     SPHIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    g_devBytes += std::max<size_t>(n, 1) * sizeof(T);
     return static_cast<T*>(p);
 }
 
@@ -64,19 +66,38 @@ inline T overlapT(T alx, T aly, T ahx, T ahy, T blx, T bly, T bhx, T bhy) {
     return (dx > T(0) && dy > T(0)) ? dx * dy : T(0);
 }
 
+// Movable cell and macro areas (fixed and terminal_NI nodes excluded).
+void movableAreas(const sp_hip_design& d, double* cellArea, double* macroArea) {
+    double c = 0, m = 0;
+    for (int i = 0; i < d.num_movable; ++i) {
+        if (d.node_flags[i] & (F_FIXED | F_NI)) continue;
+        const double a = static_cast<double>(d.node_w[i]) * d.node_h[i];
+        if (d.node_flags[i] & F_MACRO) m += a;
+        else c += a;
+    }
+    *cellArea = c;
+    *macroArea = m;
+}
+
+// Bins per side: gp_bin_dim, or the largest power of two whose bins hold
+// at least one average movable node at the target density
+// (BinGrid::initialize, bin_grid.cpp:172-189).
+int binDim(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
+    constexpr int kMaxBinDim = 1024;
+    if (cfg.gp_bin_dim > 0) return std::min(cfg.gp_bin_dim, kMaxBinDim);
+    double cellArea = 0, macroArea = 0;
+    movableAreas(d, &cellArea, &macroArea);
+    const double avgArea = d.num_movable > 0 ? (cellArea + macroArea) / d.num_movable : 1.0;
+    const double idealBin = std::max(avgArea / cfg.target_density, 1e-9);
+    const double coreArea = static_cast<double>(d.core_hx - d.core_lx) *
+                            static_cast<double>(d.core_hy - d.core_ly);
+    const double ideal = coreArea / idealBin;
+    int dim = 4;
+    while (dim * 2 <= kMaxBinDim && static_cast<double>(dim * 2) * (dim * 2) <= ideal) dim *= 2;
+    return dim;
+}
+
 // ===========================================================================
-class EngineBase {
-public:
-    virtual ~EngineBase() = default;
-    virtual void info(sp_hip_gp_info* out) const = 0;
-    virtual void optIndex(int32_t* out) const = 0;
-    virtual void evalHost(const float* x, const float* y, float lambda, float gamma,
-                          sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) = 0;
-    virtual void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) = 0;
-    virtual void positions(float* x, float* y) const = 0;
-    virtual void setProfiling(int mask) = 0;
-    virtual void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const = 0;
-};
 
 template <typename T>
 class Engine final : public EngineBase {
@@ -85,12 +106,14 @@ public:
         requireDevice();
         validateDesign(d);
         SPHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        const size_t before = g_devBytes;
         try {
             build(d);
         } catch (...) {
             release();
             throw;
         }
+        devBytes_ = g_devBytes - before;
     }
     ~Engine() override { release(); }
 
@@ -104,6 +127,7 @@ public:
         o->origin_x = originX_;
         o->origin_y = originY_;
         o->movable_area_scaled = movableAreaScaled_;
+        o->device_bytes = static_cast<int64_t>(devBytes_);
     }
 
     void optIndex(int32_t* out) const override {
@@ -216,7 +240,32 @@ public:
 
     void run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) override;
 
+    bool begin(sp_hip_gp_result* res) override {
+        if (cfg_.loop_mode == 0) throw InvalidArg("interleaved stages need loop_mode 1 or 2");
+        if (profMask_) throw InvalidArg("interleaved stages are not profiled");
+        asyncStart_ = true;
+        asyncReady_ = false;
+        try {
+            run(res, nullptr, 0);
+        } catch (...) {
+            asyncStart_ = false;
+            throw;
+        }
+        asyncStart_ = false;
+        return !asyncReady_;
+    }
+    void launchAsync() override;
+    int pollAsync(bool wait) override;
+    int inFlight() const override { return inflight_; }
+    void finishAsync(sp_hip_gp_result* res) override {
+        if (!asyncReady_) throw InvalidArg("no stage loop is running");
+        asyncReady_ = false;
+        finishDevice(res, nullptr, 0);
+    }
+    size_t deviceBytes() const override { return devBytes_; }
+
 private:
+    void finishDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows);
     void runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows, float lambda,
                    float gamma, double alpha0, double effRef, float targetTau, int maxIter,
                    int maxBt, float stepX);
@@ -341,6 +390,14 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
     int profMask_ = 0;   // bit f: time kernel family f with HIP events
+    // the device loop in flight: launches enqueued, each followed by a copy
+    // of ctl->done into hFlag_[slot] and an event (a ring of kAsyncDepth)
+    bool asyncStart_ = false, asyncReady_ = false, loopGraph_ = false;
+    int loopSlots_ = 1, inflight_ = 0, ringHead_ = 0;
+    long loopLaunches_ = 0, maxLaunches_ = 0;
+    std::chrono::steady_clock::time_point loopT0_{};
+    hipEvent_t loopEv_[kAsyncDepth] = {};
+    size_t devBytes_ = 0;
     std::vector<hipEvent_t> evStart_[SP_HIP_K_COUNT], evStop_[SP_HIP_K_COUNT];
     double kms_[SP_HIP_K_COUNT] = {}, kbytes_[SP_HIP_K_COUNT] = {}, kflops_[SP_HIP_K_COUNT] = {};
     int64_t klaunch_[SP_HIP_K_COUNT] = {};
@@ -363,6 +420,10 @@ void Engine<T>::release() {
     dRows_ = nullptr;
     if (hFlag_) (void)hipHostFree(hFlag_);
     hFlag_ = nullptr;
+    for (hipEvent_t& ev : loopEv_) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
     if (hScalars_) (void)hipHostFree(hScalars_);
     hScalars_ = nullptr;
     for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
@@ -396,28 +457,11 @@ void Engine<T>::build(const sp_hip_design& d) {
     n_ = static_cast<int>(optIdxHost_.size());
 
     // BinGrid::initialize, bin_grid.cpp:172-281.
-    constexpr int kMaxBinDim = 1024;
     const float target = cfg_.target_density;
     double cellArea = 0, macroArea = 0;
-    for (int i = 0; i < d.num_movable; ++i) {
-        if (isFixed(i) || isNI(i)) continue;
-        const double a = static_cast<double>(d.node_w[i]) * d.node_h[i];
-        if (isMacro(i)) macroArea += a;
-        else cellArea += a;
-    }
+    movableAreas(d, &cellArea, &macroArea);
     movableAreaScaled_ = cellArea + macroArea * static_cast<double>(target);
-    if (cfg_.gp_bin_dim > 0) {
-        dim_ = std::min(cfg_.gp_bin_dim, kMaxBinDim);
-    } else {
-        const double avgArea = d.num_movable > 0 ? (cellArea + macroArea) / d.num_movable : 1.0;
-        const double idealBin = std::max(avgArea / target, 1e-9);
-        const double coreArea = static_cast<double>(d.core_hx - d.core_lx) *
-                                static_cast<double>(d.core_hy - d.core_ly);
-        const double ideal = coreArea / idealBin;
-        dim_ = 4;
-        while (dim_ * 2 <= kMaxBinDim && static_cast<double>(dim_ * 2) * (dim_ * 2) <= ideal)
-            dim_ *= 2;
-    }
+    dim_ = binDim(d, cfg_);
     if (dim_ < 2) throw InvalidArg("bin grid must be at least 2x2");
     const float coreW = d.core_hx - d.core_lx, coreH = d.core_hy - d.core_ly;
     if (sizeof(T) == 4) {
@@ -704,7 +748,9 @@ void Engine<T>::build(const sp_hip_design& d) {
     dBufs_ = static_cast<T**>(own(dalloc<T*>(kNumBufs)));
     if (cfg_.loop_mode != 0)
         for (int i = 0; i < kMaxSnaps; ++i) snapPool_[i] = static_cast<T*>(own(dalloc<T>(n2)));
-    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hFlag_), sizeof(int), hipHostMallocDefault));
+    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hFlag_), kAsyncDepth * sizeof(int),
+                              hipHostMallocDefault));
+    for (hipEvent_t& ev : loopEv_) SPHIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     for (T** b : {&u_, &v_, &uPrev_, &vPrev_, &uNew_, &vNew_, &g_, &gPrev_, &gA_, &gB_, &probe_}) {
         *b = static_cast<T*>(own(dalloc<T>(n2)));
         SPHIP_CHECK(hipMemsetAsync(*b, 0, n2 * sizeof(T), s_));
@@ -1089,7 +1135,6 @@ template <typename T>
 void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows,
                           float lambda, float gamma, double alpha0, double effRef, float targetTau,
                           int maxIter, int maxBt, float stepX) {
-    const int n = n_;
     T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
     for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
     Ctl c{};
@@ -1131,6 +1176,7 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     if (rowsCap_ < maxIter) {
         if (dRows_) SPHIP_CHECK(hipFree(dRows_));
         dRows_ = dalloc<DevRow>(maxIter);
+        devBytes_ += (maxIter - rowsCap_) * sizeof(DevRow);
         rowsCap_ = maxIter;
         if (graphExec_) {   // the graph holds the old rows pointer
             SPHIP_CHECK(hipGraphExecDestroy(graphExec_));
@@ -1164,25 +1210,84 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         SPHIP_CHECK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
     }
     const int evMask = graph ? 0 : profMask_;
-    const long maxLaunches = (static_cast<long>(maxIter) * maxBt + 16) / slots + 2;
-    const auto t0 = std::chrono::steady_clock::now();
-    for (long launch = 0;; ++launch) {
-        if (launch > maxLaunches) throw std::runtime_error("device loop did not terminate");
-        if (graph) {
-            SPHIP_CHECK(hipGraphLaunch(graphExec_, s_));
-        } else {
-            for (int i = 0; i < slots; ++i) launchSlot(evMask != 0, i);
-        }
-        SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
-                                   sizeof(int), hipMemcpyDeviceToHost, s_));
-        SPHIP_CHECK(hipStreamSynchronize(s_));
-        const bool done = *hFlag_ != 0;
-        // only launches whose slots all ran a full evaluation are timed
-        if (evMask && !done) harvestK(evMask, slots);
-        if (done) break;
+    loopGraph_ = graph;
+    loopSlots_ = slots;
+    loopLaunches_ = 0;
+    maxLaunches_ = (static_cast<long>(maxIter) * maxBt + 16) / slots + 2;
+    inflight_ = 0;
+    ringHead_ = 0;
+    loopT0_ = std::chrono::steady_clock::now();
+    if (asyncStart_) {   // the caller drives the launches (launchAsync / pollAsync)
+        asyncReady_ = true;
+        return;
     }
+    if (evMask) {
+        // profiled: one launch at a time, its kernels' events harvested
+        for (long launch = 0;; ++launch) {
+            if (launch > maxLaunches_) throw std::runtime_error("device loop did not terminate");
+            for (int i = 0; i < slots; ++i) launchSlot(true, i);
+            SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
+                                       sizeof(int), hipMemcpyDeviceToHost, s_));
+            SPHIP_CHECK(hipStreamSynchronize(s_));
+            const bool done = *hFlag_ != 0;
+            // only launches whose slots all ran a full evaluation are timed
+            if (!done) harvestK(evMask, slots);
+            if (done) break;
+        }
+    } else {
+        // kAsyncDepth launches in flight: the next one is queued before the
+        // host has looked at the previous one's done flag; launches after
+        // the stage ended return at once (every kernel checks ctl->done)
+        for (int k = 0; k < kAsyncDepth; ++k) launchAsync();
+        while (!pollAsync(true)) launchAsync();
+    }
+    finishDevice(res, rows, maxRows);
+}
+
+template <typename T>
+void Engine<T>::launchAsync() {
+    if (inflight_ >= kAsyncDepth) throw std::logic_error("device loop: too many launches in flight");
+    if (++loopLaunches_ > maxLaunches_) throw std::runtime_error("device loop did not terminate");
+    if (loopGraph_) {
+        SPHIP_CHECK(hipGraphLaunch(graphExec_, s_));
+    } else {
+        for (int i = 0; i < loopSlots_; ++i) launchSlot(false, i);
+    }
+    const int k = (ringHead_ + inflight_) % kAsyncDepth;
+    SPHIP_CHECK(hipMemcpyAsync(hFlag_ + k, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
+                               sizeof(int), hipMemcpyDeviceToHost, s_));
+    SPHIP_CHECK(hipEventRecord(loopEv_[k], s_));
+    ++inflight_;
+}
+
+template <typename T>
+int Engine<T>::pollAsync(bool wait) {
+    if (inflight_ == 0) return 0;
+    hipEvent_t ev = loopEv_[ringHead_];
+    if (wait) {
+        SPHIP_CHECK(hipEventSynchronize(ev));
+    } else {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipErrorNotReady) return -1;
+        SPHIP_CHECK(q);
+    }
+    const int done = hFlag_[ringHead_] != 0 ? 1 : 0;
+    ringHead_ = (ringHead_ + 1) % kAsyncDepth;
+    --inflight_;
+    return done;
+}
+
+// The end of the device loop: the loop state, the metrics rows and the
+// delivered position.
+template <typename T>
+void Engine<T>::finishDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows) {
+    SPHIP_CHECK(hipStreamSynchronize(s_));
+    inflight_ = 0;
     res->seconds_loop =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - loopT0_).count();
+    T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
+    for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
+    Ctl c{};
     SPHIP_CHECK(hipMemcpyAsync(&c, dCtl_, sizeof(Ctl), hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
     if (c.status == kNonFinite)
@@ -1226,7 +1331,6 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     scatterPos(table[c.finalBuf]);
     res->overflow = static_cast<float>(deliveredMetrics(&hp));
     res->hpwl = hp;
-    (void)n;
 }
 
 // ===========================================================================
@@ -1322,6 +1426,49 @@ private:
 };
 
 }  // namespace
+
+std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
+    if (cfg.precision != 32 && cfg.precision != 64)
+        throw InvalidArg("precision must be 32 or 64");
+    if (cfg.stage < 0 || cfg.stage > 2) throw InvalidArg("stage must be 0, 1 or 2");
+    if (!(cfg.target_density > 0.f)) throw InvalidArg("target_density must be positive");
+    if (cfg.precision == 64) return std::make_unique<Engine<double>>(d, cfg);
+    return std::make_unique<Engine<float>>(d, cfg);
+}
+
+size_t footprintEstimate(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
+    validateDesign(d);
+    if (cfg.precision != 32 && cfg.precision != 64)
+        throw InvalidArg("precision must be 32 or 64");
+    if (!(cfg.target_density > 0.f)) throw InvalidArg("target_density must be positive");
+    const size_t t = cfg.precision == 64 ? 8 : 4;
+    const size_t total = static_cast<size_t>(d.num_nodes) + d.num_fillers;
+    const size_t pins = d.num_pins, nets = d.num_nets;
+    const size_t dim = binDim(d, cfg), bins = dim * dim;
+    const size_t padded = static_cast<size_t>(roundUp(static_cast<int>(dim), 64));
+    const size_t tiles = (dim + kTileBins - 1) / kTileBins;
+    const int maxIter = cfg.stage == 1 ? cfg.filler_only_iters : cfg.gp_max_iter;
+    size_t b = 0;
+    // per node: coordinates, sizes, flags, index, density records, CSR start
+    b += total * (12 * t + 25);
+    // per optimised node (at most every node): preconditioner, the state
+    // vectors and the snapshot pool
+    b += total * (12 + (2 * 11 + 2 * kMaxSnaps) * t);
+    // per pin: slot node, offsets, node-order entry, gradient; per net
+    b += pins * (8 + 4 * t) + 4 * (nets + 1);
+    // per bin: fixed-point grids, static densities, tile staging, and the
+    // transform buffers of the padded grid
+    b += bins * 32 + tiles * tiles * 2 * kTileWin * kTileWin * 8 + padded * padded * 8 * t;
+    b += 4 * padded * 2 * t * 2 + 4 * padded * t;   // twiddle tables, u and v
+    if (!PoissonPlan<float>::fftShape(static_cast<int>(dim)))
+        b += 6 * padded * padded * t;   // the dense transform matrices
+    // reduction partials (4 regions of at most one partial per 64 items),
+    // tile counters, metrics rows
+    b += 4 * 8 * 8 * ((std::max({total, pins, bins}) + 63) / 64 + 2) + 5 * 4 * (kTileKeys + 1);
+    b += static_cast<size_t>(std::max(maxIter, 0)) * sizeof(DevRow);
+    return b + 65536;
+}
+
 }  // namespace sphip
 
 struct sp_hip_poisson {
@@ -1386,13 +1533,8 @@ int sp_hip_gp_create(const sp_hip_design* d, const sp_hip_gp_config* cfg, sp_hip
     return guarded([&] {
         if (!d || !cfg || !out) throw sphip::InvalidArg("null argument");
         *out = nullptr;
-        if (cfg->precision != 32 && cfg->precision != 64)
-            throw sphip::InvalidArg("precision must be 32 or 64");
-        if (cfg->stage < 0 || cfg->stage > 2) throw sphip::InvalidArg("stage must be 0, 1 or 2");
-        if (!(cfg->target_density > 0.f)) throw sphip::InvalidArg("target_density must be positive");
         auto h = std::make_unique<sp_hip_gp>();
-        if (cfg->precision == 64) h->impl = std::make_unique<sphip::Engine<double>>(*d, *cfg);
-        else h->impl = std::make_unique<sphip::Engine<float>>(*d, *cfg);
+        h->impl = sphip::makeEngine(*d, *cfg);
         *out = h.release();
     });
 }

@@ -685,6 +685,7 @@ template <typename T>
 T* devAlloc(size_t n) {
     void* p = nullptr;
     SPHIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    g_devBytes += n * sizeof(T);
     return static_cast<T*>(p);
 }
 

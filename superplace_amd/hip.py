@@ -70,6 +70,11 @@ class SpMlgResult(C.Structure):
                                    "t0", "typical_worsen", "hpwl_before", "hpwl_after")]
 
 
+class SpHipJobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("device_bytes", C.c_int64),
+                ("arrival_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double)]
+
+
 class SpQpConfig(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("tol", C.c_float), ("solver_max_iter", C.c_int32),
                 ("outer_tol", C.c_float), ("min_distance", C.c_float),
@@ -136,7 +141,7 @@ class SpHipGpInfo(C.Structure):
     _fields_ = [("dim", C.c_int32), ("n_opt", C.c_int32), ("frac_bits", C.c_int32),
                 ("precision", C.c_int32), ("step_x", C.c_double), ("step_y", C.c_double),
                 ("origin_x", C.c_double), ("origin_y", C.c_double),
-                ("movable_area_scaled", C.c_double)]
+                ("movable_area_scaled", C.c_double), ("device_bytes", C.c_int64)]
 
 
 _lib = None
@@ -188,6 +193,16 @@ def lib():
                                    C.c_int64, _P]
     L.sp_abacus_legalize.argtypes = [_P, _P, _P, _P, C.c_int, _P]
     L.sp_abacus_segments.argtypes = [_P, _P, _P, _P, _P, C.c_int32, _P]
+    L.sp_hip_gp_footprint.argtypes = [_P, _P, _P]
+    L.sp_hip_server_create.argtypes = [C.c_int64, C.c_int32, C.POINTER(_P)]
+    L.sp_hip_server_submit.argtypes = [_P, _P, _P, C.c_double, _P]
+    L.sp_hip_server_pump.argtypes = [_P, _P, _P, _P, _P]
+    L.sp_hip_server_drain.argtypes = [_P, C.c_double]
+    L.sp_hip_server_now.argtypes = [_P, _P]
+    L.sp_hip_server_job.argtypes = [_P, C.c_int32, _P, _P, _P, _P]
+    L.sp_hip_server_release.argtypes = [_P, C.c_int32]
+    L.sp_hip_server_destroy.argtypes = [_P]
+    L.sp_hip_server_destroy.restype = None
     L.sp_macro_legalize.argtypes = [_P, _P, C.c_float, C.c_float, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _P]
     L.sp_metrics_create.argtypes = [C.c_char_p, C.POINTER(_P)]
@@ -404,6 +419,12 @@ class GpEngine:
         m = [{k: getattr(row, k) for k, _ in SpHipIterMetrics._fields_}
              for row in rows[:min(res.iterations, max_rows)]]
         return r, m
+
+    def device_bytes(self) -> int:
+        """Device memory the instance has allocated (set-up and loop buffers)."""
+        info = SpHipGpInfo()
+        check(lib().sp_hip_gp_info_get(self._h, C.byref(info)))
+        return int(info.device_bytes)
 
     def positions(self):
         t = self.design.total_nodes

@@ -28,7 +28,10 @@ def test_struct_layouts_match_header():
         "sp_hip_design": hip.SpHipDesign, "sp_hip_gp_config": hip.SpHipGpConfig,
         "sp_hip_iter_metrics": hip.SpHipIterMetrics, "sp_hip_gp_result": hip.SpHipGpResult,
         "sp_hip_eval_scalars": hip.SpHipEvalScalars, "sp_hip_eval_debug": hip.SpHipEvalDebug,
-        "sp_hip_gp_info": hip.SpHipGpInfo,
+        "sp_hip_gp_info": hip.SpHipGpInfo, "sp_hip_job_info": hip.SpHipJobInfo,
+        "sp_mlg_config": hip.SpMlgConfig, "sp_mlg_result": hip.SpMlgResult,
+        "sp_abacus_result": hip.SpAbacusResult, "sp_qp_config": hip.SpQpConfig,
+        "sp_filler_info": hip.SpFillerInfo,
     }
     src = "#include <stdio.h>\n#include <stddef.h>\n#include \"superplace_hip.h\"\nint main(void){\n"
     for cname in fields:
@@ -56,6 +59,23 @@ def test_no_device_paths_report_enodev():
     from superplace_amd import qp
     with pytest.raises(hip.NoDevice):
         qp.qp_place(D.fixture_eplace(20, 0))
+    from superplace_amd import serve
+    with pytest.raises(hip.NoDevice):
+        serve.Server(1 << 30)
+
+
+def test_footprint_estimate_without_device():
+    """The admission estimate is host arithmetic: it grows with the design,
+    the bin grid and the precision."""
+    from superplace_amd import design as D
+    from superplace_amd import serve
+    small, big = D.fixture_eplace(200, 2), D.fixture_eplace(2000, 8)
+    f = lambda d, **kw: serve.footprint(d, hip.GpConfig(**kw))
+    assert 0 < f(small) < f(big)
+    assert f(small, gp_bin_dim=256) > f(small, gp_bin_dim=64)
+    assert f(small, precision=64) > f(small, precision=32)
+    with pytest.raises(hip.InvalidArgument):
+        serve.footprint(small, hip.GpConfig(precision=16))
 
 
 def test_invalid_arguments_rejected_before_device():
