@@ -241,6 +241,12 @@ int sp_bookshelf_read_pl(sp_bookshelf* b, const char* pl_path);
  * reference's exact format (bookshelf_writer.cpp:16-37). */
 int sp_bookshelf_write_pl(const sp_bookshelf* b, const char* path, const float* x,
                           const float* y);
+/* writePl of a database whose flags and orientations changed since reading
+ * (mLG fixes macros and mirrors flipped ones): num_nodes flags (bit 2 writes
+ * /FIXED) and BookShelf orientation codes; either may be NULL (as read). */
+int sp_bookshelf_write_pl_state(const sp_bookshelf* b, const char* path, const float* x,
+                                const float* y, const uint8_t* node_flags,
+                                const uint8_t* node_orient);
 void sp_bookshelf_destroy(sp_bookshelf* b);
 
 /* ---------------------------------------------------- Quadratic placement

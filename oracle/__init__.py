@@ -99,6 +99,21 @@ def lib():
     return _lib
 
 
+CHECKER_PATH = HERE / "_ref" / "superplace-check"
+
+
+def check_placement(aux_path, pl_path):
+    """The reference's standalone legality checker (tools/check/main.cpp,
+    built into oracle/_ref): (passed, report text), or None if not built."""
+    if not CHECKER_PATH.exists():
+        return None
+    p = subprocess.run([str(CHECKER_PATH), str(aux_path), str(pl_path)], capture_output=True,
+                       text=True, timeout=300)
+    if p.returncode not in (0, 1):
+        raise RuntimeError(f"superplace-check failed to run: {p.stderr.strip()}")
+    return p.returncode == 0, p.stdout
+
+
 def ref_lib():
     """The reference's own WA/HPWL/Nesterov (oracle/_ref), or None if not built."""
     global _ref

@@ -77,11 +77,18 @@ class BookshelfFile:
     def read_pl(self, pl_path: str) -> None:
         hip.check(hip.lib().sp_bookshelf_read_pl(self._h, str(pl_path).encode()))
 
-    def write_pl(self, path: str, x, y) -> None:
+    def write_pl(self, path: str, x, y, flags=None, orient=None) -> None:
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.float32)
-        hip.check(hip.lib().sp_bookshelf_write_pl(self._h, str(path).encode(),
-                                                  x.ctypes.data, y.ctypes.data))
+        if flags is None and orient is None:
+            hip.check(hip.lib().sp_bookshelf_write_pl(self._h, str(path).encode(),
+                                                      x.ctypes.data, y.ctypes.data))
+            return
+        fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        ori = None if orient is None else np.ascontiguousarray(orient, np.uint8)
+        hip.check(hip.lib().sp_bookshelf_write_pl_state(
+            self._h, str(path).encode(), x.ctypes.data, y.ctypes.data,
+            None if fl is None else fl.ctypes.data, None if ori is None else ori.ctypes.data))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -102,14 +109,17 @@ def read(aux_path: str) -> Design:
 
 def write_pl(path: str, design: Design, x=None, y=None) -> None:
     """BookshelfWriter::writePl for a design from read(): the first num_nodes
-    centres (x, y; default the design's) as lower-left corners."""
+    centres (x, y; default the design's) as lower-left corners, with the
+    design's current flags (/FIXED) and orientations."""
     bf = design.meta.get("bookshelf")
     if bf is None:
         raise ValueError("write_pl needs a design read from BookShelf files")
     n = design.num_nodes
     x = design.node_x[:n] if x is None else np.asarray(x)[:n]
     y = design.node_y[:n] if y is None else np.asarray(y)[:n]
-    bf.write_pl(path, x, y)
+    orient = design.meta.get("node_orient")
+    bf.write_pl(path, x, y, flags=design.node_flags[:n],
+                orient=None if orient is None else np.asarray(orient)[:n])
 
 
 def _num(v) -> str:
@@ -117,10 +127,11 @@ def _num(v) -> str:
     return np.format_float_positional(np.float32(v), unique=True, trim="-")
 
 
-def write_bookshelf(design: Design, directory: str, name: str) -> str:
+def write_bookshelf(design: Design, directory: str, name: str, site: float = 1.0) -> str:
     """Writes a design (without fillers) as a BookShelf set: name.aux, .nodes,
     .nets, .pl, .scl, .wts; returns the .aux path.  Fixed nodes are
-    terminals; the .pl holds lower-left corners; rows of unit sites."""
+    terminals; the .pl holds lower-left corners; rows of whole sites of
+    width `site` from each row's lx."""
     import os
     os.makedirs(directory, exist_ok=True)
     d = design
@@ -148,11 +159,13 @@ def write_bookshelf(design: Design, directory: str, name: str) -> str:
             f.write(f"{nodes[i]}\t{_num(llx)}\t{_num(lly)}\t: N{' /FIXED' if fixed[i] else ''}\n")
     with open(os.path.join(directory, f"{name}.scl"), "w") as f:
         f.write(f"UCLA scl 1.0\n\nNumRows : {d.num_rows}\n\n")
+        step = _num(site)
         for r in range(d.num_rows):
-            sites = int(round(float(d.row_hx[r] - d.row_lx[r])))
+            span = float(d.row_hx[r] - d.row_lx[r])
+            sites = int(round(span)) if site == 1.0 else int(np.floor(span / float(np.float32(site)) + 1e-6))
             f.write(f"CoreRow Horizontal\n  Coordinate    :   {_num(d.row_ly[r])}\n"
-                    f"  Height        :   {_num(d.row_height[r])}\n  Sitewidth     :    1\n"
-                    f"  Sitespacing   :    1\n  Siteorient    :    1\n  Sitesymmetry  :    1\n"
+                    f"  Height        :   {_num(d.row_height[r])}\n  Sitewidth     :    {step}\n"
+                    f"  Sitespacing   :    {step}\n  Siteorient    :    1\n  Sitesymmetry  :    1\n"
                     f"  SubrowOrigin  :    {_num(d.row_lx[r])}\tNumSites  :  {sites}\nEnd\n")
     with open(os.path.join(directory, f"{name}.wts"), "w") as f:
         f.write("UCLA wts 1.0\n\n")

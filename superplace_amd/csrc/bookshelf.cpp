@@ -135,7 +135,8 @@ struct Bookshelf {
 
     void read(const std::string& auxPath);
     void readPlacement(const std::string& plPath);
-    void writePl(const std::string& path, const float* cx, const float* cy) const;
+    void writePl(const std::string& path, const float* cx, const float* cy,
+                 const uint8_t* nodeFlags = nullptr, const uint8_t* nodeOrient = nullptr) const;
 
 private:
     std::unordered_map<std::string_view, int> nodeIndex() const {
@@ -395,7 +396,10 @@ void Bookshelf::readWts(const std::string& path) {
     }
 }
 
-void Bookshelf::writePl(const std::string& path, const float* cx, const float* cy) const {
+void Bookshelf::writePl(const std::string& path, const float* cx, const float* cy,
+                        const uint8_t* nodeFlags, const uint8_t* nodeOrient) const {
+    const uint8_t* fl = nodeFlags ? nodeFlags : flags.data();
+    const uint8_t* orn = nodeOrient ? nodeOrient : orient.data();
     std::FILE* out = std::fopen(path.c_str(), "wb");
     if (!out) throw InvalidArg("cannot write " + path);
     std::fputs("UCLA pl 1.0\n# Generated by superPlace\n\n", out);
@@ -403,8 +407,7 @@ void Bookshelf::writePl(const std::string& path, const float* cx, const float* c
         const double llx = static_cast<double>(cx[i]) - 0.5 * w[i];
         const double lly = static_cast<double>(cy[i]) - 0.5 * h[i];
         std::fprintf(out, "%s\t%.6f\t%.6f\t: %s%s\n", names[i].c_str(), llx, lly,
-                     kOrients[orient[i] < 8 ? orient[i] : 0],
-                     (flags[i] & F_FIXED) ? " /FIXED" : "");
+                     kOrients[orn[i] < 8 ? orn[i] : 0], (fl[i] & F_FIXED) ? " /FIXED" : "");
     }
     if (std::fclose(out) != 0) throw InvalidArg("cannot write " + path);
 }
@@ -507,6 +510,15 @@ int sp_bookshelf_write_pl(const sp_bookshelf* b, const char* path, const float* 
     return guarded([&] {
         if (!b || !path || !x || !y) throw InvalidArg("null argument");
         b->db.writePl(path, x, y);
+    });
+}
+
+int sp_bookshelf_write_pl_state(const sp_bookshelf* b, const char* path, const float* x,
+                                const float* y, const uint8_t* node_flags,
+                                const uint8_t* node_orient) {
+    return guarded([&] {
+        if (!b || !path || !x || !y) throw InvalidArg("null argument");
+        b->db.writePl(path, x, y, node_flags, node_orient);
     });
 }
 

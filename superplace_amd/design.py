@@ -279,14 +279,21 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
                  row_height: float = 12.0, tail: str = "geometric", tail_max: int = 32,
                  fixed_frac: float = 0.10, seed: int = 1002, init_sigma_frac: float = 0.05,
                  locality: float = 8.0, fixed_pin_frac: float = 0.01, name: str = "synth",
-                 target_density: float = 1.0, with_fillers: bool = True) -> Design:
+                 target_density: float = 1.0, with_fillers: bool = True,
+                 num_macros: int = 0, macro_area_frac: float = 0.0) -> Design:
     """Seeded synthetic standard-cell design (the generator behind the
     benchmark workloads): num_cells movable cells of width U{1..8} sites and
     one row height, num_fixed fixed macros covering about fixed_frac of the
     die, num_nets nets with P(2)=0.55, P(3)=0.2 and a geometric (config-1
     style) or power-law tail.  The site pitch is chosen so that
     movable / (row area - fixed area) equals util (the reference's
-    utilisation definition, src/db/db_stats.h:75-78)."""
+    utilisation definition, src/db/db_stats.h:75-78).
+
+    num_macros movable macros (after the cells in node order) together take
+    macro_area_frac of the movable area: areas U(0.5, 1.5) times the mean,
+    aspect ratios U(0.5, 2), heights rounded to whole rows and at least two
+    rows (the reader's macro rule is taller than 1.5 rows), widths keeping
+    the drawn area."""
     rng = Rng(seed)
     nrows = int(die // row_height)
     rows = (np.arange(nrows, dtype=np.float32) * row_height,
@@ -304,10 +311,21 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
     fixed_area = float(np.sum(fw.astype(np.float64) * fh))
     row_area = float(nrows) * row_height * die
     sites = rng.integers(num_cells, 1, 8).astype(np.float64)
-    pitch = util * (row_area - fixed_area) / (num_cells * row_height * sites.mean())
+    movable_area = util * (row_area - fixed_area)
+    cell_area = movable_area * (1.0 - macro_area_frac) if num_macros > 0 else movable_area
+    pitch = cell_area / (num_cells * row_height * sites.mean())
     cw = (sites * pitch).astype(np.float32)
     ch = np.full(num_cells, row_height, np.float32)
-    cx, cy = _cluster_init(rng, num_cells, 0.5 * die, 0.5 * core_h, init_sigma_frac * die)
+    if num_macros > 0:
+        rel = rng.uniform(num_macros, 0.5, 1.5)
+        area = rel / rel.sum() * movable_area * macro_area_frac
+        aspect = rng.uniform(num_macros, 0.5, 2.0)
+        mh = np.maximum(np.round(np.sqrt(area / aspect) / row_height), 2.0) * row_height
+        mw = area / mh
+        cw = np.concatenate([cw, mw.astype(np.float32)])
+        ch = np.concatenate([ch, mh.astype(np.float32)])
+    n_mov = num_cells + num_macros
+    cx, cy = _cluster_init(rng, n_mov, 0.5 * die, 0.5 * core_h, init_sigma_frac * die)
     # degrees
     if tail == "geometric":
         q = 0.7
@@ -317,14 +335,15 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
         a = _degree_tail_powerlaw((4.27 - 0.55 * 2 - 0.2 * 3) / 0.25, 4, tail_max)
         tail_probs = np.arange(4, tail_max + 1, dtype=np.float64) ** -a
     degrees = _sample_degrees(rng, num_nets, tail_probs, 4)
-    pin2node, pin2net = _netlist(rng, degrees, num_cells, n_fixed, fixed_pin_frac, locality)
+    pin2node, pin2net = _netlist(rng, degrees, n_mov, n_fixed, fixed_pin_frac, locality)
     node_w = np.concatenate([cw, fw])
     node_h = np.concatenate([ch, fh])
     offx = (rng.uniform(len(pin2node), -0.5, 0.5) * node_w[pin2node]).astype(np.float32)
     offy = (rng.uniform(len(pin2node), -0.5, 0.5) * node_h[pin2node]).astype(np.float32)
-    flags = np.concatenate([np.zeros(num_cells, np.uint8), np.full(n_fixed, F_FIXED, np.uint8)])
+    flags = np.concatenate([np.zeros(num_cells, np.uint8), np.full(num_macros, F_MACRO, np.uint8),
+                            np.full(n_fixed, F_FIXED, np.uint8)])
     d = make_design(np.concatenate([cx, fx]), np.concatenate([cy, fy]), node_w, node_h, flags,
-                    num_cells, pin2node, pin2net, offx, offy, num_nets, rows, name=name,
+                    n_mov, pin2node, pin2net, offx, offy, num_nets, rows, name=name,
                     meta=dict(seed=seed, die=die, util=util, site_pitch=float(pitch),
                               fixed_macros=n_fixed, fixed_area_frac=fixed_area / (die * core_h),
                               tail=tail, tail_exponent=a, tail_max=tail_max,
@@ -341,6 +360,15 @@ def config_small(seed: int = 1002, with_fillers: bool = True) -> Design:
     return synth_design(2000, 8, 2200, die=424.0, util=0.70, tail="geometric", tail_max=32,
                         fixed_frac=0.14, seed=seed, name="synth-2k", with_fillers=with_fillers,
                         locality=6.0)
+
+
+def config_mixed(seed: int = 1002, with_fillers: bool = True) -> Design:
+    """Mixed-size: the 2,000-cell design above plus 63 movable macros
+    (aspect 0.5-2, together 56.9% of the movable area), 70% utilisation
+    counting the macros (1024x1024 bins in the workload)."""
+    return synth_design(2000, 8, 2200, die=424.0, util=0.70, tail="geometric", tail_max=32,
+                        fixed_frac=0.14, seed=seed, name="synth-mixed", with_fillers=with_fillers,
+                        locality=6.0, num_macros=63, macro_area_frac=0.569)
 
 
 def config_adaptec1(seed: int = 1002, with_fillers: bool = True) -> Design:

@@ -187,6 +187,7 @@ def lib():
     L.sp_bookshelf_names.argtypes = [_P, C.c_int, _P, C.c_int64, C.POINTER(C.c_int64)]
     L.sp_bookshelf_read_pl.argtypes = [_P, C.c_char_p]
     L.sp_bookshelf_write_pl.argtypes = [_P, C.c_char_p, _P, _P]
+    L.sp_bookshelf_write_pl_state.argtypes = [_P, C.c_char_p, _P, _P, _P, _P]
     L.sp_bookshelf_destroy.argtypes = [_P]
     L.sp_qp_place.argtypes = [_P, _P, _P, _P, _P, _P, C.c_int, _P]
     L.sp_qp_b2b_system.argtypes = [_P, _P, C.c_int, C.c_float, C.c_int32, _P, _P, _P, _P, _P,
