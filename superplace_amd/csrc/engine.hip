@@ -20,6 +20,7 @@
 #include "../../include/superplace_hip.h"
 #include "abi.h"
 #include "engine.h"
+#include "wa_device.h"
 #include "gp_kernels.h"
 #include "poisson.h"
 
@@ -1112,12 +1113,24 @@ void Engine<T>::launchSlot(bool events, int slot) {
     launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
                       binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, dCtl_, s_);
     e(SP_HIP_K_FINALIZE);
-    b(SP_HIP_K_POISSON);
-    poisson_->solve(false, skip);
-    e(SP_HIP_K_POISSON);
-    b(SP_HIP_K_WA_NETS);
-    launchWaNets<T>(dd_, 0.f, cfg_.ignore_net_degree, ws_, dCtl_, s_);
-    e(SP_HIP_K_WA_NETS);
+    // Unprofiled slots run the WA pass's blocks inside the Poisson column
+    // pass's launch (same blocks, same partials); profiled slots launch the
+    // two apart so that each family has its own events.
+    bool fused = false;
+    if (!events) {
+        const int waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
+        fused = poisson_->solveWithWa(
+            WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws_.region(1), dCtl_}, waBlocks, skip);
+        if (fused) ws_.blocks[1] = waBlocks;
+    }
+    if (!fused) {
+        b(SP_HIP_K_POISSON);
+        poisson_->solve(false, skip);
+        e(SP_HIP_K_POISSON);
+        b(SP_HIP_K_WA_NETS);
+        launchWaNets<T>(dd_, 0.f, cfg_.ignore_net_degree, ws_, dCtl_, s_);
+        e(SP_HIP_K_WA_NETS);
+    }
     b(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), 0.f, 0.f, nullptr,
                       nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, s_);

@@ -72,6 +72,11 @@ struct DevDesign {
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
 constexpr int kWaMaxBlocks = 1024;   // the WA pass strides its wavefront tasks over at most these
+// Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each).
+inline int waBlocksFor(int waves) {
+    const int b = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    return b < 1 ? 1 : (b > kWaMaxBlocks ? kWaMaxBlocks : b);
+}
 constexpr int kTileBins = 16;        // bins per tile side
 constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;

@@ -21,6 +21,13 @@
 namespace sphip {
 
 template <typename T>
+struct WaTask;
+
+// LDS bound of the column pass that carries the WA blocks (all blocks of a
+// launch reserve it, so a larger one starves the WA blocks of occupancy).
+constexpr size_t kFusedLdsMax = 32 * 1024;
+
+template <typename T>
 class PoissonPlan {
 public:
     // interleaved: the field is written as (xi_x, xi_y) pairs into fieldXY()
@@ -35,6 +42,11 @@ public:
     // fieldX()/fieldY() and, if withPhi, phi().  Enqueued on the stream; the
     // kernels return at once while *skip (a device flag, may be null) is set.
     void solve(bool withPhi, const int* skip = nullptr);
+    // solve(false, skip) with the WA pass's waBlocks blocks carried by the
+    // column pass's launch (the same partials as launching the WA pass
+    // alone with waBlocks blocks).  Returns false, enqueuing nothing, where
+    // the grid does not allow it (dense path, or LDS above kFusedLdsMax).
+    bool solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip);
 
     int nx() const { return nx_; }
     int ny() const { return ny_; }
@@ -70,7 +82,7 @@ private:
     // (complex, interleaved); LDS bytes of the three passes
     bool fft_ = false;
     T *tabX_ = nullptr, *tabY_ = nullptr;
-    size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0;
+    size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0, ldsBW_ = 0;
     int colPairs_ = 1;   // column pairs per block of the column pass
     void setFftLds();
     void solveFft(bool withPhi, const int* skip);

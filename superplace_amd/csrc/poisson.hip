@@ -5,10 +5,12 @@
 // every axis transform is a matrix product on the matrix cores:
 // v_mfma_f32_16x16x4_f32 (FP32 path) or v_mfma_f64_16x16x4_f64 (FP64 path),
 // LDS-staged 64-wide wavefront tiles.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "poisson.h"
+#include "wa_device.h"
 
 namespace sphip {
 namespace {
@@ -429,21 +431,22 @@ constexpr int kFftBatch = 8;
 
 // Threads per block of the FFT passes: one block transforms one line pair
 // (passes A, B) or one row (pass C); two waves for lines of 512 and more.
-inline int fftThreads(int L) { return L >= 512 ? 128 : 64; }
+__host__ __device__ inline int fftThreads(int L) { return L >= 512 ? 128 : 64; }
 
-// Pass A: REDFT10 along y of rows 2 blockIdx.x and 2 blockIdx.x + 1.
+// Pass A: REDFT10 along y of row pairs, one pair per team of
+// nt = fftThreads(ny) threads: team q of block `block` takes rows
+// 2 (block * teams + q) and the next.
 template <typename T>
-__global__ __launch_bounds__(128) void k_dct_rows(const T* __restrict__ rho, T* __restrict__ out,
-                                                  int nx, int ny, int ld, FftTab<T> tab,
-                                                  const int* skip) {
-    if (skip && *skip) return;
+__device__ inline void dctRowsBlock(const T* __restrict__ rho, T* __restrict__ out, int nx,
+                                    int ny, int ld, const FftTab<T>& tab, int block) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int L = ny, nt = blockDim.x, tid = threadIdx.x;
+    const int L = ny, nt = fftThreads(L), teams = blockDim.x / nt;
+    const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
     const FftLds<T> f = fftLds<T>(smem, L);
-    cplx<T>* a[1] = {f.waves};
-    cplx<T>* b[1] = {f.waves + L};
+    cplx<T>* a[1] = {f.waves + static_cast<long>(team) * 2 * L};
+    cplx<T>* b[1] = {a[0] + L};
     loadTables<T>(f, tab.tw, tab.hf, nullptr, L);
-    const int r0 = 2 * blockIdx.x;
+    const int r0 = 2 * (block * teams + team);
     const T* xa = rho + static_cast<long>(r0) * ld;
     const T* xb = xa + ld;
     for (int i0 = tid; i0 < L; i0 += nt * kFftBatch) {
@@ -475,6 +478,15 @@ __global__ __launch_bounds__(128) void k_dct_rows(const T* __restrict__ rho, T* 
             ob[L - k] = rbM;
         }
     }
+    (void)nx;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dct_rows(const T* __restrict__ rho, T* __restrict__ out,
+                                                     int nx, int ny, int ld, FftTab<T> tab,
+                                                     const int* skip) {
+    if (skip && *skip) return;
+    dctRowsBlock<T>(rho, out, nx, ny, ld, tab, blockIdx.x);
 }
 
 // Pass B: G column pairs per block (pair g: columns c0 + 2g, c0 + 2g + 1),
@@ -482,26 +494,38 @@ __global__ __launch_bounds__(128) void k_dct_rows(const T* __restrict__ rho, T* 
 // spectral divide (cpu_poisson.cpp:163-173), then both inverse transforms
 // along x at once (cosine -> Hc, shifted sine of u P -> Hs).  The block
 // loads and stores 2G adjacent values per row.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
-                                                   T* __restrict__ Hs, int G, int nx, int ny,
-                                                   int ld, FftTab<T> tab, const T* __restrict__ u,
-                                                   const T* __restrict__ v, T invNorm,
-                                                   const int* skip) {
-    if (skip && *skip) return;
+// LDS_TABLES: the twiddle, half-angle and u tables are copied into LDS
+// ahead of the teams' buffers; otherwise they are read from global memory
+// (through the caches) and the buffers start at LDS offset 0.
+template <typename T, bool LDS_TABLES>
+__device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc,
+                                    T* __restrict__ Hs, int G, int nx, int ny, int ld,
+                                    const FftTab<T>& tab, const T* __restrict__ u,
+                                    const T* __restrict__ v, T invNorm, int block) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int L = nx, nt = blockDim.x / G;
     const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
-    const FftLds<T> f = fftLds<T>(smem, L);
+    FftLds<T> f;
+    if (LDS_TABLES) {
+        f = fftLds<T>(smem, L);
+        loadTables<T>(f, tab.tw, tab.hf, u, L);
+    } else {
+        f.tw = const_cast<cplx<T>*>(tab.tw);
+        f.hf = const_cast<cplx<T>*>(tab.hf);
+        f.sc = const_cast<T*>(u);
+        f.waves = reinterpret_cast<cplx<T>*>(smem);
+    }
     const cplx<T>* hf = f.hf;
     const T* us = f.sc;
-    cplx<T>* region = f.waves;   // team g owns region[4 L g, 4 L (g + 1))
-    cplx<T>* b0 = region + static_cast<long>(team) * 4 * L;
+    // team g owns region[NB L g, NB L (g + 1)): four line buffers, or three
+    // when the tables are not in LDS (the inverse transforms then run one
+    // after the other, sharing the third buffer)
+    constexpr int NB = LDS_TABLES ? 4 : 3;
+    cplx<T>* region = f.waves;
+    cplx<T>* b0 = region + static_cast<long>(team) * NB * L;
     cplx<T>* b1 = b0 + L;
     cplx<T>* b2 = b1 + L;
-    cplx<T>* b3 = b2 + L;
-    loadTables<T>(f, tab.tw, tab.hf, u, L);
-    const int C = 2 * G, c0 = blockIdx.x * C;
+    const int C = 2 * G, c0 = block * C;
     // element e: row e / C, column c0 + e % C (C adjacent values per row)
     const int total = L * C;
     for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kFftBatch) {
@@ -517,7 +541,7 @@ __global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* 
             const int e = e0 + blockDim.x * q;
             if (e < total) {
                 const int x = e / C, c = e - x * C;
-                reinterpret_cast<T*>(region + static_cast<long>(c >> 1) * 4 * L + dctPos(x, L))[c & 1] =
+                reinterpret_cast<T*>(region + static_cast<long>(c >> 1) * NB * L + dctPos(x, L))[c & 1] =
                     vals[q];
             }
         }
@@ -566,14 +590,29 @@ __global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* 
         }
     }
     __syncthreads();
-    cplx<T>* in[2] = {o, r};
-    cplx<T>* sc2[2] = {b2, b3};
-    blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
+    const cplx<T>* res1;
+    const cplx<T>* res2;
+    if (LDS_TABLES) {
+        cplx<T>* in[2] = {o, r};
+        cplx<T>* sc2[2] = {b2, b2 + L};
+        blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
+        res1 = in[0];
+        res2 = in[1];
+    } else {
+        cplx<T>* i1[1] = {o};
+        cplx<T>* s1[1] = {b2};
+        blockFft<T, true, 1>(i1, s1, f.tw, L, tid, nt);
+        cplx<T>* i2[1] = {r};
+        cplx<T>* s2[1] = {i1[0] == o ? b2 : o};
+        blockFft<T, true, 1>(i2, s2, f.tw, L, tid, nt);
+        res1 = i1[0];
+        res2 = i2[0];
+    }
     // the results sit at the same offsets in every team's region
-    const long off1 = in[0] - b0, off2 = in[1] - b0;
+    const long off1 = res1 - b0, off2 = res2 - b0;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         const int x = e / C, c = e - x * C;
-        const cplx<T>* w = region + static_cast<long>(c >> 1) * 4 * L;
+        const cplx<T>* w = region + static_cast<long>(c >> 1) * NB * L;
         const int pos = dctPos(x, L);
         const cplx<T> z1 = w[off1 + pos], z2 = w[off2 + pos];
         const long at = static_cast<long>(x) * ld + c0 + c;
@@ -583,29 +622,59 @@ __global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* 
     }
 }
 
-// Pass C: row blockIdx.x: xi_x = REDFT01_y(Hs[x]) and xi_y = shifted sine of
-// v Hc[x], packed in one complex transform, and phi = REDFT01_y(Hc[x]) when
-// requested (both transforms at once).  The field goes to fx[e * cstep],
-// fy[e * cstep] (cstep 2 interleaves).
 template <typename T>
-__global__ __launch_bounds__(128) void k_idct_rows(const T* __restrict__ Hc,
-                                                   const T* __restrict__ Hs, T* __restrict__ fx,
-                                                   T* __restrict__ fy, T* __restrict__ phi,
-                                                   int cstep, int nx, int ny, int ld,
-                                                   FftTab<T> tab, const T* __restrict__ v,
+__global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
+                                                   T* __restrict__ Hs, int G, int nx, int ny,
+                                                   int ld, FftTab<T> tab, const T* __restrict__ u,
+                                                   const T* __restrict__ v, T invNorm,
                                                    const int* skip) {
     if (skip && *skip) return;
+    dctColsBlock<T, true>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm, blockIdx.x);
+}
+
+// Pass B with the WA pass's blocks in the same launch: blocks below
+// colBlocks run the column pass (kBlock threads = G teams, tables read from
+// global memory so that its LDS leaves room for the WA blocks), the rest WA
+// blocks waFirst, waFirst + 1, ... of a WA pass of waTotal blocks.  The WA
+// pass depends only on the positions, so it fills the CUs that the column
+// pass leaves idle.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dct_cols_wa(const T* __restrict__ F1,
+                                                        T* __restrict__ Hc, T* __restrict__ Hs,
+                                                        int G, int nx, int ny, int ld,
+                                                        FftTab<T> tab, const T* __restrict__ u,
+                                                        const T* __restrict__ v, T invNorm,
+                                                        const int* skip, WaTask<T> wa,
+                                                        int colBlocks, int waFirst, int waTotal) {
+    if (skip && *skip) return;
+    if (static_cast<int>(blockIdx.x) < colBlocks)
+        dctColsBlock<T, false>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm, blockIdx.x);
+    else
+        waBlockTask<T>(wa, waFirst + (blockIdx.x - colBlocks), waTotal);
+}
+
+// Pass C: one row per team of nt = fftThreads(ny) threads (team q of block
+// `block` takes row x = block * teams + q): xi_x = REDFT01_y(Hs[x]) and
+// xi_y = shifted sine of v Hc[x], packed in one complex transform, and
+// phi = REDFT01_y(Hc[x]) when requested (both transforms at once).  The
+// field goes to fx[e * cstep], fy[e * cstep] (cstep 2 interleaves).
+template <typename T>
+__device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restrict__ Hs,
+                                     T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ phi,
+                                     int cstep, int nx, int ny, int ld, const FftTab<T>& tab,
+                                     const T* __restrict__ v, int block) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int L = ny, nt = blockDim.x, tid = threadIdx.x;
+    const int L = ny, nt = fftThreads(L), teams = blockDim.x / nt;
+    const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
     const FftLds<T> f = fftLds<T>(smem, L);
     const cplx<T>* hf = f.hf;
     const T* vs = f.sc;
-    cplx<T>* a = f.waves;
+    cplx<T>* a = f.waves + static_cast<long>(team) * (phi ? 4 : 2) * L;
     cplx<T>* b = a + L;
     cplx<T>* c = phi ? b + L : nullptr;
     cplx<T>* d = phi ? c + L : nullptr;
     loadTables<T>(f, tab.tw, tab.hf, v, L);
-    const int x = blockIdx.x;
+    const int x = block * teams + team;
     // the row pair (Hs[x][j], Hc[x][j]) into b
     const T* hs = Hs + static_cast<long>(x) * ld;
     const T* hc = Hc + static_cast<long>(x) * ld;
@@ -660,6 +729,18 @@ __global__ __launch_bounds__(128) void k_idct_rows(const T* __restrict__ Hc,
         fx[(row + y) * cstep] = w.x;
         fy[(row + y) * cstep] = (y & 1) ? -w.y : w.y;
     }
+    (void)nx;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_idct_rows(const T* __restrict__ Hc,
+                                                      const T* __restrict__ Hs,
+                                                      T* __restrict__ fx, T* __restrict__ fy,
+                                                      T* __restrict__ phi, int cstep, int nx,
+                                                      int ny, int ld, FftTab<T> tab,
+                                                      const T* __restrict__ v, const int* skip) {
+    if (skip && *skip) return;
+    idctRowsBlock<T>(Hc, Hs, fx, fy, phi, cstep, nx, ny, ld, tab, v, blockIdx.x);
 }
 
 template <typename T, int EPI, bool TRANS>
@@ -849,6 +930,18 @@ void PoissonPlan<T>::setFftLds() {
         colPairs_ >>= 1;
     ldsB_ = (kFftLdsHead(nx_) + 4 * static_cast<size_t>(colPairs_) * nx_) * c;
     ldsC_ = (kFftLdsHead(ny_) + 4 * static_cast<size_t>(ny_)) * c;
+    // the column pass sharing its launch with the WA blocks: kBlock threads,
+    // so kBlock / team column pairs per block, its tables read from global
+    // memory (every block of the launch reserves the same LDS, and the WA
+    // blocks need the room); only where that LDS stays within kFusedLdsMax
+    {
+        const int nt = fftThreads(nx_), G = kBlock / nt;
+        const size_t lds = 3 * static_cast<size_t>(G) * nx_ * c;
+        ldsBW_ = (kBlock % nt == 0 && (ny_ / 2) % G == 0 && lds <= kFusedLdsMax) ? lds : 0;
+        if (ldsBW_)
+            SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols_wa<T>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, ldsBW_));
+    }
     SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
     SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<T>),
@@ -877,6 +970,31 @@ void PoissonPlan<T>::solveFft(bool withPhi, const int* skip) {
                  stream_, t1_, t2_, fx, fy, withPhi ? phi_ : nullptr, interleaved_ ? 2 : 1, nx_,
                  ny_, ld, ty, v_, skip);
     SPHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+bool PoissonPlan<T>::solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip) {
+    if (!fft_ || ldsBW_ == 0) return false;
+    using C2 = cplx<T>;
+    const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
+    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    const int ld = my_;
+    launchKernel(k_dct_rows<T>, dim3(nx_ / 2), dim3(fftThreads(ny_)),
+                 static_cast<uint32_t>(ldsA_), stream_, rho_, t0_, nx_, ny_, ld, ty, skip);
+    SPHIP_CHECK(hipGetLastError());
+    const int G = kBlock / fftThreads(nx_), colBlocks = ny_ / (2 * G);
+    launchKernel(k_dct_cols_wa<T>, dim3(colBlocks + waBlocks), dim3(kBlock),
+                 static_cast<uint32_t>(ldsBW_), stream_, t0_, t1_, t2_, G, nx_, ny_, ld, tx, u_,
+                 v_, invNorm_, skip, wa, colBlocks, 0, waBlocks);
+    SPHIP_CHECK(hipGetLastError());
+    T* fx = interleaved_ ? exy_ : ex_;
+    T* fy = interleaved_ ? exy_ + 1 : ey_;
+    const size_t lds = ldsC_ - 2 * static_cast<size_t>(ny_) * 2 * sizeof(T);
+    launchKernel(k_idct_rows<T>, dim3(nx_), dim3(fftThreads(ny_)), static_cast<uint32_t>(lds),
+                 stream_, t1_, t2_, fx, fy, static_cast<T*>(nullptr), interleaved_ ? 2 : 1, nx_,
+                 ny_, ld, ty, v_, skip);
+    SPHIP_CHECK(hipGetLastError());
+    return true;
 }
 
 template <typename T>

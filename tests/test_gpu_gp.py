@@ -213,3 +213,27 @@ def test_graph_loop_backtracks_and_hits_iteration_limit():
     assert res["hpwl"] == pytest.approx(oracle.hpwl(d, 64, x, y), rel=1e-6)
     for a, b in zip(rows[:-1], rows[1:]):
         assert b["eval_lambda"] == a["lambda_"] and b["eval_gamma"] == a["gamma"]
+
+
+@pytest.mark.parametrize("name", ["synth2k", "fixture"])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_wa_in_column_pass_is_bitwise_neutral(name, prec):
+    """The device loop carries the WA pass's blocks in the Poisson column
+    pass's launch; profiled runs launch them apart.  Both give the same
+    rows and positions bit for bit."""
+    d = designs()[name]
+    cfg = hip.GpConfig(precision=prec, gp_bin_dim=BIN_DIM[name], gp_max_iter=60)
+    a = hip.GpEngine(d, cfg)
+    ra, rows_a = a.run()
+    xa, ya = a.positions()
+    b = hip.GpEngine(d, cfg)
+    b.set_profiling()
+    rb, rows_b = b.run()
+    xb, yb = b.positions()
+    keys = ["iter", "evaluations", "hpwl", "overflow", "lambda_", "gamma", "step_size",
+            "grad_norm_wl", "grad_norm_den"]
+    assert len(rows_a) == len(rows_b) == ra["iterations"]
+    for p, q in zip(rows_a, rows_b):
+        assert all(p[k] == q[k] for k in keys if k in p), (p, q)
+    assert ra["hpwl"] == rb["hpwl"] and ra["overflow"] == rb["overflow"]
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
