@@ -231,7 +231,8 @@ def main():
     #    family carrying its HIP event pair) to find the dominant one
     probe = timed_run(d, 32, args.steps, args.warmup,
                       profile_mask=(1 << len(hip.KERNEL_FAMILIES)) - 1)
-    fam_us = {f: 1e3 * st["ms"] / max(st["launches"], 1) for f, st in probe["stats"].items()}
+    fam_us = {f: 1e3 * st["ms"] / st["launches"] for f, st in probe["stats"].items()
+              if st["launches"] > 0}
     dominant = max(fam_us, key=fam_us.get)
     log("kernel families, average us per launch: " +
         ", ".join(f"{f} {v:.1f}" for f, v in fam_us.items()) + f"; dominant {dominant}")
