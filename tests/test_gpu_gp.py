@@ -237,3 +237,27 @@ def test_wa_in_column_pass_is_bitwise_neutral(name, prec):
         assert all(p[k] == q[k] for k in keys if k in p), (p, q)
     assert ra["hpwl"] == rb["hpwl"] and ra["overflow"] == rb["overflow"]
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+
+
+def test_benchmark_design_quality():
+    """The benchmark workload (211k cells, 512x512 bins) run to overflow
+    0.10: the FP32 path's final HPWL within 1% of the FP64 GPU path's and of
+    the FP64 oracle's (the quality bar of the FP32 bulk arithmetic)."""
+    d = D.config_adaptec1()
+    res = {}
+    for prec in (32, 64):
+        r, _ = hip.GpEngine(d, hip.GpConfig(precision=prec, gp_bin_dim=512)).run()
+        assert r["converged"] and r["overflow"] <= 0.10
+        res[prec] = r
+    oracle.set_fast_poisson(True)
+    try:
+        ores, _, _, _ = oracle.eplace_run(d, oracle.default_config(precision=64, gp_bin_dim=512),
+                                          threads=16)
+    finally:
+        oracle.set_fast_poisson(False)
+    print({p: (r["iterations"], r["hpwl"]) for p, r in res.items()}, ores.iterations, ores.hpwl,
+          flush=True)
+    assert ores.overflow <= 0.10 + 1e-6
+    assert abs(res[32]["hpwl"] / res[64]["hpwl"] - 1) < 0.01
+    assert abs(res[32]["hpwl"] / ores.hpwl - 1) < 0.01
+    assert abs(res[32]["iterations"] - ores.iterations) <= 10
