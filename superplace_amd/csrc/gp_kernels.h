@@ -71,7 +71,7 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
-constexpr int kWaMaxBlocks = 1024;   // the WA pass strides its wavefront tasks over at most these
+constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
 // Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each).
 inline int waBlocksFor(int waves) {
     const int b = (waves + kBlock / 64 - 1) / (kBlock / 64);
