@@ -28,8 +28,13 @@
 //  * macros end fixed, flipped ones with mirrored pin x offsets and N<->FN,
 //    S<->FS orientation; per-step summaries go to a metrics sink.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <limits>
 #include <random>
 #include <string>
@@ -260,8 +265,20 @@ public:
             for (int gx = x0; gx <= x1; ++gx)
                 for (int gy = y0; gy <= y1; ++gy) blockCells_[idx(gx, gy)].push_back(static_cast<int>(b));
         }
-        seen_.assign(n, -1);
-        blockSeen_.assign(blocks_.size(), -1);
+        own_ = scratch();
+    }
+
+    // Query scratch of at(): candidate lists and visit stamps.  at() uses the
+    // index's own; a thread querying beside others passes its own to atWith().
+    struct Scratch {
+        std::vector<int> seen, blockSeen, cand, bcand;
+        int tick = 0;
+    };
+    Scratch scratch() const {
+        Scratch sc;
+        sc.seen.assign(w_.size(), -1);
+        sc.blockSeen.assign(blocks_.size(), -1);
+        return sc;
     }
 
     const std::vector<float>& xs() const { return cx_; }
@@ -285,16 +302,17 @@ public:
 
     // Overlap of macro s placed at (px, py) with the other macros and the
     // blockages.
-    double at(int s, float px, float py) const {
+    double at(int s, float px, float py) const { return atWith(own_, s, px, py); }
+    double atWith(Scratch& sc, int s, float px, float py) const {
         const Box a = rect(s, px, py);
         int x0, x1, y0, y1;
         range(a, x0, x1, y0, y1);
-        gatherMacros(x0, x1, y0, y1, s);
+        gatherMacros(sc, x0, x1, y0, y1, s);
         double sum = 0.0;
-        for (int t : cand_) sum += overlapOf(a, rect(t, cx_[t], cy_[t]));
+        for (int t : sc.cand) sum += overlapOf(a, rect(t, cx_[t], cy_[t]));
         if (!blocks_.empty()) {
-            gatherBlocks(x0, x1, y0, y1);
-            for (int b : bcand_) sum += overlapOf(a, blocks_[b]);
+            gatherBlocks(sc, x0, x1, y0, y1);
+            for (int b : sc.bcand) sum += overlapOf(a, blocks_[b]);
         }
         return sum;
     }
@@ -306,8 +324,8 @@ public:
         int x0, x1, y0, y1;
         range(a, x0, x1, y0, y1);
         double best = 0.0;
-        gatherMacros(x0, x1, y0, y1, s);
-        for (int t : cand_) {
+        gatherMacros(own_, x0, x1, y0, y1, s);
+        for (int t : own_.cand) {
             const Box b = rect(t, cx_[t], cy_[t]);
             const double o = overlapOf(a, b);
             if (o > best) {
@@ -316,8 +334,8 @@ public:
             }
         }
         if (!blocks_.empty()) {
-            gatherBlocks(x0, x1, y0, y1);
-            for (int b : bcand_) {
+            gatherBlocks(own_, x0, x1, y0, y1);
+            for (int b : own_.bcand) {
                 const double o = overlapOf(a, blocks_[b]);
                 if (o > best) {
                     best = o;
@@ -333,12 +351,12 @@ public:
         int x0, x1, y0, y1;
         range(r, x0, x1, y0, y1);
         out.clear();
-        ++tick_;
+        const int tick = ++own_.tick;
         for (int gx = x0; gx <= x1; ++gx)
             for (int gy = y0; gy <= y1; ++gy)
                 for (int t : cells_[idx(gx, gy)]) {
-                    if (t == skip || seen_[t] == tick_) continue;
-                    seen_[t] = tick_;
+                    if (t == skip || own_.seen[t] == tick) continue;
+                    own_.seen[t] = tick;
                     if (overlapOf(r, rect(t, cx_[t], cy_[t])) > 0.f) out.push_back(t);
                 }
         std::sort(out.begin(), out.end());
@@ -379,28 +397,28 @@ private:
         y0 = std::clamp(static_cast<int>((r.ly - oy_) / ch_), 0, dim_ - 1);
         y1 = std::clamp(static_cast<int>((r.hy - oy_) / ch_), 0, dim_ - 1);
     }
-    void gatherMacros(int x0, int x1, int y0, int y1, int skip) const {
-        cand_.clear();
-        ++tick_;
+    void gatherMacros(Scratch& sc, int x0, int x1, int y0, int y1, int skip) const {
+        sc.cand.clear();
+        const int tick = ++sc.tick;
         for (int gx = x0; gx <= x1; ++gx)
             for (int gy = y0; gy <= y1; ++gy)
                 for (int t : cells_[idx(gx, gy)]) {
-                    if (t == skip || seen_[t] == tick_) continue;
-                    seen_[t] = tick_;
-                    cand_.push_back(t);
+                    if (t == skip || sc.seen[t] == tick) continue;
+                    sc.seen[t] = tick;
+                    sc.cand.push_back(t);
                 }
-        std::sort(cand_.begin(), cand_.end());
+        std::sort(sc.cand.begin(), sc.cand.end());
     }
-    void gatherBlocks(int x0, int x1, int y0, int y1) const {
-        bcand_.clear();
+    void gatherBlocks(Scratch& sc, int x0, int x1, int y0, int y1) const {
+        sc.bcand.clear();
         for (int gx = x0; gx <= x1; ++gx)
             for (int gy = y0; gy <= y1; ++gy)
                 for (int b : blockCells_[idx(gx, gy)]) {
-                    if (blockSeen_[b] == tick_) continue;
-                    blockSeen_[b] = tick_;
-                    bcand_.push_back(b);
+                    if (sc.blockSeen[b] == sc.tick) continue;
+                    sc.blockSeen[b] = sc.tick;
+                    sc.bcand.push_back(b);
                 }
-        std::sort(bcand_.begin(), bcand_.end());
+        std::sort(sc.bcand.begin(), sc.bcand.end());
     }
     void insert(int s) {
         int x0, x1, y0, y1;
@@ -428,8 +446,7 @@ private:
     int dim_ = 1;
     float ox_ = 0.f, oy_ = 0.f, cw_ = 1.f, ch_ = 1.f;
     std::vector<std::vector<int>> cells_, blockCells_;
-    mutable std::vector<int> seen_, blockSeen_, cand_, bcand_;
-    mutable int tick_ = 0;
+    mutable Scratch own_;
 };
 
 // Snap a macro centre to the site / row grid, clamped into the core.
@@ -449,34 +466,46 @@ struct Grid {
 
 // Positions where the macro touches a nearby rectangle (both sides of each,
 // and the core edges), the start and the current point, within radius of
-// the start (repair_candidates.h:14-45).
-std::vector<std::pair<float, float>> contacts(float sx, float sy, float curX, float curY, float w,
-                                              float h, const Box& core,
-                                              const std::vector<Box>& obstacles, double radius) {
+// the start (repair_candidates.h:14-45): the grid xs x ys, in that order,
+// less the points that contactOk() rejects.
+struct ContactAxes {
+    std::vector<float> xs, ys;
+};
+ContactAxes contactAxes(float sx, float sy, float curX, float curY, float w, float h,
+                        const Box& core, const std::vector<Box>& obstacles, double radius) {
     const float hw = w * .5f, hh = h * .5f;
-    std::vector<float> xs{sx, curX, core.lx + hw, core.hx - hw};
-    std::vector<float> ys{sy, curY, core.ly + hh, core.hy - hh};
+    ContactAxes a;
+    a.xs = {sx, curX, core.lx + hw, core.hx - hw};
+    a.ys = {sy, curY, core.ly + hh, core.hy - hh};
     for (const Box& b : obstacles) {
         if (b.hx < sx - radius - hw || b.lx > sx + radius + hw || b.hy < sy - radius - hh ||
             b.ly > sy + radius + hh)
             continue;
-        xs.push_back(b.lx - hw);
-        xs.push_back(b.hx + hw);
-        ys.push_back(b.ly - hh);
-        ys.push_back(b.hy + hh);
+        a.xs.push_back(b.lx - hw);
+        a.xs.push_back(b.hx + hw);
+        a.ys.push_back(b.ly - hh);
+        a.ys.push_back(b.hy + hh);
     }
-    for (auto* v : {&xs, &ys}) {
+    for (auto* v : {&a.xs, &a.ys}) {
         std::sort(v->begin(), v->end());
         v->erase(std::unique(v->begin(), v->end()), v->end());
     }
+    return a;
+}
+inline bool contactOk(float x, float y, float sx, float sy, float w, float h, const Box& core,
+                      double radius) {
+    const float hw = w * .5f, hh = h * .5f;
+    return !(x < core.lx + hw || x > core.hx - hw || y < core.ly + hh || y > core.hy - hh ||
+             std::hypot(double(x - sx), double(y - sy)) > radius);
+}
+std::vector<std::pair<float, float>> contacts(float sx, float sy, float curX, float curY, float w,
+                                              float h, const Box& core,
+                                              const std::vector<Box>& obstacles, double radius) {
+    const ContactAxes a = contactAxes(sx, sy, curX, curY, w, h, core, obstacles, radius);
     std::vector<std::pair<float, float>> out;
-    for (float x : xs)
-        for (float y : ys) {
-            if (x < core.lx + hw || x > core.hx - hw || y < core.ly + hh || y > core.hy - hh ||
-                std::hypot(double(x - sx), double(y - sy)) > radius)
-                continue;
-            out.emplace_back(x, y);
-        }
+    for (float x : a.xs)
+        for (float y : a.ys)
+            if (contactOk(x, y, sx, sy, w, h, core, radius)) out.emplace_back(x, y);
     return out;
 }
 
@@ -491,6 +520,79 @@ public:
 private:
     std::mt19937 gen_;
 };
+
+// A fixed set of worker threads running f(i) for i in [0, count) with the
+// calling thread joining in; run() returns when every i is done.  Used for
+// the candidate evaluations of the grid repair, which are independent
+// queries of the overlap index (each thread with its own scratch).
+class WorkerPool {
+public:
+    explicit WorkerPool(int threads) {
+        for (int t = 1; t < threads; ++t) workers_.emplace_back([this, t] { loop(t); });
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& w : workers_) w.join();
+    }
+    int size() const { return static_cast<int>(workers_.size()) + 1; }
+    // f(i, thread) for every i; thread in [0, size()).
+    void run(int count, const std::function<void(int, int)>& f) {
+        if (workers_.empty() || count < 2) {
+            for (int i = 0; i < count; ++i) f(i, 0);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &f;
+            count_ = count;
+            next_.store(0);
+            busy_ = static_cast<int>(workers_.size());
+            ++gen_;
+        }
+        cv_.notify_all();
+        work(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void work(int t) {
+        for (int i = next_.fetch_add(1); i < count_; i = next_.fetch_add(1)) (*job_)(i, t);
+    }
+    void loop(int t) {
+        unsigned seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work(t);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int, int)>* job_ = nullptr;
+    int count_ = 0, busy_ = 0;
+    unsigned gen_ = 0;
+    std::atomic<int> next_{0};
+    bool stop_ = false;
+};
+
+// Threads of the grid repair: the host's hardware threads, at most 16.
+inline int repairThreads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::clamp(hw == 0 ? 1u : hw, 1u, 16u));
+}
 
 class MacroLegalizer {
 public:
@@ -573,6 +675,17 @@ void MacroLegalizer::gridRepair(double zeroCap, int& repaired) {
     const Box core{v_.d.core_lx, v_.d.core_ly, v_.d.core_hx, v_.d.core_hy};
     std::vector<size_t> order(nM);
     for (size_t s = 0; s < nM; ++s) order[s] = s;
+    // The candidates of one macro are independent queries of the index: they
+    // are evaluated by all threads into arrays, then scanned in the sequential
+    // order, so the choice is the one a single thread makes.
+    WorkerPool pool(repairThreads());
+    std::vector<OverlapIndex::Scratch> scratch(pool.size());
+    for (auto& sc : scratch) sc = om_.scratch();
+    struct Cand {
+        float x, y;
+        double d, o;
+    };
+    std::vector<Cand> cand;
     for (int pass = 0; pass < 12 && overlap_ > 0.0; ++pass) {
         std::vector<double> ov(nM);
         for (size_t s = 0; s < nM; ++s) ov[s] = om_.at(static_cast<int>(s), cx[s], cy[s]);
@@ -590,45 +703,77 @@ void MacroLegalizer::gridRepair(double zeroCap, int& repaired) {
             const double side = std::max(om_.width(si), om_.height(si));
             const double zeroLimit = zeroCap * side;
             const double maxDisp = std::min(static_cast<double>(2 + pass) * side, zeroLimit);
-            for (int ry = -span; ry <= span; ++ry)
-                for (int rx = -span; rx <= span; ++rx) {
-                    float nx = cx[s] + static_cast<float>(rx) * stepX;
-                    float ny = cy[s] + static_cast<float>(ry) * stepY;
-                    grid_(om_, si, nx, ny);
-                    const double dd = dispOf(si, nx, ny);
-                    const double o = om_.at(si, nx, ny);
-                    if (o <= 0.0 && dd <= zeroLimit) {
-                        if (dd < zDist) {
-                            zDist = dd;
-                            zx = nx;
-                            zy = ny;
-                            haveZero = true;
-                        }
-                    } else if (!haveZero && dd <= maxDisp && o < fOv - 1e-9) {
-                        fOv = o;
-                        fx = nx;
-                        fy = ny;
-                        haveFb = true;
+            const int edge = 2 * span + 1;
+            cand.resize(static_cast<size_t>(edge) * edge);
+            const float sx = cx[s], sy = cy[s];
+            pool.run(edge * edge, [&](int k, int t) {
+                const int ry = k / edge - span, rx = k % edge - span;
+                float nx = sx + static_cast<float>(rx) * stepX;
+                float ny = sy + static_cast<float>(ry) * stepY;
+                grid_(om_, si, nx, ny);
+                cand[k] = Cand{nx, ny, dispOf(si, nx, ny), om_.atWith(scratch[t], si, nx, ny)};
+            });
+            for (const Cand& c : cand) {   // ry outer, rx inner
+                if (c.o <= 0.0 && c.d <= zeroLimit) {
+                    if (c.d < zDist) {
+                        zDist = c.d;
+                        zx = c.x;
+                        zy = c.y;
+                        haveZero = true;
                     }
+                } else if (!haveZero && c.d <= maxDisp && c.o < fOv - 1e-9) {
+                    fOv = c.o;
+                    fx = c.x;
+                    fy = c.y;
+                    haveFb = true;
                 }
+            }
             std::vector<Box> obstacles = om_.blocks();
             for (size_t t = 0; t < nM; ++t)
                 if (t != s) obstacles.push_back(om_.rect(static_cast<int>(t), cx[t], cy[t]));
             const double radius = std::min(zeroLimit, static_cast<double>(2 + pass) * side);
-            for (auto [px, py] : contacts(x0_[s], y0_[s], cx[s], cy[s], om_.width(si),
-                                          om_.height(si), core, obstacles, radius))
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        float nx = px + dx * grid_.site, ny = py + dy * grid_.row;
-                        grid_(om_, si, nx, ny);
-                        const double dd = dispOf(si, nx, ny);
-                        if (dd <= zeroLimit && dd < zDist && om_.at(si, nx, ny) <= 0.0) {
-                            zDist = dd;
-                            zx = nx;
-                            zy = ny;
-                            haveZero = true;
+            // The sequential scan over the contact points (x outer, y inner,
+            // then the 3 x 3 neighbours) keeps the first candidate of least
+            // displacement below zDist with zero overlap: the minimum of
+            // (displacement, position in the scan) over those candidates,
+            // found here per thread over rows of xs and then combined.
+            const float w = om_.width(si), h = om_.height(si), ox = x0_[s], oy = y0_[s];
+            const ContactAxes ax = contactAxes(ox, oy, cx[s], cy[s], w, h, core, obstacles, radius);
+            const long ny9 = static_cast<long>(ax.ys.size()) * 9;
+            struct Best {
+                double d;
+                long at;
+                float x, y;
+            };
+            std::vector<Best> best(pool.size(), Best{zDist, -1, 0.f, 0.f});
+            pool.run(static_cast<int>(ax.xs.size()), [&](int xi, int t) {
+                Best& b = best[t];
+                const float px = ax.xs[xi];
+                for (size_t yi = 0; yi < ax.ys.size(); ++yi) {
+                    const float py = ax.ys[yi];
+                    if (!contactOk(px, py, ox, oy, w, h, core, radius)) continue;
+                    for (int dx = -1; dx <= 1; ++dx)
+                        for (int dy = -1; dy <= 1; ++dy) {
+                            const long at = xi * ny9 + static_cast<long>(yi) * 9 + (dx + 1) * 3 + dy + 1;
+                            float nx = px + dx * grid_.site, ny = py + dy * grid_.row;
+                            grid_(om_, si, nx, ny);
+                            const double dd = dispOf(si, nx, ny);
+                            if (!(dd <= zeroLimit) || !(dd < zDist)) continue;
+                            if (dd > b.d || (dd == b.d && b.at >= 0 && at > b.at)) continue;
+                            if (om_.atWith(scratch[t], si, nx, ny) <= 0.0) b = Best{dd, at, nx, ny};
                         }
-                    }
+                }
+            });
+            const Best* pick = nullptr;
+            for (const Best& b : best)
+                if (b.at >= 0 && (!pick || b.d < pick->d || (b.d == pick->d && b.at < pick->at)))
+                    pick = &b;
+            if (pick) {
+                zDist = pick->d;
+                zx = pick->x;
+                zy = pick->y;
+                haveZero = true;
+            }
             if (haveZero) {
                 om_.move(si, zx, zy);
                 ++repaired;
