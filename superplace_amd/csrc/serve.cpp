@@ -8,9 +8,9 @@
 // graph.  The scheduler keeps kAsyncDepth launches of every running job in
 // flight and tops them up as they complete, so the jobs' kernels overlap on
 // the device; a job that arrives while others run is admitted as soon as a
-// slot and enough of the device budget are free.  The host set-up of the next
-// job (design upload, initial evaluations, graph capture) runs on a worker
-// thread meanwhile.  Each job's result is the same as when it runs alone: an
+// slot and enough of the device budget are free.  The host set-ups of the
+// next jobs (design upload, initial evaluations, graph capture) run on worker
+// threads meanwhile, up to kSetupThreads at once.  Each job's result is the same as when it runs alone: an
 // instance's kernels touch only its own buffers and every kernel is
 // deterministic.
 #include <algorithm>
@@ -75,7 +75,9 @@ struct Job {
     std::unique_ptr<EngineBase> eng;
 };
 
-// What the worker thread hands back after a job's set-up.
+constexpr int kSetupThreads = 4;   // job set-ups in flight at once
+
+// What a set-up thread hands back.
 struct Admission {
     std::unique_ptr<EngineBase> eng;
     bool finished = false;   // the stage ended before its loop
@@ -89,7 +91,7 @@ public:
         : budget_(budget), maxActive_(maxActive), t0_(std::chrono::steady_clock::now()) {}
 
     ~Server() {
-        if (admitting_.valid()) admitting_.wait();
+        for (auto& a : admitting_) a.second.wait();
     }
 
     double now() const {
@@ -118,12 +120,15 @@ public:
     // One scheduling round; returns whether anything changed.
     bool pump() {
         bool progress = false;
-        // a finished set-up joins the running jobs
-        if (admitting_.valid() &&
-            admitting_.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
-            Admission a = admitting_.get();
-            Job& j = *jobs_[admittingId_];
-            admittingId_ = -1;
+        // finished set-ups join the running jobs
+        for (size_t k = 0; k < admitting_.size();) {
+            if (admitting_[k].second.wait_for(std::chrono::seconds(0)) != std::future_status::ready) {
+                ++k;
+                continue;
+            }
+            Admission a = admitting_[k].second.get();
+            Job& j = *jobs_[admitting_[k].first];
+            admitting_.erase(admitting_.begin() + static_cast<long>(k));
             progress = true;
             if (a.status != SP_HIP_OK) {
                 fail(j, a.status, a.error);
@@ -157,48 +162,50 @@ public:
                 ++k;
             }
         }
-        // start the set-up of the next job that has arrived, if it fits
-        if (!admitting_.valid() && !queued_.empty()) {
-            const double t = now();
-            std::stable_sort(queued_.begin(), queued_.end(), [&](int a, int b) {
-                return jobs_[a]->info.arrival_s < jobs_[b]->info.arrival_s;
-            });
+        // start the set-up of the next jobs that have arrived, while they fit
+        // (up to kSetupThreads at once, each on a thread of its own)
+        const double t = now();
+        std::stable_sort(queued_.begin(), queued_.end(), [&](int a, int b) {
+            return jobs_[a]->info.arrival_s < jobs_[b]->info.arrival_s;
+        });
+        while (!queued_.empty() && static_cast<int>(admitting_.size()) < kSetupThreads) {
             Job& j = *jobs_[queued_.front()];
             if (j.info.device_bytes > budget_) {
                 queued_.erase(queued_.begin());
                 fail(j, SP_HIP_EOOM, "job footprint exceeds the device budget");
                 progress = true;
-            } else if (j.info.arrival_s <= t &&
-                       static_cast<int>(running_.size()) < maxActive_ &&
-                       used_ + j.info.device_bytes <= budget_) {
-                queued_.erase(queued_.begin());
-                used_ += j.info.device_bytes;
-                j.info.admit_s = t;
-                j.info.state = SP_HIP_JOB_RUNNING;
-                admittingId_ = j.id;
-                const sp_hip_design* d = &j.design->view;
-                const sp_hip_gp_config cfg = j.cfg;
-                sp_hip_gp_result* res = &j.res;
-                admitting_ = std::async(std::launch::async, [d, cfg, res] {
-                    Admission a;
-                    a.status = guarded([&] {
-                        a.eng = makeEngine(*d, cfg);
-                        a.finished = a.eng->begin(res);
-                    });
-                    if (a.status != SP_HIP_OK) {
-                        a.error = g_lastError;
-                        a.eng.reset();
-                    }
-                    return a;
-                });
-                progress = true;
+                continue;
             }
+            if (!(j.info.arrival_s <= t) ||
+                static_cast<int>(running_.size() + admitting_.size()) >= maxActive_ ||
+                used_ + j.info.device_bytes > budget_)
+                break;
+            queued_.erase(queued_.begin());
+            used_ += j.info.device_bytes;
+            j.info.admit_s = t;
+            j.info.state = SP_HIP_JOB_RUNNING;
+            const sp_hip_design* d = &j.design->view;
+            const sp_hip_gp_config cfg = j.cfg;
+            sp_hip_gp_result* res = &j.res;
+            admitting_.emplace_back(j.id, std::async(std::launch::async, [d, cfg, res] {
+                Admission a;
+                a.status = guarded([&] {
+                    a.eng = makeEngine(*d, cfg);
+                    a.finished = a.eng->begin(res);
+                });
+                if (a.status != SP_HIP_OK) {
+                    a.error = g_lastError;
+                    a.eng.reset();
+                }
+                return a;
+            }));
+            progress = true;
         }
         return progress;
     }
 
     void counts(int32_t* active, int32_t* queued, int32_t* finished) const {
-        if (active) *active = static_cast<int32_t>(running_.size()) + (admittingId_ >= 0 ? 1 : 0);
+        if (active) *active = static_cast<int32_t>(running_.size() + admitting_.size());
         if (queued) *queued = static_cast<int32_t>(queued_.size());
         if (finished) *finished = finished_;
     }
@@ -209,7 +216,7 @@ public:
         while (finished_ < static_cast<int>(jobs_.size())) {
             if (now() > end) throw InvalidArg("drain: time limit reached");
             if (pump()) continue;
-            if (running_.empty() && !admitting_.valid() && !queued_.empty()) {
+            if (running_.empty() && admitting_.empty() && !queued_.empty()) {
                 // idle until the next arrival
                 double next = end;
                 for (int id : queued_) next = std::min(next, jobs_[id]->info.arrival_s);
@@ -301,8 +308,7 @@ private:
     std::chrono::steady_clock::time_point t0_;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::vector<int> queued_, running_;
-    std::future<Admission> admitting_;
-    int admittingId_ = -1;
+    std::vector<std::pair<int, std::future<Admission>>> admitting_;   // (job, set-up)
     int64_t used_ = 0;
     int finished_ = 0;
 };
