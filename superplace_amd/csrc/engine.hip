@@ -21,6 +21,7 @@
 #include "abi.h"
 #include "engine.h"
 #include "wa_device.h"
+#include "density_device.h"
 #include "gp_kernels.h"
 #include "poisson.h"
 
@@ -770,6 +771,12 @@ void Engine<T>::build(const sp_hip_design& d) {
     // finalize: atomic grids read and cleared, windows read, static terrain, rho
     bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + 8 * 2 + ts) + stageBytes;
     bytesPer_[SP_HIP_K_POISSON] = poisson_->bytes(false);
+    if (poisson_->takesDensity()) {
+        // the first pass finalizes the grids: the finalize bytes join the
+        // solve's, less rho's write and read
+        bytesPer_[SP_HIP_K_POISSON] += bytesPer_[SP_HIP_K_FINALIZE] - 2.0 * nb_ * ts;
+        bytesPer_[SP_HIP_K_FINALIZE] = 0;
+    }
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
     // WA net pass: per slot node id, offsets, the node's x/y, entry id and
     // the gradient write; netStart per net
@@ -1109,23 +1116,39 @@ void Engine<T>::launchSlot(bool events, int slot) {
     b(SP_HIP_K_SCATTER);
     launchScatter<T>(dd_, geom_, qscale, nodeQ_, fillQ_, sortSlot, ws_, dCtl_, s_);
     e(SP_HIP_K_SCATTER);
-    b(SP_HIP_K_FINALIZE);
-    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, dCtl_, s_);
-    e(SP_HIP_K_FINALIZE);
+    // On the FFT path the first Poisson pass finalizes the density grids
+    // itself (profiled or not, so the tau partials are the same either way;
+    // the finalize family then has no launches of its own).
+    const DensityIn<T> din{geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_,
+                           std::ldexp(1.0, -fracBits_), binArea_, rhoShift_, ws_.region(0),
+                           nullptr, nullptr};
+    const bool ownFinalize = !poisson_->takesDensity();
+    if (ownFinalize) {
+        b(SP_HIP_K_FINALIZE);
+        launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_,
+                          std::ldexp(1.0, -fracBits_), binArea_, rhoShift_, poisson_->rho(),
+                          nullptr, nullptr, ws_, dCtl_, s_);
+        e(SP_HIP_K_FINALIZE);
+    }
     // Unprofiled slots run the WA pass's blocks inside the Poisson column
     // pass's launch (same blocks, same partials); profiled slots launch the
     // two apart so that each family has its own events.
     bool fused = false;
     if (!events) {
         const int waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
+        int densityBlocks = 0;
         fused = poisson_->solveWithWa(
-            WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws_.region(1), dCtl_}, waBlocks, skip);
-        if (fused) ws_.blocks[1] = waBlocks;
+            WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws_.region(1), dCtl_}, waBlocks, skip,
+            ownFinalize ? nullptr : &din, &densityBlocks);
+        if (fused) {
+            ws_.blocks[1] = waBlocks;
+            if (!ownFinalize) ws_.blocks[0] = densityBlocks;
+        }
     }
     if (!fused) {
         b(SP_HIP_K_POISSON);
-        poisson_->solve(false, skip);
+        if (ownFinalize) poisson_->solve(false, skip);
+        else ws_.blocks[0] = poisson_->solveFromDensity(din, skip);
         e(SP_HIP_K_POISSON);
         b(SP_HIP_K_WA_NETS);
         launchWaNets<T>(dd_, 0.f, cfg_.ignore_net_degree, ws_, dCtl_, s_);

@@ -10,6 +10,7 @@
 
 #include "gp_kernels.h"
 #include "wa_device.h"
+#include "density_device.h"
 
 namespace sphip {
 namespace {
@@ -349,65 +350,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
 }
 
 // ------------------------------------------------------------------ rho + tau
-// Node / filler density of a bin = the atomic grid plus the staged windows
-// that cover it (its own tile's, and a neighbour's for halo bins), all
-// integers, so the sum is exact.  Then rho = (node + filler + terminal +
-// base) / binArea (bin_grid.cpp:361-368) and the tau numerator
-// sum_b max(0, (node + terminal + base)/area - target) area
-// (bin_grid.cpp:373-393, fillers excluded).  Clears the atomic grids and the
-// tile counters for the next evaluation.
+// finalizeBin over every bin (density_device.h), rho into the padded layout,
+// one tau partial per block; clears the tile counters.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_finalize(GridGeom<T> g, TileGrid tg,
-                                                     unsigned long long* nodeQ,
-                                                     unsigned long long* fillQ,
-                                                     const double* terminal, const double* base,
-                                                     double invQ, double binArea, double rhoShift,
-                                                     T* rho, long long* dbgNode,
-                                                     long long* dbgFill, double* partials,
-                                                     const Ctl* ctl) {
+__global__ __launch_bounds__(kBlock) void k_finalize(DensityIn<T> in, T* rho, const Ctl* ctl) {
     if (ctl && ctl->done) return;
-    constexpr int WIN2 = kTileWin * kTileWin;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nb = g.dim * g.dim;
+    const int nb = in.g.dim * in.g.dim;
     double v[1] = {0.0};
-    if (b < tg.nKeys) {
-        tg.count[b] = 0;
-        tg.fill[b] = 0;
-    }
+    clearTileKeys(in.tg, b);
     if (b < nb) {
-        const int bx = b / g.dim, by = b - bx * g.dim;
-        unsigned long long sn = nodeQ[b], sf = fillQ[b];
-        nodeQ[b] = 0;
-        fillQ[b] = 0;
-        const int tx = bx / kTileBins, ty = by / kTileBins;
-        const int ux = bx - tx * kTileBins + kTileHalo, uy = by - ty * kTileBins + kTileHalo;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int sx = ux - dx * kTileBins;
-            if (sx < 0 || sx >= kTileWin || tx + dx < 0 || tx + dx >= tg.tilesX) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int sy = uy - dy * kTileBins;
-                if (sy < 0 || sy >= kTileWin || ty + dy < 0 || ty + dy >= tg.tilesY) continue;
-                const long at = static_cast<long>((tx + dx) * tg.tilesY + ty + dy) * 2 * WIN2 +
-                                sx * kTileWin + sy;
-                sn += tg.stage[at];
-                sf += tg.stage[at + WIN2];
-            }
-        }
-        const long long qn = static_cast<long long>(sn);
-        const long long qf = static_cast<long long>(sf);
-        const double node = static_cast<double>(qn) * invQ;
-        const double fill = static_cast<double>(qf) * invQ;
-        const double tb = terminal[b], bb = base[b];
-        // a constant shift only changes the DC term, which the solve discards;
-        // it keeps the transform inputs small (smaller FP32 rounding)
-        rho[static_cast<long>(bx) * g.ld + by] =
-            static_cast<T>((node + fill + tb + bb) / binArea - rhoShift);
-        const double over = (node + tb + bb) / binArea - static_cast<double>(g.target);
-        v[0] = over > 0.0 ? over * binArea : 0.0;
-        if (dbgNode) dbgNode[b] = qn;
-        if (dbgFill) dbgFill[b] = qf;
+        const T r = finalizeBin<T>(in, b, &v[0]);
+        const int bx = b / in.g.dim, by = b - bx * in.g.dim;
+        rho[static_cast<long>(bx) * in.g.ld + by] = r;
     }
-    blockPartial<1>(v, partials);
+    blockPartial<1>(v, in.partials);
 }
 
 // WaWirelength::compute (wa_wirelength.cpp:119-225) as one pin-parallel pass
@@ -870,9 +827,9 @@ void launchFinalize(const GridGeom<T>& g, const TileGrid& tg, unsigned long long
                     long long* dbgFill, ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
     const long nb = static_cast<long>(g.dim) * g.dim;
     ws.blocks[0] = static_cast<int>(gridFor(std::max<long>(nb, tg.nKeys)));
-    launchKernel(k_finalize<T>, ws.blocks[0], kBlock, 0, s, g, tg, nodeQ, fillQ, terminal, base, invQ,
-                                                 binArea, rhoShift, rho, dbgNode, dbgFill,
-                                                 ws.region(0), ctl);
+    const DensityIn<T> in{g, tg, nodeQ, fillQ, terminal, base, invQ, binArea, rhoShift,
+                          ws.region(0), dbgNode, dbgFill};
+    launchKernel(k_finalize<T>, ws.blocks[0], kBlock, 0, s, in, rho, ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 

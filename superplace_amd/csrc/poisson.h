@@ -22,6 +22,8 @@ namespace sphip {
 
 template <typename T>
 struct WaTask;
+template <typename T>
+struct DensityIn;
 
 // LDS bound of the column pass that carries the WA blocks (all blocks of a
 // launch reserve it, so a larger one starves the WA blocks of occupancy).
@@ -46,7 +48,15 @@ public:
     // column pass's launch (the same partials as launching the WA pass
     // alone with waBlocks blocks).  Returns false, enqueuing nothing, where
     // the grid does not allow it (dense path, or LDS above kFusedLdsMax).
-    bool solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip);
+    // With din, the first pass finalizes the density grids itself (rho is
+    // then not written): one tau partial per block of that pass into
+    // din->partials, their count into *densityBlocks.
+    bool solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip,
+                     const DensityIn<T>* din = nullptr, int* densityBlocks = nullptr);
+    // solve(false, skip) from the integer density grids (takesDensity()
+    // plans only); returns the tau partial count.
+    int solveFromDensity(const DensityIn<T>& din, const int* skip);
+    bool takesDensity() const;
 
     int nx() const { return nx_; }
     int ny() const { return ny_; }
@@ -86,6 +96,8 @@ private:
     int colPairs_ = 1;   // column pairs per block of the column pass
     void setFftLds();
     void solveFft(bool withPhi, const int* skip);
+    int passA(const DensityIn<T>* din, const int* skip);   // returns the tau partial count
+    void passBC(const int* skip, bool withPhi = false);
 };
 
 // Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on

@@ -11,6 +11,7 @@
 
 #include "poisson.h"
 #include "wa_device.h"
+#include "density_device.h"
 
 namespace sphip {
 namespace {
@@ -481,6 +482,89 @@ __device__ inline void dctRowsBlock(const T* __restrict__ rho, T* __restrict__ o
     (void)nx;
 }
 
+// Pass A reading the integer density grids instead of rho, one row pair
+// per block of kRhoThreads threads: the block's threads finalize the pair's
+// bins (finalizeBin: densities, rho, tau term, clearing) straight into the
+// transform input, kRhoThreads consecutive bins per round, then the first
+// fftThreads(ny) threads run the transform.  The launch clears the tile
+// counters.
+constexpr int kRhoThreads = 2 * kBlock;
+constexpr int kRhoRounds = 4;   // rows of at most 1024 bins
+static_assert(kRhoThreads == 2 * kBlock, "a block is two finalize blocks wide");
+
+// The tau partials of the fused pass.  Round r's bins are 2 kBlock
+// consecutive bins; each half of the block reduces its kBlock bins with the
+// finalize kernel's tree (the same wave shuffles, waves summed in the same
+// order), so partial first + 2 r + half holds the bits the finalize kernel
+// writes for the same bins.
+__device__ inline void halfBlockPartials(double (&v)[kRhoRounds], int rounds, double* partials,
+                                         long first) {
+    constexpr int W = kBlock / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < kRhoRounds; ++j) v[j] += __shfl_down(v[j], off, 64);
+    __shared__ double sh[kRhoThreads / 64][kRhoRounds];
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < kRhoRounds; ++j) sh[wave][j] = v[j];
+    __syncthreads();
+    const int half = threadIdx.x / kBlock;
+    if (threadIdx.x % kBlock == 0)
+        for (int j = 0; j < rounds; ++j) {
+            double s = 0;
+            for (int w = 0; w < W; ++w) s += sh[half * W + w][j];
+            partials[first + 2 * j + half] = s;
+        }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRhoThreads) void k_dct_rows_rho(DensityIn<T> in, T* __restrict__ out,
+                                                              int nx, int ny, int ld,
+                                                              FftTab<T> tab, const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int L = ny, nt = fftThreads(L), rounds = 2 * L / kRhoThreads;
+    const FftLds<T> f = fftLds<T>(smem, L);
+    cplx<T>* a[1] = {f.waves};
+    cplx<T>* b[1] = {f.waves + L};
+    loadTables<T>(f, tab.tw, tab.hf, nullptr, L);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < in.tg.nKeys; t += gridDim.x * blockDim.x)
+        clearTileKeys(in.tg, t);
+    const int r0 = 2 * blockIdx.x;
+    double over[kRhoRounds] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kRhoRounds; ++j) {
+        if (j < rounds) {
+            const int e = j * kRhoThreads + threadIdx.x;
+            const int row = e / L, i = e - row * L;
+            const T val = finalizeBin<T>(in, r0 * L + e, &over[j]);
+            reinterpret_cast<T*>(a[0] + dctPos(i, L))[row] = val;
+        }
+    }
+    __syncthreads();
+    // threads past the team take no butterflies (their index is past every
+    // stage's count) but meet every barrier
+    const int tid = static_cast<int>(threadIdx.x) < nt ? static_cast<int>(threadIdx.x) : L;
+    blockFft<T, false, 1>(a, b, f.tw, L, tid, nt);
+    const cplx<T>* Z = a[0];
+    T* oa = out + static_cast<long>(r0) * ld;
+    T* ob = oa + ld;
+    for (int k = tid; k <= L / 2; k += nt) {
+        T ra, rb, raM, rbM;
+        dct2Pair<T>(Z, f.hf, k, L, &ra, &rb, &raM, &rbM);
+        oa[k] = ra;
+        ob[k] = rb;
+        if (k > 0 && k < L / 2) {
+            oa[L - k] = raM;
+            ob[L - k] = rbM;
+        }
+    }
+    halfBlockPartials(over, rounds, in.partials, static_cast<long>(blockIdx.x) * 2 * rounds);
+    (void)nx;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dct_rows(const T* __restrict__ rho, T* __restrict__ out,
                                                      int nx, int ny, int ld, FftTab<T> tab,
@@ -944,6 +1028,8 @@ void PoissonPlan<T>::setFftLds() {
     }
     SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows_rho<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
     SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, ldsB_));
     SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_idct_rows<T>),
@@ -952,13 +1038,17 @@ void PoissonPlan<T>::setFftLds() {
 
 template <typename T>
 void PoissonPlan<T>::solveFft(bool withPhi, const int* skip) {
+    passA(nullptr, skip);
+    passBC(skip, withPhi);
+}
+
+// Passes B and C of the FFT path.
+template <typename T>
+void PoissonPlan<T>::passBC(const int* skip, bool withPhi) {
     using C2 = cplx<T>;
     const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
     const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
     const int ld = my_;
-    launchKernel(k_dct_rows<T>, dim3(nx_ / 2), dim3(fftThreads(ny_)),
-                 static_cast<uint32_t>(ldsA_), stream_, rho_, t0_, nx_, ny_, ld, ty, skip);
-    SPHIP_CHECK(hipGetLastError());
     launchKernel(k_dct_cols<T>, dim3(ny_ / (2 * colPairs_)), dim3(colPairs_ * fftThreads(nx_)),
                  static_cast<uint32_t>(ldsB_), stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx,
                  u_, v_, invNorm_, skip);
@@ -973,15 +1063,48 @@ void PoissonPlan<T>::solveFft(bool withPhi, const int* skip) {
 }
 
 template <typename T>
-bool PoissonPlan<T>::solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip) {
-    if (!fft_ || ldsBW_ == 0) return false;
+int PoissonPlan<T>::passA(const DensityIn<T>* din, const int* skip) {
+    using C2 = cplx<T>;
+    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    if (din) {
+        launchKernel(k_dct_rows_rho<T>, dim3(nx_ / 2), dim3(kRhoThreads),
+                     static_cast<uint32_t>(ldsA_), stream_, *din, t0_, nx_, ny_, my_, ty, skip);
+        SPHIP_CHECK(hipGetLastError());
+        return (nx_ / 2) * (2 * ny_ / kBlock);
+    }
+    launchKernel(k_dct_rows<T>, dim3(nx_ / 2), dim3(fftThreads(ny_)),
+                 static_cast<uint32_t>(ldsA_), stream_, rho_, t0_, nx_, ny_, my_, ty, skip);
+    SPHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+bool PoissonPlan<T>::takesDensity() const {
+    // the bins of a row pair split evenly into the fused pass's rounds, and
+    // the transform team fits in its block
+    return fft_ && nx_ == ny_ && (2 * ny_) % kRhoThreads == 0 &&
+           2 * ny_ <= kRhoRounds * kRhoThreads && fftThreads(ny_) <= kRhoThreads;
+}
+
+template <typename T>
+int PoissonPlan<T>::solveFromDensity(const DensityIn<T>& din, const int* skip) {
+    if (!takesDensity() || din.g.dim != ny_)
+        throw std::logic_error("this plan does not take the density grids");
+    const int blocks = passA(&din, skip);
+    passBC(skip);
+    return blocks;
+}
+
+template <typename T>
+bool PoissonPlan<T>::solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip,
+                                 const DensityIn<T>* din, int* densityBlocks) {
+    if (!fft_ || ldsBW_ == 0 || (din && (!takesDensity() || din->g.dim != ny_))) return false;
     using C2 = cplx<T>;
     const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
     const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
     const int ld = my_;
-    launchKernel(k_dct_rows<T>, dim3(nx_ / 2), dim3(fftThreads(ny_)),
-                 static_cast<uint32_t>(ldsA_), stream_, rho_, t0_, nx_, ny_, ld, ty, skip);
-    SPHIP_CHECK(hipGetLastError());
+    const int blocksA = passA(din, skip);
+    if (densityBlocks) *densityBlocks = blocksA;
     const int G = kBlock / fftThreads(nx_), colBlocks = ny_ / (2 * G);
     launchKernel(k_dct_cols_wa<T>, dim3(colBlocks + waBlocks), dim3(kBlock),
                  static_cast<uint32_t>(ldsBW_), stream_, t0_, t1_, t2_, G, nx_, ny_, ld, tx, u_,
