@@ -706,6 +706,17 @@ __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc
     }
 }
 
+// MI355X deals a launch's workgroups to its 8 XCDs in turn (workgroup b to
+// XCD b % 8), and each XCD has its own L2.  Pass B's blocks own adjacent
+// column groups whose row segments share cache lines; logical block
+// (b % 8) (n / 8) + b / 8 gives each XCD n / 8 adjacent groups, so the
+// segments of one line meet in one L2 instead of eight.  Bijective on
+// [0, n) when 8 divides n; the identity otherwise.
+constexpr int kXcds = 8;
+__device__ inline int xcdBlock(int b, int n) {
+    return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
                                                    T* __restrict__ Hs, int G, int nx, int ny,
@@ -713,7 +724,8 @@ __global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* 
                                                    const T* __restrict__ v, T invNorm,
                                                    const int* skip) {
     if (skip && *skip) return;
-    dctColsBlock<T, true>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm, blockIdx.x);
+    dctColsBlock<T, true>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm,
+                          xcdBlock(blockIdx.x, gridDim.x));
 }
 
 // Pass B with the WA pass's blocks in the same launch: blocks below
@@ -732,7 +744,8 @@ __global__ __launch_bounds__(kBlock) void k_dct_cols_wa(const T* __restrict__ F1
                                                         int colBlocks, int waFirst, int waTotal) {
     if (skip && *skip) return;
     if (static_cast<int>(blockIdx.x) < colBlocks)
-        dctColsBlock<T, false>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm, blockIdx.x);
+        dctColsBlock<T, false>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm,
+                               xcdBlock(blockIdx.x, colBlocks));
     else
         waBlockTask<T>(wa, waFirst + (blockIdx.x - colBlocks), waTotal);
 }
