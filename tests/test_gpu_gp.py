@@ -239,6 +239,29 @@ def test_wa_in_column_pass_is_bitwise_neutral(name, prec):
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
 
 
+@pytest.mark.parametrize("bins", [256, 512])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_first_pass_finalize_matches_finalize_kernel(bins, prec):
+    """On 256- to 1024-bin grids the device loop's first Poisson pass
+    finalizes the density grids itself; host-driven evaluations (loop mode 0)
+    launch the finalize kernel and then the plain transforms.  The first
+    iteration evaluates the same start in both, so its overflow (the tau
+    partials: same bins per partial, same summation tree) and HPWL agree bit
+    for bit; the grids it clears leave the rest of the run on the same
+    course."""
+    d = designs()["synth2k"]
+    host, rows_h = hip.GpEngine(d, hip.GpConfig(precision=prec, gp_bin_dim=bins, loop_mode=0,
+                                                 gp_max_iter=40)).run()
+    dev, rows_d = hip.GpEngine(d, hip.GpConfig(precision=prec, gp_bin_dim=bins, loop_mode=2,
+                                                gp_max_iter=40)).run()
+    print(bins, prec, rows_h[0]["overflow"], rows_d[0]["overflow"], flush=True)
+    assert rows_d[0]["overflow"] == rows_h[0]["overflow"]
+    assert rows_d[0]["hpwl"] == rows_h[0]["hpwl"]
+    for a, b in zip(rows_d[:10], rows_h[:10]):
+        assert a["overflow"] == pytest.approx(b["overflow"], rel=1e-4, abs=1e-6)
+    assert abs(len(rows_d) - len(rows_h)) <= 2
+
+
 def test_benchmark_design_quality():
     """The benchmark workload (211k cells, 512x512 bins) run to overflow
     0.10: the FP32 path's final HPWL within 1% of the FP64 GPU path's and of
