@@ -490,7 +490,6 @@ __device__ inline void dctRowsBlock(const T* __restrict__ rho, T* __restrict__ o
 // counters.
 constexpr int kRhoThreads = 2 * kBlock;
 constexpr int kRhoRounds = 4;   // rows of at most 1024 bins
-static_assert(kRhoThreads == 2 * kBlock, "a block is two finalize blocks wide");
 
 // The tau partials of the fused pass.  Round r's bins are 2 kBlock
 // consecutive bins; each half of the block reduces its kBlock bins with the
