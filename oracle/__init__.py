@@ -302,6 +302,39 @@ def lambda_multiplier(dh, ref):
     return lib().ora_lambda_multiplier(float(dh), float(ref))
 
 
+def hpwl_reference(relative, delta_ref, delta_relative, initial_hpwl, filler_only):
+    """The lambda schedule's HPWL scale, or None where the reference throws."""
+    L = lib()
+    L.ora_hpwl_reference.argtypes = [C.c_int, C.c_float, C.c_double, C.c_double, C.c_int, _P]
+    out = C.c_double(0)
+    rc = L.ora_hpwl_reference(int(relative), float(delta_ref), float(delta_relative),
+                              float(initial_hpwl), int(filler_only), C.byref(out))
+    return None if rc else out.value
+
+
+def ref_compute_gamma(tau, step):
+    L = ref_lib()
+    L.ref_compute_gamma.restype = C.c_float
+    L.ref_compute_gamma.argtypes = [C.c_float, C.c_float]
+    return L.ref_compute_gamma(float(tau), float(step))
+
+
+def ref_lambda_multiplier(dh, ref):
+    L = ref_lib()
+    L.ref_lambda_multiplier.restype = C.c_float
+    L.ref_lambda_multiplier.argtypes = [C.c_double, C.c_double]
+    return L.ref_lambda_multiplier(float(dh), float(ref))
+
+
+def ref_hpwl_reference(relative, delta_ref, delta_relative, initial_hpwl, filler_only):
+    L = ref_lib()
+    L.ref_hpwl_reference.argtypes = [C.c_int, C.c_float, C.c_double, C.c_double, C.c_int, _P]
+    out = C.c_double(0)
+    rc = L.ref_hpwl_reference(int(relative), float(delta_ref), float(delta_relative),
+                              float(initial_hpwl), int(filler_only), C.byref(out))
+    return None if rc else out.value
+
+
 def filler_plan(d, target_density=1.0):
     ds = design_struct(d)
     out = np.zeros(6)

@@ -21,6 +21,7 @@
 #include "lg/macro_sa.h"
 #include "io/bookshelf_reader.h"
 #include "io/bookshelf_writer.h"
+#include "gp/lambda_schedule.h"
 #include "gp/wa_wirelength.h"
 #include "numeric/optimizer.h"
 #include "util/config.h"
@@ -81,6 +82,26 @@ double ref_wa(int numMovable, int numNodes, int numPins, int numNets, const floa
     double wl = 0.0;
     wa.compute(db, gamma, ignoreDeg, &wl, grad);
     return wl;
+}
+
+// computeGamma (src/gp/wa_wirelength.cpp:18-29).
+float ref_compute_gamma(float tau, float binStep) { return sp::computeGamma(tau, binStep); }
+
+// lambdaMultiplier (src/gp/lambda_schedule.h:22-25).
+float ref_lambda_multiplier(double deltaHpwl, double reference) {
+    return sp::lambdaMultiplier(deltaHpwl, reference);
+}
+
+// hpwlReference (src/gp/lambda_schedule.h:11-20) for a Config with the given
+// lambda_schedule ("relative" when relative != 0), delta_hpwl_ref and
+// delta_hpwl_relative; returns 1 when the reference throws.
+int ref_hpwl_reference(int relative, float deltaRef, double deltaRelative, double initialHpwl,
+                       int fillerOnly, double* out) {
+    sp::Config cfg;
+    cfg.lambda_schedule = relative ? "relative" : "fixed";
+    cfg.delta_hpwl_ref = deltaRef;
+    cfg.delta_hpwl_relative = deltaRelative;
+    return refGuard([&] { *out = sp::hpwlReference(cfg, initialHpwl, fillerOnly != 0); });
 }
 
 // computeHPWL on the given netlist.

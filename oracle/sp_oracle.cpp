@@ -89,9 +89,8 @@ struct Footprint {
 };
 
 // BinGrid::footprintOf, bin_grid.cpp:130-162 (local smoothing).
-template <typename T>
-Footprint footprintOf(const ora_grid* g, const ora_design* d, const float* x, const float* y,
-                      int i) {
+template <typename T, typename C>
+Footprint footprintOf(const ora_grid* g, const ora_design* d, const C* x, const C* y, int i) {
     const T w = d->node_w[i], h = d->node_h[i];
     const T cx = x[i], cy = y[i];
     const T stepX = T(g->stepX), stepY = T(g->stepY);
@@ -130,9 +129,9 @@ T chargeWeight(const ora_grid* g, const ora_design* d, int i, T scale) {
 
 // Visits every bin overlapped by node i's smoothed footprint with its charge
 // q = overlapArea(bin, rect) * w (bin_grid.cpp:323-339 and :403-421).
-template <typename T, typename V>
-void forEachBinCharge(const ora_grid* g, const ora_design* d, const float* x, const float* y,
-                      int i, V&& visit) {
+template <typename T, typename C, typename V>
+void forEachBinCharge(const ora_grid* g, const ora_design* d, const C* x, const C* y, int i,
+                      V&& visit) {
     const Footprint f = footprintOf<T>(g, d, x, y, i);
     const T w = chargeWeight<T>(g, d, i, T(f.scale));
     const T stepX = T(g->stepX), stepY = T(g->stepY);
@@ -244,9 +243,9 @@ void gridInit(ora_grid* g, const ora_design* d, const float* x, const float* y, 
 
 // BinGrid::accumulate, bin_grid.cpp:299-359: 8 fixed chunks of local grids,
 // then a per-bin sum in chunk order (FP32); FP64 mode sums in double.
-template <typename T>
-void accumulate(const ora_grid* g, const ora_design* d, const float* x, const float* y,
-                double* nodeD, double* fillD) {
+template <typename T, typename C>
+void accumulate(const ora_grid* g, const ora_design* d, const C* x, const C* y, double* nodeD,
+                double* fillD) {
     const size_t nb = static_cast<size_t>(g->dim) * g->dim;
     const int nc = 8;
     const int total = totalNodes(d);
@@ -506,9 +505,9 @@ void poisson(int nx, int ny, double stepX, double stepY, const double* rho, doub
 // ---------------------------------------------------------------------------
 // WaWirelength::compute, wa_wirelength.cpp:92-226 (two passes, max-shifted
 // exponentials, centred numerators).
-template <typename T>
-double waCompute(const ora_design* d, const float* x, const float* y, float gammaIn,
-                 int ignoreDeg, double* grad) {
+template <typename T, typename C>
+double waCompute(const ora_design* d, const C* x, const C* y, float gammaIn, int ignoreDeg,
+                 double* grad) {
     const int numNets = d->num_nets, numMov = d->num_movable;
     const T g = std::max(T(gammaIn), T(1e-4f));
     const T invG = T(1) / g;
@@ -582,8 +581,8 @@ double waCompute(const ora_design* d, const float* x, const float* y, float gamm
 }
 
 // computeHPWL, src/db/hpwl.cpp:13-39 (degree < 2 skipped, large nets kept).
-template <typename T>
-double hpwl(const ora_design* d, const float* x, const float* y) {
+template <typename T, typename C>
+double hpwl(const ora_design* d, const C* x, const C* y) {
     return chunkedSum(d->num_nets, [&](int64_t kk) -> double {
         const int k = static_cast<int>(kk);
         const int b = d->net2pin_start[k], e = d->net2pin_start[k + 1];
@@ -615,9 +614,9 @@ double overflowOf(const ora_grid* g, const double* nodeD) {
     return over / g->movableAreaScaled;
 }
 
-template <typename T>
-void gather(const ora_grid* g, const ora_design* d, const float* x, const float* y,
-            const double* ex, const double* ey, double* force) {
+template <typename T, typename C>
+void gather(const ora_grid* g, const ora_design* d, const C* x, const C* y, const double* ex,
+            const double* ey, double* force) {
     const int total = totalNodes(d);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < total; ++i) {
@@ -635,12 +634,29 @@ void gather(const ora_grid* g, const ora_design* d, const float* x, const float*
 }
 
 float computeGamma(float tau, float binStep) {
-    // computeGamma, wa_wirelength.cpp:64-75.
+    // computeGamma, src/gp/wa_wirelength.cpp:18-29.
     float gamma = 8.0f * binStep;
     if (tau > 1.0f) gamma *= 10.0f;
     else if (tau < 0.1f) gamma *= 0.1f;
     else gamma *= std::pow(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
     return std::max(gamma, 1e-4f);
+}
+
+// hpwlReference, src/gp/lambda_schedule.h:11-20: the HPWL scale of the lambda
+// schedule; returns false where the reference throws (a non-positive or
+// non-finite reference).  A zero initial HPWL under the relative schedule
+// returns delta_hpwl_ref before that check, whatever its value.
+bool hpwlReference(int relative, float deltaRef, double deltaRelative, double initialHpwl,
+                   bool fillerOnly, double* out) {
+    const bool rel = relative != 0 && !fillerOnly;
+    if (rel && initialHpwl == 0.0) {
+        *out = deltaRef;
+        return true;
+    }
+    const double ref = rel ? initialHpwl * deltaRelative : static_cast<double>(deltaRef);
+    if (!(ref > 0.0) || !std::isfinite(ref)) return false;
+    *out = ref;
+    return true;
 }
 
 float lambdaMultiplier(double dHpwl, double ref) {
@@ -825,7 +841,9 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
     const size_t nb = static_cast<size_t>(dim) * dim;
     std::vector<double> nodeD(nb), fillD(nb), rho(nb), phi(nb), ex(nb), ey(nb);
     std::vector<double> force(2 * static_cast<size_t>(total)), gradWl(2 * static_cast<size_t>(nMov));
-    std::vector<float> cx(x, x + total), cy(y, y + total);   // working db coordinates
+    // working db coordinates, at the arithmetic precision (the reference keeps
+    // FP32 coordinates, PlaceDB::node_x; the FP64 path keeps FP64 ones)
+    std::vector<T> cx(x, x + total), cy(y, y + total);
     const float stepX = static_cast<float>(grid.stepX);
     const double binArea = grid.stepX * grid.stepY;
 
@@ -844,8 +862,8 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
     };
     auto scatterPos = [&](const T* p) {
         for (int k = 0; k < n; ++k) {
-            cx[optIdx[k]] = static_cast<float>(p[k]);
-            cy[optIdx[k]] = static_cast<float>(p[n + k]);
+            cx[optIdx[k]] = p[k];
+            cy[optIdx[k]] = p[n + k];
         }
     };
 
@@ -921,10 +939,10 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
     }
     const float targetTau = stage == 2 ? cfg->cgp_target_overflow : cfg->target_overflow;
     const int maxIter = stage == 1 ? cfg->filler_only_iters : cfg->gp_max_iter;
-    // hpwlReference, lambda_schedule.h:11-20.
-    double effRef = (cfg->lambda_relative && stage != 1) ? hp * cfg->delta_hpwl_relative
-                                                         : cfg->delta_hpwl_ref;
-    if (hp == 0.0 && cfg->lambda_relative && stage != 1) effRef = cfg->delta_hpwl_ref;
+    double effRef = 0.0;
+    if (!hpwlReference(cfg->lambda_relative, cfg->delta_hpwl_ref, cfg->delta_hpwl_relative, hp,
+                       stage == 1, &effRef))
+        return 1;
     res->initial_hpwl = hp;
     res->initial_overflow = tau;
     res->initial_lambda = lambda;
@@ -937,8 +955,8 @@ int eplaceRun(const ora_design* d, float* x, float* y, const ora_gp_config* cfg,
         res->overflow = static_cast<float>(overflowOf(&grid, nodeD.data()));
         res->hpwl = hpwl<T>(d, cx.data(), cy.data());
         for (int k = 0; k < n; ++k) {
-            x[optIdx[k]] = cx[optIdx[k]];
-            y[optIdx[k]] = cy[optIdx[k]];
+            x[optIdx[k]] = static_cast<float>(cx[optIdx[k]]);
+            y[optIdx[k]] = static_cast<float>(cy[optIdx[k]]);
         }
     };
 
@@ -1203,6 +1221,11 @@ double ora_hpwl(const ora_design* d, const float* x, const float* y, int precisi
 }
 
 float ora_lambda_multiplier(double dH, double ref) { return lambdaMultiplier(dH, ref); }
+int ora_hpwl_reference(int relative, float delta_ref, double delta_relative, double initial_hpwl,
+                       int filler_only, double* out) {
+    return hpwlReference(relative, delta_ref, delta_relative, initial_hpwl, filler_only != 0, out)
+               ? 0 : 1;
+}
 
 void ora_filler_plan(const ora_design* d, const float* x, const float* y, float target,
                      float rowHeight, double totalRowArea, double* out) {

@@ -100,7 +100,7 @@ void ora_r2r_1d_fast(int kind, int n, const double* in, double* out);
 void ora_set_fast_poisson(int on);
 
 /* ---- wirelength ------------------------------------------------------------ */
-/* computeGamma, src/gp/wa_wirelength.cpp:64-75 (FP32 as in the reference). */
+/* computeGamma, src/gp/wa_wirelength.cpp:18-29 (FP32 as in the reference). */
 float ora_compute_gamma(float tau, float bin_step);
 /* WaWirelength::compute, wa_wirelength.cpp:92-226.  grad has 2*num_movable. */
 double ora_wa(const ora_design* d, const float* x, const float* y, float gamma,
@@ -110,6 +110,10 @@ double ora_hpwl(const ora_design* d, const float* x, const float* y, int precisi
 
 /* ---- schedules (src/gp/lambda_schedule.h:11-25) ----------------------------- */
 float ora_lambda_multiplier(double delta_hpwl, double reference);
+/* hpwlReference: the lambda schedule's HPWL scale; returns 1 where the
+ * reference throws (non-positive or non-finite). */
+int ora_hpwl_reference(int relative, float delta_ref, double delta_relative, double initial_hpwl,
+                       int filler_only, double* out);
 
 /* ---- fillers (src/gp/filler.cpp:47-139) ------------------------------------- */
 /* out: count, width, height, whitespace area, node area scaled, filler area */

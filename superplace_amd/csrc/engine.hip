@@ -46,12 +46,21 @@ T* dupload(const std::vector<S>& h, hipStream_t s) {
     return d;
 }
 
-// computeGamma, src/gp/wa_wirelength.cpp:64-75.
+// computeGamma, src/gp/wa_wirelength.cpp:18-29, with the power taken as 10^e
+// in FP64 rounded once to FP32 (the correctly rounded value), as the device
+// loop's gammaOf takes it, so that host and device decisions agree bit for
+// bit.  (The reference's powf rounds differently for about 0.07% of
+// exponents.)
 float computeGamma(float tau, float binStep) {
     float gamma = 8.0f * binStep;
-    if (tau > 1.0f) gamma *= 10.0f;
-    else if (tau < 0.1f) gamma *= 0.1f;
-    else gamma *= std::pow(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
+    if (tau > 1.0f) {
+        gamma *= 10.0f;
+    } else if (tau < 0.1f) {
+        gamma *= 0.1f;
+    } else {
+        const float e = (20.0f / 9.0f) * (tau - 0.1f) - 1.0f;
+        gamma *= static_cast<float>(std::pow(10.0, static_cast<double>(e)));
+    }
     return std::max(gamma, 1e-4f);
 }
 
@@ -887,10 +896,14 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
     res->initial_lambda = lambda;
     const float targetTau = stage == 2 ? cfg_.cgp_target_overflow : cfg_.target_overflow;
     const int maxIter = stage == 1 ? cfg_.filler_only_iters : cfg_.gp_max_iter;
-    double effRef = (cfg_.lambda_relative && stage != 1) ? hpwl * cfg_.delta_hpwl_relative
-                                                         : static_cast<double>(cfg_.delta_hpwl_ref);
-    if (hpwl == 0.0 && cfg_.lambda_relative && stage != 1) effRef = cfg_.delta_hpwl_ref;
-    if (!(effRef > 0.0) || !std::isfinite(effRef))
+    // hpwlReference (src/gp/lambda_schedule.h:11-20): a zero HPWL under the
+    // relative schedule takes delta_hpwl_ref as it is; otherwise the scale must
+    // be positive and finite.
+    const bool relSched = cfg_.lambda_relative && stage != 1;
+    double effRef = relSched ? hpwl * cfg_.delta_hpwl_relative
+                             : static_cast<double>(cfg_.delta_hpwl_ref);
+    if (relSched && hpwl == 0.0) effRef = cfg_.delta_hpwl_ref;
+    else if (!(effRef > 0.0) || !std::isfinite(effRef))
         throw InvalidArg("invalid effective HPWL reference");
 
     auto deliver = [&](const T* pos) {

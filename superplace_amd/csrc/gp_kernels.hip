@@ -487,16 +487,24 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
 }
 
 // ------------------------------------------------------------------ (d) decisions
-// computeGamma (wa_wirelength.cpp:64-75) and lambdaMultiplier (lambda_schedule.h:22-25).
+// computeGamma (wa_wirelength.cpp:18-29) and lambdaMultiplier (lambda_schedule.h:22-25).
+// The FP32 arithmetic is not contracted (the reference's is not), and the
+// power is 10^e in FP64 rounded once to FP32: the correctly rounded value,
+// which the host loop (pow10f, engine.hip) takes too.
 __device__ inline float gammaOf(float tau, float binStep) {
+#pragma clang fp contract(off)
     float gamma = 8.0f * binStep;
     if (tau > 1.0f) gamma *= 10.0f;
     else if (tau < 0.1f) gamma *= 0.1f;
-    else gamma *= powf(10.0f, (20.0f / 9.0f) * (tau - 0.1f) - 1.0f);
+    else {
+        const float e = (20.0f / 9.0f) * (tau - 0.1f) - 1.0f;
+        gamma *= static_cast<float>(pow(10.0, static_cast<double>(e)));
+    }
     return fmaxf(gamma, 1e-4f);
 }
 
 __device__ inline float lambdaMul(double dH, double ref) {
+#pragma clang fp contract(off)
     const double mu = pow(1.1, -dH / ref + 1.0);
     return static_cast<float>(fmin(1.05, fmax(0.95, mu)));
 }
@@ -511,6 +519,7 @@ __device__ inline float lambdaMul(double dH, double ref) {
 // (nesterov.cpp:109-118).
 template <typename T>
 __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
+#pragma clang fp contract(off)
     for (int r = 0; r < B_ROLES; ++r) c.cur[r] = c.nxt[r];
     c.snapWrite = -1;
     const double tau = sc[0], wl = sc[1], hpwl = sc[2], sumWl = sc[3], sumDen = sc[4];

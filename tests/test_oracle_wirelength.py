@@ -111,21 +111,6 @@ def test_large_nets_skipped_in_wa_not_hpwl():
     assert oracle.hpwl(d) == pytest.approx(980.0)
 
 
-def test_schedules_closed_forms():
-    # computeGamma: 8*wb*10^((20/9)(tau-0.1)-1), x10 above 1, x0.1 below 0.1
-    wb = 20.0
-    assert oracle.compute_gamma(1.5, wb) == pytest.approx(8 * wb * 10, rel=1e-6)
-    assert oracle.compute_gamma(0.05, wb) == pytest.approx(8 * wb * 0.1, rel=1e-6)
-    assert oracle.compute_gamma(0.5, wb) == pytest.approx(
-        8 * wb * 10 ** ((20 / 9) * 0.4 - 1), rel=1e-6)
-    assert oracle.compute_gamma(1.5, wb) > oracle.compute_gamma(0.5, wb) > oracle.compute_gamma(0.05, wb)
-    # lambdaMultiplier: clamp(1.1^(1 - dH/ref), 0.95, 1.05)
-    assert oracle.lambda_multiplier(0.0, 3.5e5) == pytest.approx(1.05)
-    assert oracle.lambda_multiplier(3.5e5, 3.5e5) == pytest.approx(1.0)
-    assert oracle.lambda_multiplier(1e9, 3.5e5) == pytest.approx(0.95)
-    assert oracle.lambda_multiplier(1.5 * 3.5e5, 3.5e5) == pytest.approx(1.1 ** -0.5, rel=1e-6)
-
-
 def test_filler_plan_matches_product_python():
     d = D.fixture_eplace(3000, 12)
     assert oracle.filler_plan(d)[:3] == D.filler_plan(d, 1.0)[:3]
