@@ -21,6 +21,7 @@ time-to-converge and jobs/h, against the same arrivals served one at a time
 on the FP64 path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-serve]
+       python bench.py --roofline-only FAMILY [--steps K] [--warmup W]   (that leg alone)
 """
 from __future__ import annotations
 
@@ -58,9 +59,83 @@ def timed_run(design, prec, steps, warmup, profile_mask=0, max_iter=None):
     t_ms = rows[-1]["elapsed_ms"] - rows[warmup - 1]["elapsed_ms"] if warmup > 0 else rows[-1]["elapsed_ms"]
     evals = rows[-1]["evaluations"] - (rows[warmup - 1]["evaluations"] if warmup > 0 else 0)
     stats = eng.kernel_stats()
+    sizes = dict(n=eng.n_opt, B=eng.dim * eng.dim, dim=eng.dim, P=design.num_pins,
+                 E=design.num_nets, Nmov=design.num_movable,
+                 Nfix=design.num_nodes - design.num_movable)
     eng.close()
     return dict(iters=k, ms=t_ms, iters_per_s=1000.0 * k / t_ms, evals=evals, setup_s=setup,
-                result=res, last=rows[-1], stats=stats)
+                result=res, last=rows[-1], stats=stats, sizes=sizes)
+
+
+def algorithmic_work(z, fused_finalize):
+    """Algorithmic bytes (and flops) per launch of each kernel family, from
+    the per-kernel formulas of the hot-path measurement plan (n optimised
+    nodes, B bins, P pins, E nets, FP32):
+      scatter    17n + 8B            (x, y, w, h, flags in; int64 grid out)
+      finalize   16B                 (int64 grid + static terms in, rho out)
+      poisson    16B, 2 nx ny (4 nx + 3 ny) flops (DCT as dense products);
+                 with the finalize fused into its first pass, 16B + 16B - 8B
+                 (rho is neither written nor read back)
+      wa_nets    16P + 4E + 8(Nmov + Nfix) + 8 Nmov
+      node_grad  25n + 8B (gather) + 8P + 4 Nmov (node-pin pass)
+      update     2n * 28 + 8n        (u, g, v, u_prev, g_prev in; u, v out; w, h)
+      sort, reduce: no algorithmic bytes of their own (ordering and FP64
+                 block sums of this implementation)."""
+    n, B, P, E, nm, nf, dim = (z["n"], z["B"], z["P"], z["E"], z["Nmov"], z["Nfix"], z["dim"])
+    dct_flops = 2.0 * dim * dim * (4 * dim + 3 * dim)
+    return {
+        "scatter": (17 * n + 8 * B, 0.0),
+        "finalize": (16 * B, 0.0),
+        "poisson": ((24 if fused_finalize else 16) * B, dct_flops),
+        "wa_nets": (16 * P + 4 * E + 8 * (nm + nf) + 8 * nm, 0.0),
+        "node_grad": (25 * n + 8 * B + 8 * P + 4 * nm, 0.0),
+        "update": (64 * n, 0.0),
+        "sort": (0.0, 0.0),
+        "reduce": (0.0, 0.0),
+    }
+
+
+def iteration_roofline_us(z, evals_per_iter):
+    """Roofline time of one mGP iteration: the algorithmic bytes of one
+    evaluation at 8 TB/s plus the DCT flops at the FP32 matrix peak, times the
+    evaluations per iteration."""
+    w = algorithmic_work(z, False)
+    # scatter, rho/tau, WA + HPWL, gather + node pass, Nesterov: the bytes;
+    # the transforms: their flops
+    byts = sum(w[f][0] for f in ("scatter", "finalize", "wa_nets", "node_grad", "update"))
+    flops = w["poisson"][1]
+    return evals_per_iter * (byts / (HBM_PEAK_GBS * 1e9) + flops / (MFMA_PEAK[32] * 1e12)) * 1e6
+
+
+def roofline_run(d, family, steps, warmup):
+    """The dominant family timed alone with HIP events over every launch of
+    its kernels in one FP32 run (set-up evaluations, the loop slots and the
+    delivery), the population a kernel trace of the same run counts."""
+    from superplace_amd import hip
+    bit = 1 << hip.KERNEL_FAMILIES.index(family)
+    rr = timed_run(d, 32, steps, warmup, profile_mask=bit)
+    st = rr["stats"][family]
+    launches = max(st["launches"], 1)
+    avg_s = st["ms"] / launches * 1e-3
+    fused = family == "poisson" and rr["stats"]["finalize"]["launches"] == 0
+    alg_bytes, alg_flops = algorithmic_work(rr["sizes"], fused)[family]
+    model_bytes = st["bytes"] / launches   # the engine's own byte model (windows, records)
+    if family == "poisson":
+        achieved = alg_flops / avg_s / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=MFMA_PEAK[32], unit="TFLOP/s",
+                    frac=achieved / MFMA_PEAK[32])
+        algorithmic = alg_flops
+    else:
+        achieved = alg_bytes / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS)
+        algorithmic = alg_bytes
+    roof["traffic"] = measured_traffic(family)
+    roof.update(kernel=family, avg_launch_us=avg_s * 1e6, launches=st["launches"],
+                algorithmic_per_launch=algorithmic,
+                algorithmic_bytes_per_launch=alg_bytes,
+                overhead_bytes_per_launch=max(model_bytes - alg_bytes, 0.0))
+    return roof
 
 
 def auto_bin_dim(d, lo, hi):
@@ -213,6 +288,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--roofline-only", default="", metavar="FAMILY",
+                    help="run only the roofline leg of this kernel family")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -225,6 +302,11 @@ def main():
 
     t0 = time.time()
     d = D.config_adaptec1()
+    if args.roofline_only:
+        # the roofline leg alone (for a kernel trace of exactly its launches)
+        roof = roofline_run(d, args.roofline_only, args.steps, args.warmup)
+        print(json.dumps({"roofline": roof}), flush=True)
+        return
     log(f"design: {d.num_movable} cells, {d.num_nodes - d.num_movable} fixed, {d.num_fillers} "
         f"fillers, {d.num_pins} pins, {d.num_nets} nets ({time.time() - t0:.1f} s)")
 
@@ -234,36 +316,26 @@ def main():
         f"{r32['iters_per_s']:.1f} iters/s ({r32['evals']} evaluations); tau "
         f"{r32['last']['overflow']:.4f} HPWL {r32['last']['hpwl']:.6g}")
 
-    # 2) all kernel families timed (slots launched directly, every kernel of a
-    #    family carrying its HIP event pair) to find the dominant one
+    # 2) all kernel families timed (slots launched one at a time, every kernel
+    #    of a family carrying its HIP event pair) to find the dominant one:
+    #    the family with the most time in the run
     probe = timed_run(d, 32, args.steps, args.warmup,
                       profile_mask=(1 << len(hip.KERNEL_FAMILIES)) - 1)
     fam_us = {f: 1e3 * st["ms"] / st["launches"] for f, st in probe["stats"].items()
               if st["launches"] > 0}
-    dominant = max(fam_us, key=fam_us.get)
+    fam_total = {f: st["ms"] for f, st in probe["stats"].items() if st["launches"] > 0}
+    dominant = max(fam_total, key=fam_total.get)
     log("kernel families, average us per launch: " +
         ", ".join(f"{f} {v:.1f}" for f, v in fam_us.items()) + f"; dominant {dominant}")
 
-    # 3) roofline of the dominant family over the same timed region, timed alone
-    dom_bit = 1 << hip.KERNEL_FAMILIES.index(dominant)
-    rr = timed_run(d, 32, args.steps, args.warmup, profile_mask=dom_bit)
-    st = rr["stats"][dominant]
-    avg_ms = st["ms"] / max(st["launches"], 1)
-    mfma = st["flops"] > 0   # dense-transform Poisson; every other kernel is memory-bound
-    if mfma:
-        achieved = st["flops"] / max(st["launches"], 1) / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=MFMA_PEAK[32], unit="TFLOP/s",
-                    frac=achieved / MFMA_PEAK[32], traffic=None)
-    else:
-        achieved = st["bytes"] / max(st["launches"], 1) / (avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None)
-    roof["traffic"] = measured_traffic(dominant)
-    roof.update(kernel=dominant, avg_launch_us=avg_ms * 1e3, launches=st["launches"],
-                algorithmic_per_launch=(st["flops"] if mfma else st["bytes"])
-                / max(st["launches"], 1))
-    log(f"roofline {dominant}: {achieved:.1f} {roof['unit']} over {st['launches']} launches, "
-        f"{avg_ms * 1e3:.1f} us each")
+    # 3) roofline of the dominant family, timed alone
+    roof = roofline_run(d, dominant, args.steps, args.warmup)
+    log(f"roofline {dominant}: {roof['achieved']:.1f} {roof['unit']} over {roof['launches']} "
+        f"launches, {roof['avg_launch_us']:.2f} us each")
+    evals_per_iter = r32["evals"] / r32["iters"]
+    t_roof = iteration_roofline_us(r32["sizes"], evals_per_iter)
+    iteration = dict(roofline_us=t_roof, measured_us=1e3 * r32["ms"] / r32["iters"],
+                     frac=t_roof / (1e3 * r32["ms"] / r32["iters"]))
 
     # 3) FP64 path
     r64 = None
@@ -314,7 +386,8 @@ def main():
         "cpu_baseline": cpu,
         "fp64_iters_per_s": r64["iters_per_s"] if r64 else None,
         "fp32_over_fp64": (r32["iters_per_s"] / r64["iters_per_s"]) if r64 else None,
-        "evals_per_iter": r32["evals"] / r32["iters"],
+        "evals_per_iter": evals_per_iter,
+        "iteration_roofline": iteration,
         "kernel_us_per_launch": {f: round(v, 2) for f, v in fam_us.items()},
         "batch": batch,
         "serving": served,

@@ -344,12 +344,12 @@ private:
     LaunchTimer timer_;
     std::vector<int> evUsed_;   // kernels timed per (family, slot) in the last launch of the slot
     // Adds, per profiled family, the summed kernel durations of slots
-    // [0, nslots) (one launch of the family per slot that ran it).
-    void harvestK(int mask, int nslots) {
+    // [lo, hi) (one launch of the family per slot that ran it).
+    void harvestK(int mask, int lo, int hi) {
         const size_t perF = evUsed_.size() / SP_HIP_K_COUNT;
         for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
             if (!(mask & (1 << f))) continue;
-            for (int i = 0; i < nslots; ++i) {
+            for (int i = lo; i < hi; ++i) {
                 const int used = evUsed_[f * perF + i];
                 if (used == 0) continue;   // the family did not run in slot i
                 double ms = 0;
@@ -366,6 +366,22 @@ private:
                 kflops_[f] += flopsPer_[f];
             }
         }
+    }
+    // A launch outside the loop slots (host-driven evaluations, delivery,
+    // position uploads): with family f profiled, its kernels carry event pairs
+    // and are harvested at once, so that a profiled run times every launch of
+    // the family, as a kernel trace of the same run counts them.
+    template <typename F>
+    void hostTimed(int f, F&& launch) {
+        if (!(profMask_ & (1 << f))) {
+            launch();
+            return;
+        }
+        ensureEvents(std::max(1, cfg_.graph_slots));
+        beginK(f, 0);
+        launch();
+        endK(f, 0);
+        harvestK(1 << f, 0, 1);
     }
 
     sp_hip_gp_config cfg_;
@@ -808,16 +824,27 @@ template <typename T>
 EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda,
                                 float gamma, long long* dbgNode, long long* dbgFill, T* dbgForce,
                                 T* dbgWl) {
-    launchTileSort<T>(dd_, geom_, nullptr, s_);
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
-                     ws_, nullptr, s_);
-    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, rhoShift_, poisson_->rho(), dbgNode, dbgFill, ws_, nullptr, s_);
-    poisson_->solve(false);
-    launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, nullptr, s_);
-    launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), gamma, lambda, u, up,
-                      gp, grad, dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
-    launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
+    hostTimed(SP_HIP_K_SORT, [&] { launchTileSort<T>(dd_, geom_, nullptr, s_); });
+    hostTimed(SP_HIP_K_SCATTER, [&] {
+        launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_,
+                         true, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_FINALIZE, [&] {
+        launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_,
+                          std::ldexp(1.0, -fracBits_), binArea_, rhoShift_, poisson_->rho(), dbgNode,
+                          dbgFill, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_POISSON, [&] { poisson_->solve(false); });
+    hostTimed(SP_HIP_K_WA_NETS, [&] {
+        launchWaNets<T>(dd_, gamma, cfg_.ignore_net_degree, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_NODE_GRAD, [&] {
+        launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), gamma, lambda, u, up, gp, grad,
+                          dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_REDUCE, [&] {
+        launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
+    });
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
@@ -828,20 +855,32 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
 
 template <typename T>
 void Engine<T>::scatterPos(const T* pos) {
-    launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr, s_);
+    hostTimed(SP_HIP_K_UPDATE, [&] {
+        launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr,
+                        s_);
+    });
     curPos_ = pos;
 }
 
 // Delivery (eplace.cpp:478-494): positions already scattered; tau and HPWL.
 template <typename T>
 double Engine<T>::deliveredMetrics(double* hpwlOut) {
-    launchTileSort<T>(dd_, geom_, nullptr, s_);
-    launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_, true,
-                     ws_, nullptr, s_);
-    launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_, std::ldexp(1.0, -fracBits_),
-                      binArea_, rhoShift_, poisson_->rho(), nullptr, nullptr, ws_, nullptr, s_);
-    launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
-    launchReduce<T>(ws_, movableAreaScaled_, false, nullptr, nullptr, s_);
+    hostTimed(SP_HIP_K_SORT, [&] { launchTileSort<T>(dd_, geom_, nullptr, s_); });
+    hostTimed(SP_HIP_K_SCATTER, [&] {
+        launchScatter<T>(dd_, geom_, static_cast<T>(std::ldexp(1.0, fracBits_)), nodeQ_, fillQ_,
+                         true, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_FINALIZE, [&] {
+        launchFinalize<T>(geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_,
+                          std::ldexp(1.0, -fracBits_), binArea_, rhoShift_, poisson_->rho(), nullptr,
+                          nullptr, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_WA_NETS, [&] {
+        launchWaNets<T>(dd_, 1.0f, cfg_.ignore_net_degree, ws_, nullptr, s_);
+    });
+    hostTimed(SP_HIP_K_REDUCE, [&] {
+        launchReduce<T>(ws_, movableAreaScaled_, false, nullptr, nullptr, s_);
+    });
     SPHIP_CHECK(hipMemcpyAsync(hScalars_, ws_.scalars, kNumScalars * sizeof(double),
                                hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
@@ -934,8 +973,10 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         const double posNorm = std::sqrt(sA.dx2), gNorm = std::sqrt(sA.gg);
         if (gNorm > 0.0 && std::isfinite(gNorm)) {
             const double scale = 0.1 * (posNorm > 0.0 ? posNorm : 1.0) / gNorm;
-            launchUpdate<T>(dd_, u_, gA_, nullptr, static_cast<double>(static_cast<T>(scale)), 0.0,
-                            probe_, nullptr, nullptr, nullptr, s_);
+            hostTimed(SP_HIP_K_UPDATE, [&] {
+                launchUpdate<T>(dd_, u_, gA_, nullptr, static_cast<double>(static_cast<T>(scale)),
+                                0.0, probe_, nullptr, nullptr, nullptr, s_);
+            });
             curPos_ = probe_;
             EvalScalars sB = evaluate(probe_, u_, gA_, gB_, lambda, gamma);
             checkFinite(sB);
@@ -1271,17 +1312,21 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         return;
     }
     if (evMask) {
-        // profiled: one launch at a time, its kernels' events harvested
-        for (long launch = 0;; ++launch) {
+        // profiled: the slots one at a time (the same slot sequence as the
+        // graph holds), each slot's events harvested, and no slot launched
+        // once the stage has ended, so every timed launch did its work
+        bool done = false;
+        for (long launch = 0; !done; ++launch) {
             if (launch > maxLaunches_) throw std::runtime_error("device loop did not terminate");
-            for (int i = 0; i < slots; ++i) launchSlot(true, i);
-            SPHIP_CHECK(hipMemcpyAsync(hFlag_, reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
-                                       sizeof(int), hipMemcpyDeviceToHost, s_));
-            SPHIP_CHECK(hipStreamSynchronize(s_));
-            const bool done = *hFlag_ != 0;
-            // only launches whose slots all ran a full evaluation are timed
-            if (!done) harvestK(evMask, slots);
-            if (done) break;
+            for (int i = 0; i < slots && !done; ++i) {
+                launchSlot(true, i);
+                SPHIP_CHECK(hipMemcpyAsync(hFlag_,
+                                           reinterpret_cast<char*>(dCtl_) + offsetof(Ctl, done),
+                                           sizeof(int), hipMemcpyDeviceToHost, s_));
+                SPHIP_CHECK(hipStreamSynchronize(s_));
+                harvestK(evMask, i, i + 1);
+                done = *hFlag_ != 0;
+            }
         }
     } else {
         // kAsyncDepth launches in flight: the next one is queued before the

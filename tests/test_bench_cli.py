@@ -50,3 +50,20 @@ def test_cpu_baseline_record():
     assert r["value"] > 0 and r["unit"] == "iters/s" and r["kind"] == "port"
     assert r["cores"] >= 1
     assert r["sample"].startswith("1 runs of the first 3 mGP iterations")
+
+
+def test_algorithmic_bytes_follow_the_measurement_plan():
+    """Per-launch algorithmic work at the benchmark sizes (n = 370,176
+    optimised nodes, 944k pins, 221k nets, 512^2 bins): scatter 8.4 MB,
+    rho/tau 4.2, WA + HPWL 19.4 + node pass 19.7 (27.8 + gather 11.4
+    together), Nesterov 23.7 MB; DCT 1.879 GFLOP; one evaluation at roofline
+    about 21 us."""
+    z = dict(n=370176, B=512 * 512, dim=512, P=944000, E=221000, Nmov=211000, Nfix=543)
+    w = bench.algorithmic_work(z, False)
+    mb = {k: v[0] / 1e6 for k, v in w.items()}
+    assert abs(mb["scatter"] - 8.39) < 0.01 and abs(mb["finalize"] - 4.19) < 0.01
+    assert abs(mb["update"] - 23.69) < 0.01
+    assert abs(mb["wa_nets"] + mb["node_grad"] - (27.8 + 11.4)) < 0.2
+    assert abs(w["poisson"][1] / 1e9 - 1.879) < 0.001
+    assert abs(bench.algorithmic_work(z, True)["poisson"][0] - 24 * z["B"]) == 0
+    assert 20.5 < bench.iteration_roofline_us(z, 1.0) < 22.0

@@ -180,29 +180,6 @@ def test_nonfinite_input_rejected():
         hip.GpEngine(d, hip.GpConfig())
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_loop_modes_agree(mode):
-    """Host decisions (0), device decisions launched slot by slot (1) and the
-    hipGraph-captured loop (2) take the same decisions: same iteration count
-    and evaluation count within 2, final HPWL within 0.5% (the schedules'
-    powf may differ by an ulp between host and device)."""
-    d = D.with_fillers(D.fixture_eplace(3000, 12))
-    ref, rref = hip.GpEngine(d, hip.GpConfig(loop_mode=0)).run()
-    res, rows = hip.GpEngine(d, hip.GpConfig(loop_mode=mode, graph_slots=5)).run()
-    print(mode, res["iterations"], res["evaluations"], res["hpwl"], "| host", ref["iterations"],
-          ref["evaluations"], ref["hpwl"], flush=True)
-    # the first rows agree closely (same arithmetic, same decisions) ...
-    for a, b in zip(rows[:10], rref[:10]):
-        assert a["evaluations"] == b["evaluations"]
-        assert a["hpwl"] == pytest.approx(b["hpwl"], rel=1e-4)
-        assert a["overflow"] == pytest.approx(b["overflow"], rel=1e-4, abs=1e-6)
-    # ... and the whole stage lands in the same place (a 180-iteration
-    # trajectory amplifies ulp-level schedule differences to ~0.5%)
-    assert res["converged"] and res["overflow"] < 0.10
-    assert abs(res["iterations"] - ref["iterations"]) <= 5
-    assert abs(res["hpwl"] / ref["hpwl"] - 1) < 0.02
-
-
 def test_graph_loop_backtracks_and_hits_iteration_limit():
     d = D.with_fillers(D.fixture_eplace(2000, 8))
     eng = hip.GpEngine(d, hip.GpConfig(loop_mode=2, gp_max_iter=37, graph_slots=8))
