@@ -66,6 +66,19 @@ int sp_hip_poisson_solve(sp_hip_poisson* p, const float* density, float* phi, fl
  * No counterpart in the reference (its CUDA backend always crosses to the
  * host, cuda_poisson.cu:194-210); it is the data path of the engine below. */
 int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* out3_dev);
+/* The FP32 solve of sp_hip_poisson_solve_dev without a handle and without
+ * allocating: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  nx, ny: powers of two from 64 to 1024
+ * (SP_HIP_EINVAL otherwise); density_dev nx*ny floats, out3_dev 3*nx*ny
+ * floats (phi | field_x | field_y, data[x*ny + y]); enqueued on `stream`
+ * (a hipStream_t; NULL = the default stream), not synchronised.  The
+ * intermediates live in a static device array of the library (12.6 MB per
+ * GPU, held until the process ends) that every call on one GPU shares:
+ * calls that overlap on one GPU race.  Results equal those of
+ * sp_hip_poisson_solve_dev (FP32 handle) on the same grid. */
+int sp_hip_poisson_solve_noalloc(int nx, int ny, float step_x, float step_y,
+                                 const float* density_dev, float* out3_dev, void* stream);
 /* PoissonBackend::name(): "hip-fft" (power-of-two grids: LDS FFT passes) or
  * "hip-mfma" (other sizes: dense transforms on the matrix cores), with
  * "-fp64" appended on the FP64 path. */

@@ -19,7 +19,7 @@ LIB = HERE / "libsuperplace_hip.so"
 OBJ_DIR = HERE / "_build"
 SOURCES = ["poisson.hip", "gp_kernels.hip", "engine.hip", "qp.hip", "bookshelf.cpp", "setup.cpp",
            "metrics.cpp", "legalize.cpp", "macro_lg.cpp",
-           "serve.cpp"]
+           "serve.cpp", "noalloc.hip"]
 ARCH = os.environ.get("SP_HIP_ARCH", "gfx950")
 
 

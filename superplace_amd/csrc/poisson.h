@@ -100,6 +100,17 @@ private:
     void passBC(const int* skip, bool withPhi = false);
 };
 
+// The FP32 FFT solve of an nx x ny grid (powers of two, 64 to 1024 per side,
+// no padding) over caller-provided device memory, allocating nothing: rho in
+// (nx * ny), out3 = phi | xi_x | xi_y (3 nx ny), and `work` of
+// fftSolveWorkFloats(nx, ny) floats for the intermediates, the tables and
+// the spectral scales (rewritten by every call).  The kernels and their
+// arithmetic are PoissonPlan<float>'s with one column pair per block, so the
+// results equal a plan's solve(true).  Enqueued on s.
+size_t fftSolveWorkFloats(int nx, int ny);
+void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, float* out3,
+                  float* work, hipStream_t s);
+
 // Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on
 // MFMA; the plan's primitive, exposed for the GEMM unit test.
 template <typename T>

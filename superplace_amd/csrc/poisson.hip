@@ -1174,6 +1174,72 @@ void PoissonPlan<T>::solve(bool withPhi, const int* skip) {
     }
 }
 
+// ------------------------------------------------------------ no allocation
+// The tables of one axis: L twiddles exp(-2 pi i t/L), then L half-angle
+// factors exp(-i pi k/2L) (complex, interleaved), in FP64 rounded once to T,
+// the values PoissonPlan uploads; and the FP32 spectral scales
+// u_j = pi j / (n step), in the reference's float arithmetic.
+__global__ __launch_bounds__(kBlock) void k_fft_tables(float* tab, float* scale, int L,
+                                                       float step) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L) return;
+    const double pi = 3.14159265358979323846;
+    tab[2 * k] = static_cast<float>(cos(2 * pi * k / L));
+    tab[2 * k + 1] = static_cast<float>(-sin(2 * pi * k / L));
+    tab[2 * L + 2 * k] = static_cast<float>(cos(pi * k / (2.0 * L)));
+    tab[2 * L + 2 * k + 1] = static_cast<float>(-sin(pi * k / (2.0 * L)));
+    scale[k] = static_cast<float>(3.14159265358979323846f) * static_cast<float>(k) /
+               (static_cast<float>(L) * step);
+}
+
+size_t fftSolveWorkFloats(int nx, int ny) {
+    return 3 * static_cast<size_t>(nx) * ny + 5 * static_cast<size_t>(nx + ny);
+}
+
+void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, float* out3,
+                  float* work, hipStream_t s) {
+    if (!PoissonPlan<float>::fftShape(nx) || !PoissonPlan<float>::fftShape(ny) || nx < 64 ||
+        ny < 64)
+        throw std::invalid_argument("fftSolveInto: grid sides must be powers of two, 64 to 1024");
+    using C2 = cplx<float>;
+    const size_t n = static_cast<size_t>(nx) * ny;
+    float* t0 = work;
+    float* t1 = t0 + n;
+    float* t2 = t1 + n;
+    float* tabX = t2 + n;
+    float* tabY = tabX + 4 * static_cast<size_t>(nx);
+    float* u = tabY + 4 * static_cast<size_t>(ny);
+    float* v = u + nx;
+    launchKernel(k_fft_tables, dim3((nx + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tabX, u, nx,
+                 stepX);
+    launchKernel(k_fft_tables, dim3((ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tabY, v, ny,
+                 stepY);
+    SPHIP_CHECK(hipGetLastError());
+    const FftTab<float> tx{reinterpret_cast<const C2*>(tabX), reinterpret_cast<const C2*>(tabX) + nx};
+    const FftTab<float> ty{reinterpret_cast<const C2*>(tabY), reinterpret_cast<const C2*>(tabY) + ny};
+    // the LDS of the plan's passes with one column pair per block (setFftLds)
+    const size_t c = 2 * sizeof(float);
+    const size_t ldsA = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
+    const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
+    const size_t ldsC = (kFftLdsHead(ny) + 4 * static_cast<size_t>(ny)) * c;
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<float>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA));
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<float>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsB));
+    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_idct_rows<float>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsC));
+    const float invNorm = static_cast<float>(1.0 / (4.0 * nx * ny));
+    launchKernel(k_dct_rows<float>, dim3(nx / 2), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsA),
+                 s, rho, t0, nx, ny, ny, ty, static_cast<const int*>(nullptr));
+    launchKernel(k_dct_cols<float>, dim3(ny / 2), dim3(fftThreads(nx)), static_cast<uint32_t>(ldsB),
+                 s, t0, t1, t2, 1, nx, ny, ny, tx, static_cast<const float*>(u),
+                 static_cast<const float*>(v), invNorm, static_cast<const int*>(nullptr));
+    launchKernel(k_idct_rows<float>, dim3(nx), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsC),
+                 s, t1, t2, out3 + n, out3 + 2 * n, out3, 1, nx, ny, ny, ty,
+                 static_cast<const float*>(v), static_cast<const int*>(nullptr));
+    SPHIP_CHECK(hipGetLastError());
+}
+
 template class PoissonPlan<float>;
 template class PoissonPlan<double>;
 template void launchGemm<float>(int, const float* const*, const float* const*, float* const*,
