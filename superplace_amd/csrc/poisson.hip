@@ -7,6 +7,7 @@
 // LDS-staged 64-wide wavefront tiles.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "poisson.h"
@@ -845,13 +846,14 @@ void gemmDispatch(int batch, const GemmBatch<T>& gb, int M, int N, int K, int ld
                   int cstep = 1) {
     if (M % 64 || N % 64 || K % 64) throw std::runtime_error("gemm: dims must be multiples of 64");
     if (batch < 1 || batch > 4) throw std::runtime_error("gemm: batch must be 1..4");
-    static bool attrSet = false;   // the panels need more than the default 64 KB of LDS
+    // the panels need more than the default 64 KB of LDS; set once per
+    // instantiation (several threads may solve at once)
     constexpr unsigned shm = 2 * PanelGeom<T>::ROWS * PanelGeom<T>::LD * sizeof(T);
-    if (!attrSet) {
+    static std::once_flag attrOnce;
+    std::call_once(attrOnce, [] {
         SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gemm<T, EPI, TRANS>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-        attrSet = true;
-    }
+    });
     dim3 grid(N / 32, M / 32, batch);
     launchKernel(k_gemm<T, EPI, TRANS>, grid, 256, shm, s, gb, M, N, K, lda, ldb, ldc, cstep, epi,
                  skip);

@@ -80,6 +80,7 @@ constexpr int kSetupThreads = 4;   // job set-ups in flight at once
 // What a set-up thread hands back.
 struct Admission {
     std::unique_ptr<EngineBase> eng;
+    sp_hip_gp_result res{};  // written by the set-up thread, copied to the job once joined
     bool finished = false;   // the stage ended before its loop
     int status = SP_HIP_OK;
     std::string error;
@@ -128,6 +129,7 @@ public:
             }
             Admission a = admitting_[k].second.get();
             Job& j = *jobs_[admitting_[k].first];
+            j.res = a.res;
             admitting_.erase(admitting_.begin() + static_cast<long>(k));
             progress = true;
             if (a.status != SP_HIP_OK) {
@@ -186,12 +188,13 @@ public:
             j.info.state = SP_HIP_JOB_RUNNING;
             const sp_hip_design* d = &j.design->view;
             const sp_hip_gp_config cfg = j.cfg;
-            sp_hip_gp_result* res = &j.res;
-            admitting_.emplace_back(j.id, std::async(std::launch::async, [d, cfg, res] {
+            const sp_hip_gp_result start = j.res;
+            admitting_.emplace_back(j.id, std::async(std::launch::async, [d, cfg, start] {
                 Admission a;
+                a.res = start;
                 a.status = guarded([&] {
                     a.eng = makeEngine(*d, cfg);
-                    a.finished = a.eng->begin(res);
+                    a.finished = a.eng->begin(&a.res);
                 });
                 if (a.status != SP_HIP_OK) {
                     a.error = g_lastError;
