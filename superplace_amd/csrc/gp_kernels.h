@@ -84,7 +84,7 @@ constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
 constexpr int kTileChunk = 512;      // nodes per scatter block (a tile's owner takes the first)
 constexpr double kSpillFrac = 0.05;  // re-sort once this fraction of nodes left their tile window
-constexpr int kSortPeriod = 4;       // ... checked at every fourth evaluation of the device loop
+constexpr int kSortPeriod = 2;       // ... checked at every second evaluation of the device loop
 constexpr int kSortBlocks = 512;     // grid of the sort passes (they stride over record chunks)
 
 // FP64 reductions: every reducing kernel writes one partial per block (plain
