@@ -139,67 +139,57 @@ def roofline_run(d, family, steps, warmup):
 
 
 def auto_bin_dim(d, lo, hi):
-    """The reference's automatic bin dimension (bin_grid.cpp:172-189: the
-    largest power of two whose bins hold one average movable node at target
-    density 1), clamped to [lo, hi]."""
-    import numpy as np
-    mov = np.arange(d.num_movable)
-    keep = (d.node_flags[mov] & (2 | 8)) == 0
-    area = float(np.sum(d.node_w[mov][keep].astype(np.float64) * d.node_h[mov][keep]))
-    ideal = (d.core[2] - d.core[0]) * (d.core[3] - d.core[1]) / max(area / max(d.num_movable, 1), 1e-9)
-    dim = 4
-    while dim * 2 <= 1024 and (dim * 2) ** 2 <= ideal:
-        dim *= 2
-    return min(max(dim, lo), hi)
+    from superplace_amd import design as D
+    return D.auto_bin_dim(d, lo, hi)
 
 
 def job_designs(count, lo, hi, seed0, dim_lo, dim_hi):
-    """Seeded synthetic designs, cell counts log-uniform in [lo, hi] (drawn
-    from the splitmix64 stream of seed 1002), config-1 degree distribution,
-    1.05 nets per cell, 57% utilisation at adaptec1's cell density."""
-    import numpy as np
     from superplace_amd import design as D
-    u = D.Rng(1002 + seed0).uniform(count)
-    out = []
-    for k in range(count):
-        cells = int(round(lo * (hi / lo) ** float(u[k])))
-        d = D.synth_design(cells, max(8, cells // 400), int(cells * 1.05), 25.25 * np.sqrt(cells),
-                           0.57, seed=seed0 + k + 1, name=f"job{k}")
-        out.append((d, auto_bin_dim(d, dim_lo, dim_hi)))
-    return out
+    return D.job_designs(count, lo, hi, seed0, dim_lo, dim_hi)
 
 
 def batch_bench(count):
-    """Uneven instances advanced together by the scheduler, against the same
-    instances run one after another."""
+    """Uneven instances advanced together, against the same instances run one
+    after another: by batched launches (sp_hip_gp_run_batch: every kernel of
+    an iteration launched once for all the instances) and by the scheduler
+    (each instance on its own stream); each job's iterations must equal its
+    own run's."""
     from superplace_amd import hip, serve
     jobs = job_designs(count, 5000, 80000, 0, 128, 512)
     cfgs = [hip.GpConfig(gp_bin_dim=dim) for _, dim in jobs]
+    designs = [d for d, _ in jobs]
     t0 = time.time()
-    iters_seq = 0
-    for (d, _), cfg in zip(jobs, cfgs):
+    alone = []
+    for d, cfg in zip(designs, cfgs):
         eng = hip.GpEngine(d, cfg)
         res, _ = eng.run(max_rows=1)
         eng.positions()
         eng.close()
-        iters_seq += res["iterations"]
+        alone.append(res["iterations"])
     seq_s = time.time() - t0
+    t0 = time.time()
+    bres = hip.run_batch(designs, cfgs, positions=False)
+    batched_s = time.time() - t0
     srv = serve.Server(hbm_budget=200 << 30, max_active=16)
-    for (d, _), cfg in zip(jobs, cfgs):
+    for d, cfg in zip(designs, cfgs):
         srv.submit(d, cfg, 0.0)
     t0 = time.time()
     srv.drain(timeout_s=600)
-    batch_s = time.time() - t0
+    sched_s = time.time() - t0
     infos = [srv.job(i, positions=False) for i in range(count)]
     srv.close()
-    iters = sum(i["result"]["iterations"] for i in infos)
-    if iters != iters_seq or any(i["state"] != serve.DONE for i in infos):
+    if ([r["iterations"] for r in bres] != alone or
+            [i["result"]["iterations"] for i in infos] != alone or
+            any(i["state"] != serve.DONE for i in infos)):
         raise RuntimeError("batched jobs differ from the same jobs run alone")
-    cells = [d.num_movable for d, _ in jobs]
+    iters = sum(alone)
+    cells = [d.num_movable for d in designs]
     return dict(jobs=count, cells_min=min(cells), cells_max=max(cells), cells_total=sum(cells),
-                iterations=iters, batch_s=batch_s, serial_s=seq_s,
-                batch_iters_per_s=iters / batch_s, serial_iters_per_s=iters / seq_s,
-                speedup=seq_s / batch_s)
+                bins=sorted(set(c.gp_bin_dim for c in cfgs)), iterations=iters,
+                batched_s=batched_s, scheduler_s=sched_s, serial_s=seq_s,
+                batched_iters_per_s=iters / batched_s, scheduler_iters_per_s=iters / sched_s,
+                serial_iters_per_s=iters / seq_s, speedup=seq_s / batched_s,
+                scheduler_speedup=seq_s / sched_s)
 
 
 def serve_bench(count, rate):
@@ -349,8 +339,9 @@ def main():
     if not args.no_serve:
         batch = batch_bench(args.batch_jobs)
         log(f"batch: {batch['jobs']} jobs ({batch['cells_min']}-{batch['cells_max']} cells), "
-            f"{batch['iterations']} iterations in {batch['batch_s']:.2f} s together vs "
-            f"{batch['serial_s']:.2f} s one at a time ({batch['speedup']:.2f}x)")
+            f"{batch['iterations']} iterations in {batch['batched_s']:.2f} s batched, "
+            f"{batch['scheduler_s']:.2f} s scheduled, {batch['serial_s']:.2f} s one at a time "
+            f"({batch['speedup']:.2f}x, {batch['scheduler_speedup']:.2f}x)")
         served = serve_bench(args.serve_jobs, args.serve_rate)
         m, f = served["served_fp32"], served["serial_fp64"]
         log(f"serving: {served['jobs']} jobs at {args.serve_rate}/s: p50 {m['p50_s']:.3f} s, "

@@ -218,6 +218,18 @@ int sp_hip_gp_eval(sp_hip_gp* gp, const float* x, const float* y, float lambda, 
 /* EPlace::run(stage): runs the stage to convergence, stagnation or the
  * iteration limit.  rows may be NULL; at most max_rows rows are written. */
 int sp_hip_gp_run(sp_hip_gp* gp, sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int max_rows);
+/* Many placement stages run together (no counterpart in the reference,
+ * which places one design per process): count FP32 instances (power-of-two
+ * bin grids: the automatic rule or a power-of-two gp_bin_dim; any sizes, and
+ * grids of different sizes in one batch) advance through their stages in
+ * batched launches, each kernel of an iteration launched once for all of
+ * them (the Poisson passes once per grid size), `slots` iterations per
+ * hipGraph launch (<= 0: 8) until every stage has ended.  results[i] and
+ * the positions (x[i], y[i]: all nodes of design i; x, y or their entries
+ * may be NULL) equal those of sp_hip_gp_run on the same design and
+ * configuration, bit for bit. */
+int sp_hip_gp_run_batch(int count, const sp_hip_design* designs, const sp_hip_gp_config* cfgs,
+                        int slots, sp_hip_gp_result* results, float* const* x, float* const* y);
 /* Current node positions (all nodes; after run: the delivered placement). */
 int sp_hip_gp_get_positions(const sp_hip_gp* gp, float* x, float* y);
 /* mask: bit f (SP_HIP_K_*) times kernel family f with HIP events; 0 = off. */

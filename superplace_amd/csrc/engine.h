@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <memory>
+#include <vector>
 
 #include "../../include/superplace_hip.h"
 
@@ -36,6 +37,15 @@ public:
 };
 
 constexpr int kAsyncDepth = 2;   // loop launches in flight per instance
+
+// Advances many FP32 instances (each created with a device-driven
+// loop_mode, power-of-two bin grids, begin() already called: looping[i] is
+// whether instance i's stage still has its loop to run) through their
+// stages with batched launches: one hipGraph of `slots` slots, each kernel
+// launched once for all of them.  res[i] receives instance i's result,
+// equal to its own run's.
+void runBatch(const std::vector<EngineBase*>& engines, const std::vector<char>& looping,
+              sp_hip_gp_result* res, int slots);
 
 // A new instance on its own stream (precision from cfg.precision).
 std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg);

@@ -7,6 +7,10 @@
 // only the ten FP64 scalars of the evaluation to take the data-dependent
 // decisions (backtracking, step size, schedules, stopping).
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <exception>
+#include <thread>
 #include <chrono>
 #include <cstddef>
 #include <cmath>
@@ -23,6 +27,7 @@
 #include "wa_device.h"
 #include "density_device.h"
 #include "gp_kernels.h"
+#include "batch.h"
 #include "poisson.h"
 
 namespace sphip {
@@ -36,13 +41,22 @@ T* dalloc(size_t n) {
     return static_cast<T*>(p);
 }
 
+// Host copies an upload reads from until the stream is synchronised.
+using Staging = std::vector<std::vector<unsigned char>>;
+
+// A device copy of h converted to T.  With `keep`, the converted host copy
+// is kept there and the caller synchronises the stream once for all its
+// uploads; without, the upload is synchronised here.
 template <typename T, typename S>
-T* dupload(const std::vector<S>& h, hipStream_t s) {
-    std::vector<T> t(h.begin(), h.end());
-    T* d = dalloc<T>(t.size());
-    if (!t.empty())
-        SPHIP_CHECK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    SPHIP_CHECK(hipStreamSynchronize(s));
+T* dupload(const std::vector<S>& h, hipStream_t s, Staging* keep = nullptr) {
+    std::vector<unsigned char> bytes(h.size() * sizeof(T));
+    T* t = reinterpret_cast<T*>(bytes.data());
+    for (size_t i = 0; i < h.size(); ++i) t[i] = static_cast<T>(h[i]);
+    T* d = dalloc<T>(h.size());
+    if (!h.empty())
+        SPHIP_CHECK(hipMemcpyAsync(d, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+    if (keep) keep->push_back(std::move(bytes));
+    else SPHIP_CHECK(hipStreamSynchronize(s));
     return d;
 }
 
@@ -273,6 +287,61 @@ public:
         asyncReady_ = false;
         finishDevice(res, nullptr, 0);
     }
+
+    // The instance's share of a batched slot, after begin() left its device
+    // loop ready: the slot kernels' arguments, the Poisson buffers, and the
+    // instance's block count in every batched kernel (the counts of its own
+    // launches, so that every block does the same work).
+    void batchShare(SlotArgs<T>* a, PoissonBatchArgs<T>* p, int counts[BK_KINDS],
+                    int* doneOut) {
+        if (!asyncReady_) throw InvalidArg("batched slot: the stage loop is not ready");
+        if (!poisson_->usesFft())
+            throw InvalidArg("batched runs need power-of-two bin grids (FFT Poisson path)");
+        const int* skip = reinterpret_cast<const int*>(reinterpret_cast<const char*>(dCtl_) +
+                                                       offsetof(Ctl, done));
+        const TileGrid& tg = dd_.tiles;
+        counts[BK_SORT_COUNT] = static_cast<int>(
+            std::min<long>((dd_.nDens + kBlock - 1) / kBlock, kSortBlocks));
+        counts[BK_SORT_PLACE] = counts[BK_SORT_COUNT];
+        counts[BK_SCATTER] = tg.tilesX * tg.tilesY + (dd_.nDens + kTileChunk - 1) / kTileChunk;
+        counts[BK_FINALIZE] = static_cast<int>(
+            (std::max<long>(static_cast<long>(nb_), tg.nKeys) + kBlock - 1) / kBlock);
+        counts[BK_WA] = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
+        counts[BK_NODE] = (n_ + kBlock - 1) / kBlock;
+        counts[BK_REDUCE] = 1;
+        counts[BK_UPDATE] = counts[BK_NODE];
+        if (dd_.nDens == 0) counts[BK_SORT_COUNT] = counts[BK_SORT_PLACE] = 0;
+        ReduceWs ws = ws_;
+        ws.blocks[0] = counts[BK_FINALIZE];
+        ws.blocks[1] = counts[BK_WA];
+        ws.blocks[2] = counts[BK_SCATTER];
+        ws.blocks[3] = counts[BK_NODE];
+        a->d = dd_;
+        a->g = geom_;
+        a->qscale = static_cast<T>(std::ldexp(1.0, fracBits_));
+        a->nodeQ = nodeQ_;
+        a->fillQ = fillQ_;
+        a->din = DensityIn<T>{geom_, dd_.tiles, nodeQ_, fillQ_, terminal_, base_,
+                              std::ldexp(1.0, -fracBits_), binArea_, rhoShift_, ws.region(0),
+                              nullptr, nullptr};
+        a->rho = poisson_->rho();
+        a->exy = poisson_->fieldXY();
+        a->ws = ws;
+        a->ctl = dCtl_;
+        a->rows = dRows_;
+        a->bufs = dBufs_;
+        a->denom = movableAreaScaled_;
+        a->wa = WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws.region(1), dCtl_};
+        a->doneOut = doneOut;
+        *p = poisson_->batchArgs(skip);
+    }
+    int gridDim() const { return dim_; }
+    int gridLd() const { return poisson_->ld(); }
+    long maxLoopLaunches(int slots) const {
+        const int maxIter = cfg_.stage == 1 ? cfg_.filler_only_iters : cfg_.gp_max_iter;
+        const int maxBt = cfg_.max_backtrack > 0 ? cfg_.max_backtrack : 10;
+        return (static_cast<long>(maxIter) * maxBt + 16) / std::max(1, slots) + 2;
+    }
     size_t deviceBytes() const override { return devBytes_; }
 
 private:
@@ -472,6 +541,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     auto isFixed = [&](int i) { return (d.node_flags[i] & F_FIXED) != 0; };
     auto isNI = [&](int i) { return (d.node_flags[i] & F_NI) != 0; };
     auto own = [&](void* p) { owned_.push_back(p); return p; };
+    Staging staging;   // host copies of the uploads, until the stream syncs
 
     // buildOptSet, eplace.cpp:82-94.
     if (stage != 1)
@@ -550,8 +620,8 @@ void Engine<T>::build(const sp_hip_design& d) {
             base[static_cast<size_t>(bx) * dim_ + by] = T(target) * freeA;
         }
     }
-    terminal_ = static_cast<double*>(own(dupload<double>(term, s_)));
-    base_ = static_cast<double*>(own(dupload<double>(base, s_)));
+    terminal_ = static_cast<double*>(own(dupload<double>(term, s_, &staging)));
+    base_ = static_cast<double*>(own(dupload<double>(base, s_, &staging)));
 
     // fixed-point scale: the largest possible bin total (all density charge in
     // one bin) stays below 2^61
@@ -604,16 +674,16 @@ void Engine<T>::build(const sp_hip_design& d) {
             xy[2 * i] = xs[i];
             xy[2 * i + 1] = ys[i];
         }
-        dd_.xy = static_cast<T*>(own(dupload<T>(xy, s_)));
+        dd_.xy = static_cast<T*>(own(dupload<T>(xy, s_, &staging)));
     }
     std::vector<float> ws(d.node_w, d.node_w + total), hs(d.node_h, d.node_h + total);
     wHost_.assign(ws.begin(), ws.end());
     hHost_.assign(hs.begin(), hs.end());
-    dd_.w = static_cast<T*>(own(dupload<T>(ws, s_)));
-    dd_.h = static_cast<T*>(own(dupload<T>(hs, s_)));
+    dd_.w = static_cast<T*>(own(dupload<T>(ws, s_, &staging)));
+    dd_.h = static_cast<T*>(own(dupload<T>(hs, s_, &staging)));
     std::vector<uint8_t> fl(d.node_flags, d.node_flags + total);
-    dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_)));
-    dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_)));
+    dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_, &staging)));
+    dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_, &staging)));
     {
         // density records in node order to start with; the first sort orders them
         std::vector<int> recId(dens.size());
@@ -626,10 +696,10 @@ void Engine<T>::build(const sp_hip_design& d) {
             recWH[2 * t] = d.node_w[i];
             recWH[2 * t + 1] = d.node_h[i];
         }
-        dd_.recIdA = static_cast<int*>(own(dupload<int>(recId, s_)));
-        dd_.recIdB = static_cast<int*>(own(dupload<int>(recId, s_)));
-        dd_.recWHA = static_cast<T*>(own(dupload<T>(recWH, s_)));
-        dd_.recWHB = static_cast<T*>(own(dupload<T>(recWH, s_)));
+        dd_.recIdA = static_cast<int*>(own(dupload<int>(recId, s_, &staging)));
+        dd_.recIdB = static_cast<int*>(own(dupload<int>(recId, s_, &staging)));
+        dd_.recWHA = static_cast<T*>(own(dupload<T>(recWH, s_, &staging)));
+        dd_.recWHB = static_cast<T*>(own(dupload<T>(recWH, s_, &staging)));
         std::vector<float> recXY(2 * dens.size());
         std::vector<int> recPos(total, -1);
         for (size_t t = 0; t < dens.size(); ++t) {
@@ -637,9 +707,9 @@ void Engine<T>::build(const sp_hip_design& d) {
             recXY[2 * t + 1] = d.node_y[dens[t]];
             recPos[dens[t]] = static_cast<int>(t);
         }
-        dd_.recXYA = static_cast<T*>(own(dupload<T>(recXY, s_)));
-        dd_.recXYB = static_cast<T*>(own(dupload<T>(recXY, s_)));
-        dd_.recPos = static_cast<int*>(own(dupload<int>(recPos, s_)));
+        dd_.recXYA = static_cast<T*>(own(dupload<T>(recXY, s_, &staging)));
+        dd_.recXYB = static_cast<T*>(own(dupload<T>(recXY, s_, &staging)));
+        dd_.recPos = static_cast<int*>(own(dupload<int>(recPos, s_, &staging)));
     }
     {
         TileGrid& tg = dd_.tiles;
@@ -667,8 +737,8 @@ void Engine<T>::build(const sp_hip_design& d) {
         parea[k] = static_cast<double>(d.node_w[i]) * static_cast<double>(d.node_h[i]);
         pdeg[k] = static_cast<float>(d.node2pin_start[i + 1] - d.node2pin_start[i]);
     }
-    dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_)));
-    dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_)));
+    dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_, &staging)));
+    dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_, &staging)));
     // Nets are renumbered by WA size class (degree 2, 3-4, 5-8, ..., then
     // larger), so each class is a contiguous range of nets and of pin slots,
     // and inside a class by their lowest node index, so that neighbouring
@@ -731,10 +801,10 @@ void Engine<T>::build(const sp_hip_design& d) {
         dd_.waWaveBase[kWaClasses + 1] =
             dd_.waWaveBase[kWaClasses] + (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]);
     }
-    dd_.netStart = static_cast<int*>(own(dupload<int>(netStart, s_)));
-    dd_.slotNode = static_cast<int*>(own(dupload<int>(slotNode, s_)));
-    dd_.slotOffX = static_cast<T*>(own(dupload<T>(slotOffX, s_)));
-    dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_)));
+    dd_.netStart = static_cast<int*>(own(dupload<int>(netStart, s_, &staging)));
+    dd_.slotNode = static_cast<int*>(own(dupload<int>(slotNode, s_, &staging)));
+    dd_.slotOffX = static_cast<T*>(own(dupload<T>(slotOffX, s_, &staging)));
+    dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_, &staging)));
     // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
     // of pins that no net pass writes (no net, degree < 2 or above
     // ignore_net_degree) stay zero.
@@ -744,8 +814,8 @@ void Engine<T>::build(const sp_hip_design& d) {
         const int p = d.flat_node2pin[q];
         if (pinSlot[p] >= 0) slotNp[pinSlot[p]] = q;
     }
-    dd_.nodeStart = static_cast<int*>(own(dupload<int>(nodeStart, s_)));
-    dd_.slotNp = static_cast<int*>(own(dupload<int>(slotNp, s_)));
+    dd_.nodeStart = static_cast<int*>(own(dupload<int>(nodeStart, s_, &staging)));
+    dd_.slotNp = static_cast<int*>(own(dupload<int>(slotNp, s_, &staging)));
     dd_.pinGrad = static_cast<T*>(own(dalloc<T>(2 * static_cast<size_t>(std::max(1, d.num_pins)))));
     SPHIP_CHECK(hipMemsetAsync(dd_.pinGrad, 0, 2 * static_cast<size_t>(std::max(1, d.num_pins)) * sizeof(T), s_));
 
@@ -1521,6 +1591,157 @@ private:
 
 }  // namespace
 
+// Many FP32 stages advanced together by batched launches: one hipGraph of
+// `slots` batched slots (every kernel of a slot launched once for all the
+// instances, the FFT passes once per grid size) is launched until every
+// instance's stage has ended; each instance's result equals its own run's.
+void runBatch(const std::vector<EngineBase*>& engines, const std::vector<char>& looping,
+              sp_hip_gp_result* res, int slots) {
+    const int n = static_cast<int>(engines.size());
+    std::vector<Engine<float>*> live;
+    std::vector<int> liveAt;
+    for (int i = 0; i < n; ++i) {
+        auto* e = dynamic_cast<Engine<float>*>(engines[i]);
+        if (!e) throw InvalidArg("batched runs take FP32 instances");
+        if (!looping[i]) continue;   // the stage ended before its loop
+        live.push_back(e);
+        liveAt.push_back(i);
+    }
+    const int m = static_cast<int>(live.size());
+    if (m == 0) return;
+    slots = std::max(1, slots);
+    // instance shares and block maps
+    std::vector<SlotArgs<float>> args(m);
+    std::vector<PoissonBatchArgs<float>> pargs(m);
+    std::vector<std::vector<int4>> maps(BK_KINDS);
+    std::vector<int> dims;
+    std::vector<std::array<std::vector<int4>, 3>> fftMaps;   // per grid size, per pass
+    int* doneDev = nullptr;
+    SPHIP_CHECK(hipMalloc(reinterpret_cast<void**>(&doneDev), m * sizeof(int)));
+    int* doneHost = nullptr;
+    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&doneHost), 2 * m * sizeof(int),
+                              hipHostMallocDefault));
+    std::vector<void*> owned{doneDev};
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto cleanup = [&] {
+        if (s) (void)hipStreamSynchronize(s);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+        if (doneHost) (void)hipHostFree(doneHost);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    try {
+        long maxLaunches = 0;
+        for (int j = 0; j < m; ++j) {
+            int counts[BK_KINDS];
+            live[j]->batchShare(&args[j], &pargs[j], counts, doneDev);
+            for (int k = 0; k < BK_KINDS; ++k)
+                for (int b = 0; b < counts[k]; ++b) maps[k].push_back(int4{j, b, counts[k], 0});
+            const int dim = live[j]->gridDim();
+            int gi = static_cast<int>(std::find(dims.begin(), dims.end(), dim) - dims.begin());
+            if (gi == static_cast<int>(dims.size())) {
+                dims.push_back(dim);
+                fftMaps.emplace_back();
+            }
+            for (int pass = 0; pass < 3; ++pass) {
+                const int c = poissonBatchBlocks(pass, dim, dim);
+                for (int b = 0; b < c; ++b) fftMaps[gi][pass].push_back(int4{j, b, c, 0});
+            }
+            maxLaunches = std::max(maxLaunches, live[j]->maxLoopLaunches(slots));
+        }
+        SPHIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        auto upload = [&](const void* h, size_t bytes) {
+            void* d = nullptr;
+            SPHIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));
+            owned.push_back(d);
+            if (bytes) SPHIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+            return d;
+        };
+        auto* dArgs = static_cast<SlotArgs<float>*>(upload(args.data(), m * sizeof(SlotArgs<float>)));
+        auto* dPargs = static_cast<PoissonBatchArgs<float>*>(
+            upload(pargs.data(), m * sizeof(PoissonBatchArgs<float>)));
+        std::vector<int4*> dMaps(BK_KINDS);
+        for (int k = 0; k < BK_KINDS; ++k)
+            dMaps[k] = static_cast<int4*>(upload(maps[k].data(), maps[k].size() * sizeof(int4)));
+        std::vector<std::array<int4*, 3>> dFft(dims.size());
+        std::vector<int> fftLd(dims.size());
+        for (size_t g = 0; g < dims.size(); ++g) {
+            for (int pass = 0; pass < 3; ++pass)
+                dFft[g][pass] = static_cast<int4*>(
+                    upload(fftMaps[g][pass].data(), fftMaps[g][pass].size() * sizeof(int4)));
+            // every instance of a grid size has the same padded row stride
+            for (int j = 0; j < m; ++j)
+                if (live[j]->gridDim() == dims[g]) {
+                    fftLd[g] = live[j]->gridLd();
+                    break;
+                }
+        }
+        SPHIP_CHECK(hipMemsetAsync(doneDev, 0, m * sizeof(int), s));
+        SPHIP_CHECK(hipStreamSynchronize(s));
+        auto count = [&](int k) { return static_cast<int>(maps[k].size()); };
+        // capture `slots` batched slots
+        SPHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int slot = 0; slot < slots; ++slot) {
+                const int sorted = slot % kSortPeriod == 0 ? 1 : 0;
+                if (sorted) {
+                    launchSlotBatch<float>(BK_SORT_COUNT, dArgs, dMaps[BK_SORT_COUNT],
+                                           count(BK_SORT_COUNT), 1, s);
+                    launchSlotBatch<float>(BK_SORT_PLACE, dArgs, dMaps[BK_SORT_PLACE],
+                                           count(BK_SORT_PLACE), 1, s);
+                }
+                for (int k : {BK_SCATTER, BK_FINALIZE})
+                    launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
+                for (size_t g = 0; g < dims.size(); ++g)
+                    for (int pass = 0; pass < 3; ++pass)
+                        launchPoissonBatch<float>(pass, dPargs, dFft[g][pass],
+                                                  static_cast<int>(fftMaps[g][pass].size()),
+                                                  dims[g], dims[g], fftLd[g], s);
+                for (int k : {BK_WA, BK_NODE, BK_REDUCE, BK_UPDATE})
+                    launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
+            }
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SPHIP_CHECK(hipStreamEndCapture(s, &graph));
+        SPHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (hipEvent_t& e : ev) SPHIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // two launches in flight; the host reads the done flags of the older
+        long launched = 0;
+        auto launch = [&](int k) {
+            if (++launched > maxLaunches) throw std::runtime_error("batched loop did not terminate");
+            SPHIP_CHECK(hipGraphLaunch(exec, s));
+            SPHIP_CHECK(hipMemcpyAsync(doneHost + k * m, doneDev, m * sizeof(int),
+                                       hipMemcpyDeviceToHost, s));
+            SPHIP_CHECK(hipEventRecord(ev[k], s));
+        };
+        launch(0);
+        launch(1);
+        for (int k = 0;; k ^= 1) {
+            SPHIP_CHECK(hipEventSynchronize(ev[k]));
+            bool all = true;
+            for (int j = 0; j < m && all; ++j) all = doneHost[k * m + j] != 0;
+            if (all) break;
+            launch(k);
+        }
+        SPHIP_CHECK(hipStreamSynchronize(s));
+        for (int j = 0; j < m; ++j) live[j]->finishAsync(&res[liveAt[j]]);
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
 std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
     if (cfg.precision != 32 && cfg.precision != 64)
         throw InvalidArg("precision must be 32 or 64");
@@ -1659,6 +1880,46 @@ int sp_hip_gp_run(sp_hip_gp* gp, sp_hip_gp_result* res, sp_hip_iter_metrics* row
     return guarded([&] {
         if (!gp || !res) throw sphip::InvalidArg("null argument");
         gp->impl->run(res, rows, rows ? max_rows : 0);
+    });
+}
+
+int sp_hip_gp_run_batch(int count, const sp_hip_design* designs, const sp_hip_gp_config* cfgs,
+                        int slots, sp_hip_gp_result* results, float* const* x, float* const* y) {
+    return guarded([&] {
+        if (count < 0 || (count > 0 && (!designs || !cfgs || !results)))
+            throw sphip::InvalidArg("batch: null argument");
+        for (int i = 0; i < count; ++i)
+            if (cfgs[i].precision != 32) throw sphip::InvalidArg("batch: instances must be FP32");
+        // set-ups (design upload, the stage's first evaluations) on host
+        // threads, each instance on its own stream
+        std::vector<std::unique_ptr<sphip::EngineBase>> engines(count);
+        std::vector<char> looping(count, 0);
+        std::vector<std::exception_ptr> errs(count);
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+                try {
+                    sp_hip_gp_config c = cfgs[i];
+                    c.loop_mode = 1;   // the batch drives the loop; no per-instance graph
+                    engines[i] = sphip::makeEngine(designs[i], c);
+                    looping[i] = engines[i]->begin(&results[i]) ? 0 : 1;
+                } catch (...) {
+                    errs[i] = std::current_exception();
+                }
+            }
+        };
+        const int threads = std::max(1, std::min(8, count));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
+        std::vector<sphip::EngineBase*> view;
+        for (auto& e : engines) view.push_back(e.get());
+        sphip::runBatch(view, looping, results, slots > 0 ? slots : 8);
+        for (int i = 0; i < count; ++i)
+            if (x && y && x[i] && y[i]) engines[i]->positions(x[i], y[i]);
     });
 }
 

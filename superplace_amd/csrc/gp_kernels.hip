@@ -7,7 +7,9 @@
 // reduction uses atomics or fences, so results are bitwise reproducible and
 // no block has to publish its writes across the chip's L2 caches mid-kernel.
 #include <cmath>
+#include <stdexcept>
 
+#include "batch.h"
 #include "gp_kernels.h"
 #include "wa_device.h"
 #include "density_device.h"
@@ -166,13 +168,15 @@ __device__ inline int blockExclusiveScan(const int* src, int n, int* dst, int* w
 
 // Both passes stride over chunks of kBlock records with a bounded grid, so
 // that a skipped sort (ctl->sortNext clear) costs a small launch.
+// Block `vb` of `nvb` (the kernels below take their block index explicitly,
+// so that a batched launch can run the blocks of several instances).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<T> g,
-                                                       const Ctl* ctl) {
+__device__ inline void tileCountBody(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl,
+                                     int vb, int nvb) {
     if (ctl && (ctl->done || !ctl->sortNext)) return;
     __shared__ TileHist h;
     const int chunks = (d.nDens + kBlock - 1) / kBlock;
-    for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+    for (int chunk = vb; chunk < chunks; chunk += nvb) {
         int rec = 0;
         typename Vec2<T>::type p;
         blockTileHist<T>(d, g, h, chunk, &rec, &p);
@@ -184,12 +188,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<T> g,
+                                                       const Ctl* ctl) {
+    tileCountBody<T>(d, g, ctl, blockIdx.x, gridDim.x);
+}
+
 // Pass 2: every block scans the counts itself (a few KB from L2), so no
 // separate scan launch is needed.  Block 0 also publishes the key starts and
 // the extra scatter blocks of crowded tiles.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<T> g,
-                                                       const Ctl* ctl) {
+__device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl,
+                                     int vb, int nvb) {
     if (ctl && (ctl->done || !ctl->sortNext)) return;
     __shared__ TileHist h;
     __shared__ int prefix[kTileKeys];
@@ -197,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
     const TileGrid& tg = d.tiles;
     blockExclusiveScan(tg.count, tg.nKeys, prefix, waveTot);
     const int chunks = (d.nDens + kBlock - 1) / kBlock;
-    for (int chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+    for (int chunk = vb; chunk < chunks; chunk += nvb) {
         int rec = 0;
         typename Vec2<T>::type p;
         const unsigned key = blockTileHist<T>(d, g, h, chunk, &rec, &p);
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
         }
         __syncthreads();
     }
-    if (blockIdx.x == 0) {
+    if (vb == 0) {
         for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
             tg.start[k] = prefix[k];
             tg.size[k] = tg.count[k];
@@ -234,6 +244,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<T> g,
+                                                       const Ctl* ctl) {
+    tilePlaceBody<T>(d, g, ctl, blockIdx.x, gridDim.x);
+}
+
 // ------------------------------------------------------------------ (a) scatter
 // Blocks [0, tiles) own one tile each and take its first kTileChunk nodes;
 // blocks past that take chunk j >= 1 of a crowded tile (found in extraStart).
@@ -244,10 +260,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
 // halo) go to the atomic grids directly.  The block also copies its part of
 // the records back (A = B) for the next sort.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
-                                                    unsigned long long* nodeQ,
-                                                    unsigned long long* fillQ, int sorted,
-                                                    double* partials, const Ctl* ctl) {
+__device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
+                                   unsigned long long* nodeQ, unsigned long long* fillQ,
+                                   int sorted, double* partials, const Ctl* ctl, int vb) {
     if (ctl) {
         if (ctl->done) return;
         sorted = sorted && ctl->sortNext;
@@ -258,15 +273,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
     const TileGrid& tg = d.tiles;
     const int nTiles = tg.tilesX * tg.tilesY;
     int tx, ty, key, chunk;
-    if (static_cast<int>(blockIdx.x) < nTiles) {
-        tx = blockIdx.x / tg.tilesY;
-        ty = blockIdx.x - tx * tg.tilesY;
+    if (vb < nTiles) {
+        tx = vb / tg.tilesY;
+        ty = vb - tx * tg.tilesY;
         key = static_cast<int>(mortonKey(tx, ty));
         chunk = 0;
     } else {
-        const int e = blockIdx.x - nTiles;
+        const int e = vb - nTiles;
         if (e >= tg.extraStart[tg.nKeys]) {   // no chunk for this block
-            if (threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+            if (threadIdx.x == 0) partials[vb] = 0.0;
             return;
         }
         int lo = 0, hi = tg.nKeys - 1;   // last key with extraStart <= e
@@ -334,9 +349,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
             }
         if (out) spill[0] += 1.0;
     }
-    blockPartial<1>(spill, partials);   // ends with a barrier
+    blockPartial<1>(spill, partials, vb);   // ends with a barrier
     if (chunk == 0) {
-        unsigned long long* out = tg.stage + static_cast<long>(blockIdx.x) * 2 * WIN2;
+        unsigned long long* out = tg.stage + static_cast<long>(vb) * 2 * WIN2;
         for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) out[e] = win[e];
     } else {
         for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) {
@@ -349,13 +364,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> 
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
+                                                    unsigned long long* nodeQ,
+                                                    unsigned long long* fillQ, int sorted,
+                                                    double* partials, const Ctl* ctl) {
+    scatterBody<T>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, blockIdx.x);
+}
+
 // ------------------------------------------------------------------ rho + tau
 // finalizeBin over every bin (density_device.h), rho into the padded layout,
 // one tau partial per block; clears the tile counters.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_finalize(DensityIn<T> in, T* rho, const Ctl* ctl) {
+__device__ inline void finalizeBody(const DensityIn<T>& in, T* rho, const Ctl* ctl, int vb) {
     if (ctl && ctl->done) return;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = vb * kBlock + threadIdx.x;
     const int nb = in.g.dim * in.g.dim;
     double v[1] = {0.0};
     clearTileKeys(in.tg, b);
@@ -364,7 +387,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(DensityIn<T> in, T* rho, co
         const int bx = b / in.g.dim, by = b - bx * in.g.dim;
         rho[static_cast<long>(bx) * in.g.ld + by] = r;
     }
-    blockPartial<1>(v, in.partials);
+    blockPartial<1>(v, in.partials, vb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(DensityIn<T> in, T* rho, const Ctl* ctl) {
+    finalizeBody<T>(in, rho, ctl, blockIdx.x);
 }
 
 // WaWirelength::compute (wa_wirelength.cpp:119-225) as one pin-parallel pass
@@ -389,11 +417,10 @@ __global__ __launch_bounds__(kBlock) void k_wa_pins(WaTask<T> w) {
 //    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
 //    (nesterov.cpp:25-44), plus a count of non-finite gradients.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* exy,
-                                                      T invG, float lambda,
-                                                      const T* u, const T* up, const T* gp, T* grad,
-                                                      T* dbgForce, T* dbgWl, double* partials,
-                                                      const Ctl* ctl, T* const* bufs) {
+__device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
+                                    T invG, float lambda, const T* u, const T* up, const T* gp,
+                                    T* grad, T* dbgForce, T* dbgWl, double* partials,
+                                    const Ctl* ctl, T* const* bufs, int vb) {
     if (ctl) {
         if (ctl->done) return;
         lambda = ctl->lambda;
@@ -403,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
         gp = bufs[ctl->nxt[B_GP]];
         grad = bufs[ctl->nxt[B_G]];
     }
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = vb * kBlock + threadIdx.x;
     const int n = d.nOpt;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     if (k < n) {
@@ -483,7 +510,17 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
         v[5] = static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1;
         v[6] = (isfinite(g0) ? 0.0 : 1.0) + (isfinite(g1) ? 0.0 : 1.0);
     }
-    blockPartial<7>(v, partials);
+    blockPartial<7>(v, partials, vb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T> g, const T* exy,
+                                                      T invG, float lambda,
+                                                      const T* u, const T* up, const T* gp, T* grad,
+                                                      T* dbgForce, T* dbgWl, double* partials,
+                                                      const Ctl* ctl, T* const* bufs) {
+    nodeGradBody<T>(d, g, exy, invG, lambda, u, up, gp, grad, dbgForce, dbgWl, partials, ctl,
+                    bufs, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ (d) decisions
@@ -668,8 +705,8 @@ __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
 // [tau, wl, hpwl, sum_wl, sum_den, dx2, dg2, sy, gg, nonfinite], then (device
 // loop) takes the decisions.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, int withNode,
-                                                   Ctl* ctl, DevRow* rows) {
+__device__ inline void reduceBody(const ReduceWs& ws, double denom, int withNode, Ctl* ctl,
+                                  DevRow* rows) {
     if (ctl && ctl->done) return;
     // the decision thread works on an LDS copy of the loop state: one
     // cooperative copy in and out instead of dependent global round trips
@@ -731,14 +768,20 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
             reinterpret_cast<unsigned*>(ctl)[w] = ctlWords[w];
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, int withNode,
+                                                   Ctl* ctl, DevRow* rows) {
+    reduceBody<T>(ws, denom, withNode, ctl, rows);
+}
+
 // ------------------------------------------------------------------ (d) update
 // NesterovOptimizer::step / redoIfOvershot arithmetic (nesterov.cpp:97-107,
 // :156-165) with EPlace's clamp (eplace.cpp:117-133) and scatter (:97-105).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
-                                                   const T* B, T alpha, T m, T* vOut, T* uOut,
-                                                   const Ctl* ctl, T* const* bufs) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G, const T* B,
+                                  T alpha, T m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs,
+                                  int vb) {
+    const int k = vb * kBlock + threadIdx.x;
     const int n = d.nOpt;
     if (ctl) {
         // After `done` the mode and indices stay fixed, so a repeated update
@@ -789,6 +832,13 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
 }
 
 template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
+                                                   const T* B, T alpha, T m, T* vOut, T* uOut,
+                                                   const Ctl* ctl, T* const* bufs) {
+    updateBody<T>(d, A, G, B, alpha, m, vOut, uOut, ctl, bufs, blockIdx.x);
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rec_refresh(DevDesign<T> d) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.nDens) return;
@@ -798,7 +848,72 @@ __global__ __launch_bounds__(kBlock) void k_rec_refresh(DevDesign<T> d) {
 
 long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
 
+// ------------------------------------------------------------------ batches
+template <typename T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_slot_batch(const SlotArgs<T>* __restrict__ args,
+                                                       const int4* __restrict__ map, int sorted) {
+    const int4 m = map[blockIdx.x];
+    const SlotArgs<T>& a = args[m.x];
+    if constexpr (KIND == BK_SORT_COUNT) {
+        tileCountBody<T>(a.d, a.g, a.ctl, m.y, m.z);
+    } else if constexpr (KIND == BK_SORT_PLACE) {
+        tilePlaceBody<T>(a.d, a.g, a.ctl, m.y, m.z);
+    } else if constexpr (KIND == BK_SCATTER) {
+        scatterBody<T>(a.d, a.g, a.qscale, a.nodeQ, a.fillQ, sorted, a.ws.region(2), a.ctl, m.y);
+    } else if constexpr (KIND == BK_FINALIZE) {
+        finalizeBody<T>(a.din, a.rho, a.ctl, m.y);
+    } else if constexpr (KIND == BK_WA) {
+        if (a.ctl->done) return;
+        waBlockTask<T>(a.wa, m.y, m.z);
+    } else if constexpr (KIND == BK_NODE) {
+        nodeGradBody<T>(a.d, a.g, a.exy, T(0), 0.f, nullptr, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, a.ws.region(3), a.ctl, a.bufs, m.y);
+    } else if constexpr (KIND == BK_REDUCE) {
+        reduceBody<T>(a.ws, a.denom, 1, a.ctl, a.rows);
+        __syncthreads();
+        if (threadIdx.x == 0) a.doneOut[m.x] = a.ctl->done;
+    } else if constexpr (KIND == BK_UPDATE) {
+        updateBody<T>(a.d, nullptr, nullptr, nullptr, T(0), T(0), nullptr, nullptr, a.ctl, a.bufs,
+                      m.y);
+    }
+}
+
 }  // namespace
+
+template <typename T>
+void launchSlotBatch(int kind, const SlotArgs<T>* args, const int4* map, int blocks, int sorted,
+                     hipStream_t s) {
+    if (blocks <= 0) return;
+    switch (kind) {
+        case BK_SORT_COUNT:
+            launchKernel(k_slot_batch<T, BK_SORT_COUNT>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_SORT_PLACE:
+            launchKernel(k_slot_batch<T, BK_SORT_PLACE>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_SCATTER:
+            launchKernel(k_slot_batch<T, BK_SCATTER>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_FINALIZE:
+            launchKernel(k_slot_batch<T, BK_FINALIZE>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_WA:
+            launchKernel(k_slot_batch<T, BK_WA>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_NODE:
+            launchKernel(k_slot_batch<T, BK_NODE>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_REDUCE:
+            launchKernel(k_slot_batch<T, BK_REDUCE>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        case BK_UPDATE:
+            launchKernel(k_slot_batch<T, BK_UPDATE>, blocks, kBlock, 0, s, args, map, sorted);
+            break;
+        default:
+            throw std::logic_error("launchSlotBatch: unknown kernel");
+    }
+    SPHIP_CHECK(hipGetLastError());
+}
 
 template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
@@ -900,5 +1015,7 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
                                   double, T*, T*, const Ctl*, T* const*, hipStream_t);
 SPHIP_INST(float)
 SPHIP_INST(double)
+template void launchSlotBatch<float>(int, const SlotArgs<float>*, const int4*, int, int,
+                                     hipStream_t);
 
 }  // namespace sphip

@@ -29,6 +29,23 @@ struct DensityIn;
 // launch reserve it, so a larger one starves the WA blocks of occupancy).
 constexpr size_t kFusedLdsMax = 32 * 1024;
 
+// One instance's buffers and tables for a batched launch of the FFT passes
+// (instances of the same grid size; see launchPoissonBatch).
+template <typename T>
+struct PoissonBatchArgs {
+    const T* rho;
+    T* t0;
+    T* t1;
+    T* t2;
+    T* exy;             // interleaved field out
+    const T* tabX;      // per axis: twiddles, then half-angle factors (complex)
+    const T* tabY;
+    const T* u;
+    const T* v;
+    T invNorm;
+    const int* skip;    // the instance's done flag
+};
+
 template <typename T>
 class PoissonPlan {
 public:
@@ -76,6 +93,8 @@ public:
     double bytes(bool withPhi) const;
     // Power-of-two grids (4 .. 1024 per side) take the FFT path.
     bool usesFft() const { return fft_; }
+    // The plan's buffers for a batched solve (interleaved FFT plans only).
+    PoissonBatchArgs<T> batchArgs(const int* skip) const;
     static bool fftShape(int n);
 
 private:
@@ -110,6 +129,19 @@ private:
 size_t fftSolveWorkFloats(int nx, int ny);
 void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, float* out3,
                   float* work, hipStream_t s);
+
+// Pass `pass` (0: rows, 1: columns with the spectral step, 2: inverse rows
+// into the interleaved field) of the FFT solve for many instances of one
+// nx x ny grid (row stride ld) in one launch: block b runs block map[b].y of
+// instance map[b].x's pass (of map[b].z blocks), exactly as the plan's own
+// launch of that pass with one column pair per block does.  `blocks` is
+// the total; blocks() gives an instance's count per pass.
+template <typename T>
+void launchPoissonBatch(int pass, const PoissonBatchArgs<T>* args, const int4* map, int blocks,
+                        int nx, int ny, int ld, hipStream_t s);
+inline int poissonBatchBlocks(int pass, int nx, int ny) {
+    return pass == 0 ? nx / 2 : (pass == 1 ? ny / 2 : nx);
+}
 
 // Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on
 // MFMA; the plan's primitive, exposed for the GEMM unit test.

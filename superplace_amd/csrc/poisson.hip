@@ -917,38 +917,39 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
         tabY_ = tables(ny);
         setFftLds();
     }
-    // Transform matrices in FP64, zero in the padding (cpu_poisson.cpp:14-36).
-    std::vector<double> cyT(static_cast<size_t>(my_) * my_, 0.0), ayT(cyT.size(), 0.0),
-        bsv(cyT.size(), 0.0);
-    std::vector<double> cx(static_cast<size_t>(mx_) * mx_, 0.0), sxu(cx.size(), 0.0),
-        ax(cx.size(), 0.0);
     std::vector<double> u(mx_, 0.0), v(my_, 0.0);
     for (int j = 0; j < nx; ++j) u[j] = pi * j / (nx * stepX);
     for (int k = 0; k < ny; ++k) v[k] = pi * k / (ny * stepY);
-    // The products take their right operand transposed (k contiguous), so the
-    // y-axis matrices are stored as [output index][summed index].
-    for (int y = 0; y < ny; ++y)
-        for (int k = 0; k < ny; ++k) {
-            // forward REDFT10 along y: T0[x][k] = sum_y rho[x][y] * 2 cos(pi k (y+1/2)/ny)
-            cyT[static_cast<size_t>(k) * my_ + y] = 2.0 * std::cos(pi * k * (y + 0.5) / ny);
-            // inverse REDFT01 along y: T1[j][y] = sum_k P[j][k] c_k cos(pi k (y+1/2)/ny)
-            ayT[static_cast<size_t>(y) * my_ + k] =
-                (k == 0 ? 1.0 : 2.0) * std::cos(pi * k * (y + 0.5) / ny);
-            // RODFT01 along y of P shifted left and scaled by v (cpu_poisson.cpp:197-206):
-            // T2[j][y] = sum_{k>=1} P[j][k] 2 v_k sin(pi k (y+1/2)/ny)
-            bsv[static_cast<size_t>(y) * my_ + k] =
-                k == 0 ? 0.0 : 2.0 * v[k] * std::sin(pi * k * (y + 0.5) / ny);
-        }
-    for (int x = 0; x < nx; ++x)
-        for (int j = 0; j < nx; ++j) {
-            cx[static_cast<size_t>(j) * mx_ + x] = 2.0 * std::cos(pi * j * (x + 0.5) / nx);
-            ax[static_cast<size_t>(x) * mx_ + j] =
-                (j == 0 ? 1.0 : 2.0) * std::cos(pi * j * (x + 0.5) / nx);
-            // RODFT01 along x of P shifted down and scaled by u (cpu_poisson.cpp:181-195)
-            sxu[static_cast<size_t>(x) * mx_ + j] =
-                j == 0 ? 0.0 : 2.0 * u[j] * std::sin(pi * j * (x + 0.5) / nx);
-        }
     if (!fft_) {
+        // Transform matrices in FP64, zero in the padding (cpu_poisson.cpp:14-36);
+        // only the dense path multiplies by them.
+        std::vector<double> cyT(static_cast<size_t>(my_) * my_, 0.0), ayT(cyT.size(), 0.0),
+            bsv(cyT.size(), 0.0);
+        std::vector<double> cx(static_cast<size_t>(mx_) * mx_, 0.0), sxu(cx.size(), 0.0),
+            ax(cx.size(), 0.0);
+        // The products take their right operand transposed (k contiguous), so the
+        // y-axis matrices are stored as [output index][summed index].
+        for (int y = 0; y < ny; ++y)
+            for (int k = 0; k < ny; ++k) {
+                // forward REDFT10 along y: T0[x][k] = sum_y rho[x][y] * 2 cos(pi k (y+1/2)/ny)
+                cyT[static_cast<size_t>(k) * my_ + y] = 2.0 * std::cos(pi * k * (y + 0.5) / ny);
+                // inverse REDFT01 along y: T1[j][y] = sum_k P[j][k] c_k cos(pi k (y+1/2)/ny)
+                ayT[static_cast<size_t>(y) * my_ + k] =
+                    (k == 0 ? 1.0 : 2.0) * std::cos(pi * k * (y + 0.5) / ny);
+                // RODFT01 along y of P shifted left and scaled by v (cpu_poisson.cpp:197-206):
+                // T2[j][y] = sum_{k>=1} P[j][k] 2 v_k sin(pi k (y+1/2)/ny)
+                bsv[static_cast<size_t>(y) * my_ + k] =
+                    k == 0 ? 0.0 : 2.0 * v[k] * std::sin(pi * k * (y + 0.5) / ny);
+            }
+        for (int x = 0; x < nx; ++x)
+            for (int j = 0; j < nx; ++j) {
+                cx[static_cast<size_t>(j) * mx_ + x] = 2.0 * std::cos(pi * j * (x + 0.5) / nx);
+                ax[static_cast<size_t>(x) * mx_ + j] =
+                    (j == 0 ? 1.0 : 2.0) * std::cos(pi * j * (x + 0.5) / nx);
+                // RODFT01 along x of P shifted down and scaled by u (cpu_poisson.cpp:181-195)
+                sxu[static_cast<size_t>(x) * mx_ + j] =
+                    j == 0 ? 0.0 : 2.0 * u[j] * std::sin(pi * j * (x + 0.5) / nx);
+            }
         cyT_ = upload<T>(cyT, stream);
         ayT_ = upload<T>(ayT, stream);
         bsv_ = upload<T>(bsv, stream);
@@ -1242,8 +1243,74 @@ void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, fl
     SPHIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ batches
+namespace {
+
+template <typename T, int PASS>
+__global__ __launch_bounds__(kBlock) void k_fft_batch(const PoissonBatchArgs<T>* __restrict__ args,
+                                                      const int4* __restrict__ map, int nx, int ny,
+                                                      int ld) {
+    const int4 m = map[blockIdx.x];
+    const PoissonBatchArgs<T>& a = args[m.x];
+    if (a.skip && *a.skip) return;
+    using C2 = cplx<T>;
+    const FftTab<T> tx{reinterpret_cast<const C2*>(a.tabX), reinterpret_cast<const C2*>(a.tabX) + nx};
+    const FftTab<T> ty{reinterpret_cast<const C2*>(a.tabY), reinterpret_cast<const C2*>(a.tabY) + ny};
+    if constexpr (PASS == 0)
+        dctRowsBlock<T>(a.rho, a.t0, nx, ny, ld, ty, m.y);
+    else if constexpr (PASS == 1)
+        dctColsBlock<T, true>(a.t0, a.t1, a.t2, 1, nx, ny, ld, tx, a.u, a.v, a.invNorm,
+                              xcdBlock(m.y, m.z));
+    else
+        idctRowsBlock<T>(a.t1, a.t2, a.exy, a.exy + 1, static_cast<T*>(nullptr), 2, nx, ny, ld, ty,
+                         a.v, m.y);
+}
+
+}  // namespace
+
+template <typename T>
+PoissonBatchArgs<T> PoissonPlan<T>::batchArgs(const int* skip) const {
+    if (!fft_ || !interleaved_) throw std::logic_error("batched solve: FFT plan with interleaved field only");
+    return PoissonBatchArgs<T>{rho_, t0_, t1_, t2_, exy_, tabX_, tabY_, u_, v_, invNorm_, skip};
+}
+
+template <typename T>
+void launchPoissonBatch(int pass, const PoissonBatchArgs<T>* args, const int4* map, int blocks,
+                        int nx, int ny, int ld, hipStream_t s) {
+    if (blocks <= 0) return;
+    const size_t c = 2 * sizeof(T);
+    // the plan's LDS with one column pair per block (setFftLds), the inverse
+    // rows without phi
+    const size_t ldsA = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
+    const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
+    const size_t ldsC = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        // the largest grid's LDS (1024 per side) covers every smaller one
+        const size_t big = (kFftLdsHead(1024) + 4 * 1024) * 2 * sizeof(T);
+        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 0>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 1>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 2>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
+    });
+    if (pass == 0)
+        launchKernel(k_fft_batch<T, 0>, dim3(blocks), dim3(fftThreads(ny)),
+                     static_cast<uint32_t>(ldsA), s, args, map, nx, ny, ld);
+    else if (pass == 1)
+        launchKernel(k_fft_batch<T, 1>, dim3(blocks), dim3(fftThreads(nx)),
+                     static_cast<uint32_t>(ldsB), s, args, map, nx, ny, ld);
+    else
+        launchKernel(k_fft_batch<T, 2>, dim3(blocks), dim3(fftThreads(ny)),
+                     static_cast<uint32_t>(ldsC), s, args, map, nx, ny, ld);
+    SPHIP_CHECK(hipGetLastError());
+}
+
 template class PoissonPlan<float>;
 template class PoissonPlan<double>;
+template void launchPoissonBatch<float>(int, const PoissonBatchArgs<float>*, const int4*, int, int,
+                                        int, int, hipStream_t);
 template void launchGemm<float>(int, const float* const*, const float* const*, float* const*,
                                 int, int, int, int, int, int, hipStream_t);
 template void launchGemm<double>(int, const double* const*, const double* const*,

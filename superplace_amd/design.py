@@ -420,3 +420,35 @@ def fixture_eplace(num_cells: int, num_macros: int, seed: int = 7) -> Design:
 
 def with_fillers(d: Design, target_density: float = 1.0, seed: int = 1002) -> Design:
     return add_fillers(d, target_density, seed)
+
+
+def auto_bin_dim(d: Design, lo: int = 4, hi: int = 1024) -> int:
+    """The reference's automatic bin dimension (bin_grid.cpp:172-189: the
+    largest power of two whose bins hold one average movable node at target
+    density 1), clamped to [lo, hi]."""
+    mov = np.arange(d.num_movable)
+    keep = (d.node_flags[mov] & (F_FIXED | F_NI)) == 0
+    area = float(np.sum(d.node_w[mov][keep].astype(np.float64) * d.node_h[mov][keep]))
+    ideal = (d.core[2] - d.core[0]) * (d.core[3] - d.core[1]) / max(area / max(d.num_movable, 1), 1e-9)
+    dim = 4
+    while dim * 2 <= 1024 and (dim * 2) ** 2 <= ideal:
+        dim *= 2
+    return min(max(dim, lo), hi)
+
+
+def job_designs(count: int, lo: int, hi: int, seed0: int, dim_lo: int, dim_hi: int,
+                nets_per_cell: float = 1.05):
+    """The many-instance workloads: `count` seeded synthetic designs with cell
+    counts log-uniform in [lo, hi] (drawn from the splitmix64 stream of seed
+    1002 + seed0), the config-1 degree distribution, `nets_per_cell` nets per
+    cell, 57% utilisation at adaptec1's cell density; each with its bin
+    dimension by the automatic rule clamped to [dim_lo, dim_hi].  Returns
+    [(design, bins per side)]."""
+    u = Rng(1002 + seed0).uniform(count)
+    out = []
+    for k in range(count):
+        cells = int(round(lo * (hi / lo) ** float(u[k])))
+        d = synth_design(cells, max(8, cells // 400), int(cells * nets_per_cell),
+                         25.25 * np.sqrt(cells), 0.57, seed=seed0 + k + 1, name=f"job{k}")
+        out.append((d, auto_bin_dim(d, dim_lo, dim_hi)))
+    return out

@@ -369,6 +369,29 @@ class _DesignArrays:
                              **{k: _ptr(v) for k, v in self.arrs.items()})
 
 
+def run_batch(designs, cfgs=None, slots=8, positions=True):
+    """Many FP32 stages advanced together in batched launches
+    (sp_hip_gp_run_batch): one result dict per design, and its positions
+    (x, y of all nodes) when `positions`; equal to each design's own run."""
+    n = len(designs)
+    cfgs = list(cfgs) if cfgs is not None else [GpConfig() for _ in designs]
+    arrays = [_DesignArrays(d) for d in designs]
+    ds = (SpHipDesign * max(n, 1))(*[a.s for a in arrays])
+    cs = (SpHipGpConfig * max(n, 1))(*[c.struct() for c in cfgs])
+    res = (SpHipGpResult * max(n, 1))()
+    xs = [np.zeros(d.total_nodes, np.float32) for d in designs] if positions else []
+    ys = [np.zeros(d.total_nodes, np.float32) for d in designs] if positions else []
+    px = (_P * max(n, 1))(*[_ptr(a) for a in xs]) if positions else None
+    py = (_P * max(n, 1))(*[_ptr(a) for a in ys]) if positions else None
+    L = lib()
+    L.sp_hip_gp_run_batch.argtypes = [C.c_int, _P, _P, C.c_int, _P, _P, _P]
+    check(L.sp_hip_gp_run_batch(n, C.cast(ds, _P), C.cast(cs, _P), int(slots), C.cast(res, _P),
+                                C.cast(px, _P) if px is not None else None,
+                                C.cast(py, _P) if py is not None else None))
+    out = [{k: getattr(res[i], k) for k, _ in SpHipGpResult._fields_} for i in range(n)]
+    return (out, xs, ys) if positions else out
+
+
 class GpEngine:
     """One placement stage on the GPU (EPlace constructed for one stage)."""
 
