@@ -2,6 +2,7 @@
 // an entry point become an sp_hip_status and a thread-local message.
 #pragma once
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -55,12 +56,17 @@ inline int guarded(F&& f) {
 }
 
 // Throws NoDevice when no HIP device is visible.
+// Throws NoDevice without a HIP device.  A device seen once stays (the
+// count is taken again only until the first success).
 inline void requireDevice() {
+    static std::atomic<bool> seen{false};
+    if (seen.load(std::memory_order_relaxed)) return;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
         (void)hipGetLastError();
         throw NoDevice("no HIP device available");
     }
+    seen.store(true, std::memory_order_relaxed);
 }
 
 // Checks the counts, CSR spans and index ranges of a design passed in.

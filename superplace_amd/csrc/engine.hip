@@ -29,6 +29,7 @@
 #include "gp_kernels.h"
 #include "batch.h"
 #include "poisson.h"
+#include "devmem.h"
 
 namespace sphip {
 namespace {
@@ -39,25 +40,6 @@ T* dalloc(size_t n) {
     SPHIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
     g_devBytes += std::max<size_t>(n, 1) * sizeof(T);
     return static_cast<T*>(p);
-}
-
-// Host copies an upload reads from until the stream is synchronised.
-using Staging = std::vector<std::vector<unsigned char>>;
-
-// A device copy of h converted to T.  With `keep`, the converted host copy
-// is kept there and the caller synchronises the stream once for all its
-// uploads; without, the upload is synchronised here.
-template <typename T, typename S>
-T* dupload(const std::vector<S>& h, hipStream_t s, Staging* keep = nullptr) {
-    std::vector<unsigned char> bytes(h.size() * sizeof(T));
-    T* t = reinterpret_cast<T*>(bytes.data());
-    for (size_t i = 0; i < h.size(); ++i) t[i] = static_cast<T>(h[i]);
-    T* d = dalloc<T>(h.size());
-    if (!h.empty())
-        SPHIP_CHECK(hipMemcpyAsync(d, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
-    if (keep) keep->push_back(std::move(bytes));
-    else SPHIP_CHECK(hipStreamSynchronize(s));
-    return d;
 }
 
 // computeGamma, src/gp/wa_wirelength.cpp:18-29, with the power taken as 10^e
@@ -130,7 +112,8 @@ public:
     Engine(const sp_hip_design& d, const sp_hip_gp_config& cfg) : cfg_(cfg) {
         requireDevice();
         validateDesign(d);
-        SPHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        SPHIP_CHECK(hipGetDevice(&device_));
+        s_ = takeStream(device_);
         const size_t before = g_devBytes;
         try {
             build(d);
@@ -454,9 +437,11 @@ private:
     }
 
     sp_hip_gp_config cfg_;
-    hipStream_t s_ = nullptr;
+    int device_ = 0;
+    hipStream_t s_ = nullptr;   // from the stream pool
     DevDesign<T> dd_{};
-    std::vector<void*> owned_;
+    DevArena arena_;            // the design, grids and optimizer state
+    void* pinned_ = nullptr;    // pooled pinned block: hScalars_, then hFlag_
     int total_ = 0, n_ = 0, dim_ = 0;
     size_t nb_ = 0;
     double stepX_ = 1, stepY_ = 1, originX_ = 0, originY_ = 0, binArea_ = 1;
@@ -468,7 +453,7 @@ private:
     double *terminal_ = nullptr, *base_ = nullptr;
     std::unique_ptr<PoissonPlan<T>> poisson_;
     ReduceWs ws_{};
-    double* hScalars_ = nullptr;   // pinned
+    double* hScalars_ = nullptr;   // in pinned_
     std::vector<int> optIdxHost_;
     std::vector<T> wHost_, hHost_;
     T *u_ = nullptr, *v_ = nullptr, *uPrev_ = nullptr, *vPrev_ = nullptr, *uNew_ = nullptr,
@@ -482,7 +467,7 @@ private:
     DevRow* dRows_ = nullptr;
     int rowsCap_ = 0;
     T* snapPool_[kMaxSnaps] = {};
-    int* hFlag_ = nullptr;   // pinned copy of ctl->done
+    int* hFlag_ = nullptr;   // in pinned_: copies of ctl->done
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
     int profMask_ = 0;   // bit f: time kernel family f with HIP events
@@ -504,8 +489,7 @@ template <typename T>
 void Engine<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     poisson_.reset();
-    for (void* p : owned_) (void)hipFree(p);
-    owned_.clear();
+    arena_.release();
     for (T* p : snapBufs_) (void)hipFree(p);
     snapBufs_.clear();
     if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
@@ -514,21 +498,21 @@ void Engine<T>::release() {
     graph_ = nullptr;
     if (dRows_) (void)hipFree(dRows_);
     dRows_ = nullptr;
-    if (hFlag_) (void)hipHostFree(hFlag_);
-    hFlag_ = nullptr;
     for (hipEvent_t& ev : loopEv_) {
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
     }
-    if (hScalars_) (void)hipHostFree(hScalars_);
+    givePinned(pinned_);
+    pinned_ = nullptr;
     hScalars_ = nullptr;
+    hFlag_ = nullptr;
     for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
         for (hipEvent_t ev : evStart_[f]) (void)hipEventDestroy(ev);
         for (hipEvent_t ev : evStop_[f]) (void)hipEventDestroy(ev);
         evStart_[f].clear();
         evStop_[f].clear();
     }
-    if (s_) (void)hipStreamDestroy(s_);
+    giveStream(device_, s_);
     s_ = nullptr;
 }
 
@@ -540,8 +524,6 @@ void Engine<T>::build(const sp_hip_design& d) {
     auto isMacro = [&](int i) { return (d.node_flags[i] & F_MACRO) != 0; };
     auto isFixed = [&](int i) { return (d.node_flags[i] & F_FIXED) != 0; };
     auto isNI = [&](int i) { return (d.node_flags[i] & F_NI) != 0; };
-    auto own = [&](void* p) { owned_.push_back(p); return p; };
-    Staging staging;   // host copies of the uploads, until the stream syncs
 
     // buildOptSet, eplace.cpp:82-94.
     if (stage != 1)
@@ -620,8 +602,8 @@ void Engine<T>::build(const sp_hip_design& d) {
             base[static_cast<size_t>(bx) * dim_ + by] = T(target) * freeA;
         }
     }
-    terminal_ = static_cast<double*>(own(dupload<double>(term, s_, &staging)));
-    base_ = static_cast<double*>(own(dupload<double>(base, s_, &staging)));
+    arena_.upload(&terminal_, term);
+    arena_.upload(&base_, base);
 
     // fixed-point scale: the largest possible bin total (all density charge in
     // one bin) stays below 2^61
@@ -652,7 +634,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     geom_.originY = T(originY_);
     geom_.target = T(target);
 
-    poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_, true);
+    poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_, true, &arena_);
     geom_.ld = poisson_->ld();
 
     // device design
@@ -674,16 +656,16 @@ void Engine<T>::build(const sp_hip_design& d) {
             xy[2 * i] = xs[i];
             xy[2 * i + 1] = ys[i];
         }
-        dd_.xy = static_cast<T*>(own(dupload<T>(xy, s_, &staging)));
+        arena_.upload(&dd_.xy, xy);
     }
     std::vector<float> ws(d.node_w, d.node_w + total), hs(d.node_h, d.node_h + total);
     wHost_.assign(ws.begin(), ws.end());
     hHost_.assign(hs.begin(), hs.end());
-    dd_.w = static_cast<T*>(own(dupload<T>(ws, s_, &staging)));
-    dd_.h = static_cast<T*>(own(dupload<T>(hs, s_, &staging)));
+    arena_.upload(&dd_.w, ws);
+    arena_.upload(&dd_.h, hs);
     std::vector<uint8_t> fl(d.node_flags, d.node_flags + total);
-    dd_.flags = static_cast<uint8_t*>(own(dupload<uint8_t>(fl, s_, &staging)));
-    dd_.optIdx = static_cast<int*>(own(dupload<int>(optIdxHost_, s_, &staging)));
+    arena_.upload(&dd_.flags, fl);
+    arena_.upload(&dd_.optIdx, optIdxHost_);
     {
         // density records in node order to start with; the first sort orders them
         std::vector<int> recId(dens.size());
@@ -696,10 +678,10 @@ void Engine<T>::build(const sp_hip_design& d) {
             recWH[2 * t] = d.node_w[i];
             recWH[2 * t + 1] = d.node_h[i];
         }
-        dd_.recIdA = static_cast<int*>(own(dupload<int>(recId, s_, &staging)));
-        dd_.recIdB = static_cast<int*>(own(dupload<int>(recId, s_, &staging)));
-        dd_.recWHA = static_cast<T*>(own(dupload<T>(recWH, s_, &staging)));
-        dd_.recWHB = static_cast<T*>(own(dupload<T>(recWH, s_, &staging)));
+        arena_.upload(&dd_.recIdA, recId);
+        arena_.upload(&dd_.recIdB, recId);
+        arena_.upload(&dd_.recWHA, recWH);
+        arena_.upload(&dd_.recWHB, recWH);
         std::vector<float> recXY(2 * dens.size());
         std::vector<int> recPos(total, -1);
         for (size_t t = 0; t < dens.size(); ++t) {
@@ -707,9 +689,9 @@ void Engine<T>::build(const sp_hip_design& d) {
             recXY[2 * t + 1] = d.node_y[dens[t]];
             recPos[dens[t]] = static_cast<int>(t);
         }
-        dd_.recXYA = static_cast<T*>(own(dupload<T>(recXY, s_, &staging)));
-        dd_.recXYB = static_cast<T*>(own(dupload<T>(recXY, s_, &staging)));
-        dd_.recPos = static_cast<int*>(own(dupload<int>(recPos, s_, &staging)));
+        arena_.upload(&dd_.recXYA, recXY);
+        arena_.upload(&dd_.recXYB, recXY);
+        arena_.upload(&dd_.recPos, recPos);
     }
     {
         TileGrid& tg = dd_.tiles;
@@ -718,17 +700,13 @@ void Engine<T>::build(const sp_hip_design& d) {
         while (side < tg.tilesX) side <<= 1;
         tg.nKeys = side * side;
         if (tg.nKeys > kTileKeys) throw InvalidArg("bin grid wider than 1024 bins");
-        tg.count = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
-        tg.fill = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
-        tg.start = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
-        tg.size = static_cast<int*>(own(dalloc<int>(tg.nKeys)));
-        tg.extraStart = static_cast<int*>(own(dalloc<int>(tg.nKeys + 1)));
+        arena_.zeros(&tg.count, tg.nKeys);
+        arena_.zeros(&tg.fill, tg.nKeys);
+        arena_.zeros(&tg.start, tg.nKeys);
+        arena_.zeros(&tg.size, tg.nKeys);
+        arena_.zeros(&tg.extraStart, tg.nKeys + 1);
         const size_t stageLen = static_cast<size_t>(tg.tilesX) * tg.tilesY * 2 * kTileWin * kTileWin;
-        tg.stage = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(stageLen)));
-        for (int* a : {tg.count, tg.fill, tg.start, tg.size})
-            SPHIP_CHECK(hipMemsetAsync(a, 0, tg.nKeys * sizeof(int), s_));
-        SPHIP_CHECK(hipMemsetAsync(tg.extraStart, 0, (tg.nKeys + 1) * sizeof(int), s_));
-        SPHIP_CHECK(hipMemsetAsync(tg.stage, 0, stageLen * 8, s_));
+        arena_.zeros(&tg.stage, stageLen);
     }
     std::vector<double> parea(n_);
     std::vector<float> pdeg(n_);
@@ -737,8 +715,8 @@ void Engine<T>::build(const sp_hip_design& d) {
         parea[k] = static_cast<double>(d.node_w[i]) * static_cast<double>(d.node_h[i]);
         pdeg[k] = static_cast<float>(d.node2pin_start[i + 1] - d.node2pin_start[i]);
     }
-    dd_.precArea = static_cast<double*>(own(dupload<double>(parea, s_, &staging)));
-    dd_.precDeg = static_cast<float*>(own(dupload<float>(pdeg, s_, &staging)));
+    arena_.upload(&dd_.precArea, parea);
+    arena_.upload(&dd_.precDeg, pdeg);
     // Nets are renumbered by WA size class (degree 2, 3-4, 5-8, ..., then
     // larger), so each class is a contiguous range of nets and of pin slots,
     // and inside a class by their lowest node index, so that neighbouring
@@ -801,10 +779,10 @@ void Engine<T>::build(const sp_hip_design& d) {
         dd_.waWaveBase[kWaClasses + 1] =
             dd_.waWaveBase[kWaClasses] + (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]);
     }
-    dd_.netStart = static_cast<int*>(own(dupload<int>(netStart, s_, &staging)));
-    dd_.slotNode = static_cast<int*>(own(dupload<int>(slotNode, s_, &staging)));
-    dd_.slotOffX = static_cast<T*>(own(dupload<T>(slotOffX, s_, &staging)));
-    dd_.slotOffY = static_cast<T*>(own(dupload<T>(slotOffY, s_, &staging)));
+    arena_.upload(&dd_.netStart, netStart);
+    arena_.upload(&dd_.slotNode, slotNode);
+    arena_.upload(&dd_.slotOffX, slotOffX);
+    arena_.upload(&dd_.slotOffY, slotOffY);
     // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
     // of pins that no net pass writes (no net, degree < 2 or above
     // ignore_net_degree) stay zero.
@@ -814,15 +792,12 @@ void Engine<T>::build(const sp_hip_design& d) {
         const int p = d.flat_node2pin[q];
         if (pinSlot[p] >= 0) slotNp[pinSlot[p]] = q;
     }
-    dd_.nodeStart = static_cast<int*>(own(dupload<int>(nodeStart, s_, &staging)));
-    dd_.slotNp = static_cast<int*>(own(dupload<int>(slotNp, s_, &staging)));
-    dd_.pinGrad = static_cast<T*>(own(dalloc<T>(2 * static_cast<size_t>(std::max(1, d.num_pins)))));
-    SPHIP_CHECK(hipMemsetAsync(dd_.pinGrad, 0, 2 * static_cast<size_t>(std::max(1, d.num_pins)) * sizeof(T), s_));
+    arena_.upload(&dd_.nodeStart, nodeStart);
+    arena_.upload(&dd_.slotNp, slotNp);
+    arena_.zeros(&dd_.pinGrad, 2 * static_cast<size_t>(std::max(1, d.num_pins)));
 
-    nodeQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
-    fillQ_ = static_cast<unsigned long long*>(own(dalloc<unsigned long long>(nb_)));
-    SPHIP_CHECK(hipMemsetAsync(nodeQ_, 0, nb_ * 8, s_));
-    SPHIP_CHECK(hipMemsetAsync(fillQ_, 0, nb_ * 8, s_));
+    arena_.zeros(&nodeQ_, nb_);
+    arena_.zeros(&fillQ_, nb_);
 
     // reduction workspace: 4 regions of maxBlocks x 8 doubles (one block
     // partial per region, see ReduceWs)
@@ -832,27 +807,25 @@ void Engine<T>::build(const sp_hip_design& d) {
          static_cast<long>(dd_.tiles.tilesX) * dd_.tiles.tilesY + (dd_.nDens + kTileChunk - 1) / kTileChunk,
          1L});
     ws_.maxBlocks = static_cast<int>(maxBlocks) + 1;
-    ws_.partials = static_cast<double*>(own(dalloc<double>(4L * ws_.maxBlocks * 8)));
-    ws_.scalars = static_cast<double*>(own(dalloc<double>(kNumScalars)));
-    SPHIP_CHECK(hipMemsetAsync(ws_.partials, 0, 4L * ws_.maxBlocks * 8 * sizeof(double), s_));
-    SPHIP_CHECK(hipMemsetAsync(ws_.scalars, 0, kNumScalars * sizeof(double), s_));
-    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hScalars_), kNumScalars * sizeof(double),
-                              hipHostMallocDefault));
+    arena_.zeros(&ws_.partials, 4L * ws_.maxBlocks * 8);
+    arena_.zeros(&ws_.scalars, kNumScalars);
+    static_assert(kNumScalars * sizeof(double) + kAsyncDepth * sizeof(int) <= pools::kPinnedBlock,
+                  "the pinned block holds the scalars and the done flags");
+    pinned_ = takePinned();
+    std::memset(pinned_, 0, pools::kPinnedBlock);
+    hScalars_ = static_cast<double*>(pinned_);
+    hFlag_ = reinterpret_cast<int*>(static_cast<char*>(pinned_) + kNumScalars * sizeof(double));
 
     // optimizer state
     const size_t n2 = 2 * static_cast<size_t>(std::max(n_, 1));
-    dCtl_ = static_cast<Ctl*>(own(dalloc<Ctl>(1)));
-    dBufs_ = static_cast<T**>(own(dalloc<T*>(kNumBufs)));
+    arena_.zeros(&dCtl_, 1);
+    arena_.zeros(&dBufs_, kNumBufs);
     if (cfg_.loop_mode != 0)
-        for (int i = 0; i < kMaxSnaps; ++i) snapPool_[i] = static_cast<T*>(own(dalloc<T>(n2)));
-    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hFlag_), kAsyncDepth * sizeof(int),
-                              hipHostMallocDefault));
+        for (int i = 0; i < kMaxSnaps; ++i) arena_.zeros(&snapPool_[i], n2);
     for (hipEvent_t& ev : loopEv_) SPHIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    for (T** b : {&u_, &v_, &uPrev_, &vPrev_, &uNew_, &vNew_, &g_, &gPrev_, &gA_, &gB_, &probe_}) {
-        *b = static_cast<T*>(own(dalloc<T>(n2)));
-        SPHIP_CHECK(hipMemsetAsync(*b, 0, n2 * sizeof(T), s_));
-    }
-    SPHIP_CHECK(hipStreamSynchronize(s_));
+    for (T** b : {&u_, &v_, &uPrev_, &vPrev_, &uNew_, &vNew_, &g_, &gPrev_, &gA_, &gB_, &probe_})
+        arena_.zeros(b, n2);
+    arena_.place(s_);
 
     // algorithmic bytes / flops per launch (the roofline accounting)
     const double P = d.num_pins, E = d.num_nets, ts = sizeof(T), nd = dd_.nDens;

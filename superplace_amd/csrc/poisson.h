@@ -17,6 +17,7 @@
 #pragma once
 
 #include "common.h"
+#include "devmem.h"
 
 namespace sphip {
 
@@ -51,8 +52,11 @@ class PoissonPlan {
 public:
     // interleaved: the field is written as (xi_x, xi_y) pairs into fieldXY()
     // (element 2 (x * ld + y) + axis) instead of fieldX() / fieldY().
+    // arena: the plan's arrays are reserved there (and the arena owns them);
+    // they exist once the caller has placed the arena.  Without, the plan
+    // allocates them and is ready on return.
     PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
-                bool interleaved = false);
+                bool interleaved = false, DevArena* arena = nullptr);
     ~PoissonPlan();
     PoissonPlan(const PoissonPlan&) = delete;
     PoissonPlan& operator=(const PoissonPlan&) = delete;
@@ -101,6 +105,7 @@ private:
     int nx_, ny_, mx_, my_;
     hipStream_t stream_;
     bool interleaved_;
+    bool ownsMem_ = true;
     T *rho_ = nullptr, *phi_ = nullptr, *ex_ = nullptr, *ey_ = nullptr, *exy_ = nullptr;
     T *t0_ = nullptr, *p_ = nullptr, *t1_ = nullptr, *t2_ = nullptr;
     T *cyT_ = nullptr, *cx_ = nullptr, *ayT_ = nullptr, *bsv_ = nullptr, *sxu_ = nullptr,

@@ -893,13 +893,26 @@ void launchGemm(int batch, const T* const* A, const T* const* B, T* const* C, in
 
 template <typename T>
 PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
-                            bool interleaved)
+                            bool interleaved, DevArena* arena)
     : nx_(nx), ny_(ny), mx_(roundUp(nx, 64)), my_(roundUp(ny, 64)), stream_(stream),
-      interleaved_(interleaved) {
+      interleaved_(interleaved), ownsMem_(arena == nullptr) {
     if (nx < 2 || ny < 2) throw std::invalid_argument("Poisson grid must be at least 2x2");
     if (!(stepX > 0) || !(stepY > 0)) throw std::invalid_argument("bin step must be positive");
     const double pi = 3.14159265358979323846;
     const size_t gx = static_cast<size_t>(mx_) * my_;
+    // an array with host contents, and a zero-filled one
+    auto put = [&](T** dst, const std::vector<double>& h) {
+        if (arena) arena->upload(dst, h);
+        else *dst = upload<T>(h, stream);
+    };
+    auto zeroed = [&](T** dst, size_t count) {
+        if (arena) {
+            arena->zeros(dst, count);
+        } else {
+            *dst = devAlloc<T>(count);
+            SPHIP_CHECK(hipMemsetAsync(*dst, 0, count * sizeof(T), stream));
+        }
+    };
     fft_ = fftShape(nx) && fftShape(ny);
     if (fft_) {
         // twiddles exp(-2 pi i t/L) and half-angle factors exp(-i pi k/2L)
@@ -911,10 +924,10 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
                 t[2 * L + 2 * k] = std::cos(pi * k / (2.0 * L));
                 t[2 * L + 2 * k + 1] = -std::sin(pi * k / (2.0 * L));
             }
-            return upload<T>(t, stream);
+            return t;
         };
-        tabX_ = tables(nx);
-        tabY_ = tables(ny);
+        put(&tabX_, tables(nx));
+        put(&tabY_, tables(ny));
         setFftLds();
     }
     std::vector<double> u(mx_, 0.0), v(my_, 0.0);
@@ -950,12 +963,12 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
                 sxu[static_cast<size_t>(x) * mx_ + j] =
                     j == 0 ? 0.0 : 2.0 * u[j] * std::sin(pi * j * (x + 0.5) / nx);
             }
-        cyT_ = upload<T>(cyT, stream);
-        ayT_ = upload<T>(ayT, stream);
-        bsv_ = upload<T>(bsv, stream);
-        cx_ = upload<T>(cx, stream);
-        sxu_ = upload<T>(sxu, stream);
-        ax_ = upload<T>(ax, stream);
+        put(&cyT_, cyT);
+        put(&ayT_, ayT);
+        put(&bsv_, bsv);
+        put(&cx_, cx);
+        put(&sxu_, sxu);
+        put(&ax_, ax);
     }
     // the reference computes u, v in float (cpu_poisson.cpp:88-94); the FP32
     // path keeps them as float, the FP64 path in double
@@ -968,27 +981,22 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
             vf[k] = static_cast<float>(3.14159265358979323846f) * static_cast<float>(k) /
                     (static_cast<float>(ny) * static_cast<float>(stepY));
     }
-    u_ = upload<T>(uf, stream);
-    v_ = upload<T>(vf, stream);
+    put(&u_, uf);
+    put(&v_, vf);
     invNorm_ = static_cast<T>(1.0 / (4.0 * nx * ny));
-    for (T** b : {&rho_, &phi_, &t0_, &p_, &t1_, &t2_}) {
-        *b = devAlloc<T>(gx);
-        SPHIP_CHECK(hipMemsetAsync(*b, 0, gx * sizeof(T), stream));
-    }
+    for (T** b : {&rho_, &phi_, &t0_, &p_, &t1_, &t2_}) zeroed(b, gx);
     if (interleaved) {
-        exy_ = devAlloc<T>(2 * gx);
-        SPHIP_CHECK(hipMemsetAsync(exy_, 0, 2 * gx * sizeof(T), stream));
+        zeroed(&exy_, 2 * gx);
     } else {
-        for (T** b : {&ex_, &ey_}) {
-            *b = devAlloc<T>(gx);
-            SPHIP_CHECK(hipMemsetAsync(*b, 0, gx * sizeof(T), stream));
-        }
+        zeroed(&ex_, gx);
+        zeroed(&ey_, gx);
     }
-    SPHIP_CHECK(hipStreamSynchronize(stream));
+    if (!arena) SPHIP_CHECK(hipStreamSynchronize(stream));
 }
 
 template <typename T>
 PoissonPlan<T>::~PoissonPlan() {
+    if (!ownsMem_) return;
     for (T* b : {rho_, phi_, ex_, ey_, exy_, t0_, p_, t1_, t2_, cyT_, cx_, ayT_, bsv_, sxu_, ax_, u_,
                  v_, tabX_, tabY_})
         if (b) (void)hipFree(b);
