@@ -199,9 +199,7 @@ def serve_bench(count, rate):
     import numpy as np
     from superplace_amd import design as D
     from superplace_amd import hip, serve
-    jobs = job_designs(count, 10000, 640000, 100, 256, 1024)
-    gaps = -np.log(1.0 - D.Rng(1002).uniform(count)) / rate
-    arrivals = np.cumsum(gaps) - gaps[0]
+    jobs, arrivals = D.serving_stream(count, rate)
 
     def run(prec, slots):
         srv = serve.Server(hbm_budget=200 << 30, max_active=slots)

@@ -34,6 +34,9 @@ public:
     virtual int inFlight() const = 0;
     virtual void finishAsync(sp_hip_gp_result* res) = 0;
     virtual size_t deviceBytes() const = 0;
+    // Whether the instance can join batched launches (FP32, power-of-two
+    // bin grid).
+    virtual bool batchable() const = 0;
 };
 
 constexpr int kAsyncDepth = 2;   // loop launches in flight per instance
@@ -46,6 +49,38 @@ constexpr int kAsyncDepth = 2;   // loop launches in flight per instance
 // equal to its own run's.
 void runBatch(const std::vector<EngineBase*>& engines, const std::vector<char>& looping,
               sp_hip_gp_result* res, int slots);
+
+// Batched launches over a set of FP32 instances that may change between
+// launches (continuous batching).  Every launch runs `slots` slots, each
+// kernel of a slot launched once for all the members (the FFT passes once per
+// grid size); a member whose stage has ended is skipped by every kernel until
+// it leaves.  Each member's result equals its own run's: its kernels touch
+// only its own buffers and are deterministic, and every member starts at a
+// launch boundary as its own loop does.
+class BatchSet {
+public:
+    explicit BatchSet(int slots);
+    ~BatchSet();
+    BatchSet(const BatchSet&) = delete;
+    BatchSet& operator=(const BatchSet&) = delete;
+
+    // The members from now on (each begun, with its stage loop still to
+    // run).  Waits for the launches in flight, then rebuilds the block maps
+    // and the captured graph.
+    void assign(const std::vector<EngineBase*>& members);
+    // Enqueues one launch for all members (at most kAsyncDepth in flight).
+    void launch();
+    // The oldest launch in flight: -1 while it runs (wait = false), else 0
+    // with state[j] for member j: 0 still running, 1 stage done, 2 not done
+    // after more launches than its iteration and backtrack limits allow.
+    int poll(bool wait, std::vector<int>* state);
+    int inFlight() const;
+    int size() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 // A new instance on its own stream (precision from cfg.precision).
 std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg);

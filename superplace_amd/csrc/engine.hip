@@ -326,6 +326,7 @@ public:
         return (static_cast<long>(maxIter) * maxBt + 16) / std::max(1, slots) + 2;
     }
     size_t deviceBytes() const override { return devBytes_; }
+    bool batchable() const override { return sizeof(T) == 4 && poisson_->usesFft(); }
 
 private:
     void finishDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows);
@@ -1564,155 +1565,234 @@ private:
 
 }  // namespace
 
-// Many FP32 stages advanced together by batched launches: one hipGraph of
-// `slots` batched slots (every kernel of a slot launched once for all the
-// instances, the FFT passes once per grid size) is launched until every
-// instance's stage has ended; each instance's result equals its own run's.
-void runBatch(const std::vector<EngineBase*>& engines, const std::vector<char>& looping,
-              sp_hip_gp_result* res, int slots) {
-    const int n = static_cast<int>(engines.size());
-    std::vector<Engine<float>*> live;
-    std::vector<int> liveAt;
-    for (int i = 0; i < n; ++i) {
-        auto* e = dynamic_cast<Engine<float>*>(engines[i]);
-        if (!e) throw InvalidArg("batched runs take FP32 instances");
-        if (!looping[i]) continue;   // the stage ended before its loop
-        live.push_back(e);
-        liveAt.push_back(i);
+struct BatchSet::Impl {
+    int slots = 1;
+    int device = 0;
+    hipStream_t s = nullptr;   // from the stream pool
+    std::vector<Engine<float>*> members;
+    std::vector<long> launches, bound;   // per member: launches since joining, limit
+    std::unique_ptr<DevArena> arena;     // argument tables and block maps of the graph
+    int* doneDev = nullptr;              // per member: ctl->done after a launch
+    int* doneHost = nullptr;             // pinned, kAsyncDepth rows of doneCap
+    int doneCap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[kAsyncDepth] = {};
+    int head = 0, inflight = 0;
+
+    void dropGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        arena.reset();
     }
-    const int m = static_cast<int>(live.size());
-    if (m == 0) return;
-    slots = std::max(1, slots);
-    // instance shares and block maps
+    // waits for every launch in flight
+    void drain() {
+        if (inflight == 0) return;
+        SPHIP_CHECK(hipStreamSynchronize(s));
+        inflight = 0;
+        head = 0;
+    }
+    void reserveDone(int m) {
+        if (m <= doneCap) return;
+        const int cap = std::max(m, 2 * doneCap);
+        if (doneDev) (void)hipFree(doneDev);
+        if (doneHost) (void)hipHostFree(doneHost);
+        doneDev = nullptr;
+        doneHost = nullptr;
+        doneCap = 0;
+        SPHIP_CHECK(hipMalloc(reinterpret_cast<void**>(&doneDev), cap * sizeof(int)));
+        SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&doneHost),
+                                  kAsyncDepth * cap * sizeof(int), hipHostMallocDefault));
+        doneCap = cap;
+    }
+    void capture();
+};
+
+BatchSet::BatchSet(int slots) : impl_(std::make_unique<Impl>()) {
+    requireDevice();
+    impl_->slots = std::max(1, slots);
+    SPHIP_CHECK(hipGetDevice(&impl_->device));
+    impl_->s = takeStream(impl_->device);
+    try {
+        for (hipEvent_t& e : impl_->ev) SPHIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    } catch (...) {
+        for (hipEvent_t e : impl_->ev)
+            if (e) (void)hipEventDestroy(e);
+        giveStream(impl_->device, impl_->s);
+        throw;
+    }
+}
+
+BatchSet::~BatchSet() {
+    Impl& b = *impl_;
+    if (b.s) (void)hipStreamSynchronize(b.s);
+    b.dropGraph();
+    for (hipEvent_t e : b.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b.doneDev) (void)hipFree(b.doneDev);
+    if (b.doneHost) (void)hipHostFree(b.doneHost);
+    giveStream(b.device, b.s);
+}
+
+int BatchSet::inFlight() const { return impl_->inflight; }
+int BatchSet::size() const { return static_cast<int>(impl_->members.size()); }
+
+void BatchSet::assign(const std::vector<EngineBase*>& members) {
+    Impl& b = *impl_;
+    b.drain();
+    b.dropGraph();
+    b.members.clear();
+    b.launches.clear();
+    b.bound.clear();
+    for (EngineBase* e : members) {
+        auto* f = dynamic_cast<Engine<float>*>(e);
+        if (!f) throw InvalidArg("batched runs take FP32 instances");
+        b.members.push_back(f);
+        b.launches.push_back(0);
+        b.bound.push_back(f->maxLoopLaunches(b.slots));
+    }
+    if (b.members.empty()) return;
+    b.reserveDone(static_cast<int>(b.members.size()));
+    b.capture();
+}
+
+// Block maps and argument tables for the members, then the graph of
+// `slots` batched slots.
+void BatchSet::Impl::capture() {
+    const int m = static_cast<int>(members.size());
     std::vector<SlotArgs<float>> args(m);
     std::vector<PoissonBatchArgs<float>> pargs(m);
     std::vector<std::vector<int4>> maps(BK_KINDS);
-    std::vector<int> dims;
+    std::vector<int> dims, lds;
     std::vector<std::array<std::vector<int4>, 3>> fftMaps;   // per grid size, per pass
-    int* doneDev = nullptr;
-    SPHIP_CHECK(hipMalloc(reinterpret_cast<void**>(&doneDev), m * sizeof(int)));
-    int* doneHost = nullptr;
-    SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&doneHost), 2 * m * sizeof(int),
-                              hipHostMallocDefault));
-    std::vector<void*> owned{doneDev};
-    hipStream_t s = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    auto cleanup = [&] {
-        if (s) (void)hipStreamSynchronize(s);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (void* p : owned) (void)hipFree(p);
-        if (doneHost) (void)hipHostFree(doneHost);
-        if (s) (void)hipStreamDestroy(s);
-    };
-    try {
-        long maxLaunches = 0;
-        for (int j = 0; j < m; ++j) {
-            int counts[BK_KINDS];
-            live[j]->batchShare(&args[j], &pargs[j], counts, doneDev);
-            for (int k = 0; k < BK_KINDS; ++k)
-                for (int b = 0; b < counts[k]; ++b) maps[k].push_back(int4{j, b, counts[k], 0});
-            const int dim = live[j]->gridDim();
-            int gi = static_cast<int>(std::find(dims.begin(), dims.end(), dim) - dims.begin());
-            if (gi == static_cast<int>(dims.size())) {
-                dims.push_back(dim);
-                fftMaps.emplace_back();
-            }
-            for (int pass = 0; pass < 3; ++pass) {
-                const int c = poissonBatchBlocks(pass, dim, dim);
-                for (int b = 0; b < c; ++b) fftMaps[gi][pass].push_back(int4{j, b, c, 0});
-            }
-            maxLaunches = std::max(maxLaunches, live[j]->maxLoopLaunches(slots));
-        }
-        SPHIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        auto upload = [&](const void* h, size_t bytes) {
-            void* d = nullptr;
-            SPHIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));
-            owned.push_back(d);
-            if (bytes) SPHIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
-            return d;
-        };
-        auto* dArgs = static_cast<SlotArgs<float>*>(upload(args.data(), m * sizeof(SlotArgs<float>)));
-        auto* dPargs = static_cast<PoissonBatchArgs<float>*>(
-            upload(pargs.data(), m * sizeof(PoissonBatchArgs<float>)));
-        std::vector<int4*> dMaps(BK_KINDS);
+    for (int j = 0; j < m; ++j) {
+        int counts[BK_KINDS];
+        members[j]->batchShare(&args[j], &pargs[j], counts, doneDev);
         for (int k = 0; k < BK_KINDS; ++k)
-            dMaps[k] = static_cast<int4*>(upload(maps[k].data(), maps[k].size() * sizeof(int4)));
-        std::vector<std::array<int4*, 3>> dFft(dims.size());
-        std::vector<int> fftLd(dims.size());
-        for (size_t g = 0; g < dims.size(); ++g) {
-            for (int pass = 0; pass < 3; ++pass)
-                dFft[g][pass] = static_cast<int4*>(
-                    upload(fftMaps[g][pass].data(), fftMaps[g][pass].size() * sizeof(int4)));
-            // every instance of a grid size has the same padded row stride
-            for (int j = 0; j < m; ++j)
-                if (live[j]->gridDim() == dims[g]) {
-                    fftLd[g] = live[j]->gridLd();
-                    break;
-                }
+            for (int q = 0; q < counts[k]; ++q) maps[k].push_back(int4{j, q, counts[k], 0});
+        const int dim = members[j]->gridDim();
+        const int gi = static_cast<int>(std::find(dims.begin(), dims.end(), dim) - dims.begin());
+        if (gi == static_cast<int>(dims.size())) {
+            dims.push_back(dim);
+            lds.push_back(members[j]->gridLd());   // one padded stride per grid size
+            fftMaps.emplace_back();
         }
-        SPHIP_CHECK(hipMemsetAsync(doneDev, 0, m * sizeof(int), s));
-        SPHIP_CHECK(hipStreamSynchronize(s));
-        auto count = [&](int k) { return static_cast<int>(maps[k].size()); };
-        // capture `slots` batched slots
-        SPHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            for (int slot = 0; slot < slots; ++slot) {
-                const int sorted = slot % kSortPeriod == 0 ? 1 : 0;
-                if (sorted) {
-                    launchSlotBatch<float>(BK_SORT_COUNT, dArgs, dMaps[BK_SORT_COUNT],
-                                           count(BK_SORT_COUNT), 1, s);
-                    launchSlotBatch<float>(BK_SORT_PLACE, dArgs, dMaps[BK_SORT_PLACE],
-                                           count(BK_SORT_PLACE), 1, s);
-                }
-                for (int k : {BK_SCATTER, BK_FINALIZE})
-                    launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
-                for (size_t g = 0; g < dims.size(); ++g)
-                    for (int pass = 0; pass < 3; ++pass)
-                        launchPoissonBatch<float>(pass, dPargs, dFft[g][pass],
-                                                  static_cast<int>(fftMaps[g][pass].size()),
-                                                  dims[g], dims[g], fftLd[g], s);
-                for (int k : {BK_WA, BK_NODE, BK_REDUCE, BK_UPDATE})
-                    launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
-            }
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
+        for (int pass = 0; pass < 3; ++pass) {
+            const int c = poissonBatchBlocks(pass, dim, dim);
+            for (int q = 0; q < c; ++q) fftMaps[gi][pass].push_back(int4{j, q, c, 0});
         }
-        SPHIP_CHECK(hipStreamEndCapture(s, &graph));
-        SPHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        for (hipEvent_t& e : ev) SPHIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        // two launches in flight; the host reads the done flags of the older
-        long launched = 0;
-        auto launch = [&](int k) {
-            if (++launched > maxLaunches) throw std::runtime_error("batched loop did not terminate");
-            SPHIP_CHECK(hipGraphLaunch(exec, s));
-            SPHIP_CHECK(hipMemcpyAsync(doneHost + k * m, doneDev, m * sizeof(int),
-                                       hipMemcpyDeviceToHost, s));
-            SPHIP_CHECK(hipEventRecord(ev[k], s));
-        };
-        launch(0);
-        launch(1);
-        for (int k = 0;; k ^= 1) {
-            SPHIP_CHECK(hipEventSynchronize(ev[k]));
-            bool all = true;
-            for (int j = 0; j < m && all; ++j) all = doneHost[k * m + j] != 0;
-            if (all) break;
-            launch(k);
+    }
+    arena = std::make_unique<DevArena>();
+    SlotArgs<float>* dArgs = nullptr;
+    PoissonBatchArgs<float>* dPargs = nullptr;
+    arena->upload(&dArgs, args);
+    arena->upload(&dPargs, pargs);
+    std::vector<int4*> dMaps(BK_KINDS, nullptr);
+    for (int k = 0; k < BK_KINDS; ++k) arena->upload(&dMaps[k], maps[k]);
+    std::vector<std::array<int4*, 3>> dFft(dims.size());
+    for (size_t g = 0; g < dims.size(); ++g)
+        for (int pass = 0; pass < 3; ++pass) arena->upload(&dFft[g][pass], fftMaps[g][pass]);
+    arena->place(s);
+    SPHIP_CHECK(hipMemsetAsync(doneDev, 0, m * sizeof(int), s));
+    SPHIP_CHECK(hipStreamSynchronize(s));
+    auto count = [&](int k) { return static_cast<int>(maps[k].size()); };
+    SPHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int slot = 0; slot < slots; ++slot) {
+            const int sorted = slot % kSortPeriod == 0 ? 1 : 0;
+            if (sorted)
+                for (int k : {BK_SORT_COUNT, BK_SORT_PLACE})
+                    launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), 1, s);
+            for (int k : {BK_SCATTER, BK_FINALIZE})
+                launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
+            for (size_t g = 0; g < dims.size(); ++g)
+                for (int pass = 0; pass < 3; ++pass)
+                    launchPoissonBatch<float>(pass, dPargs, dFft[g][pass],
+                                              static_cast<int>(fftMaps[g][pass].size()), dims[g],
+                                              dims[g], lds[g], s);
+            for (int k : {BK_WA, BK_NODE, BK_REDUCE, BK_UPDATE})
+                launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), sorted, s);
         }
-        SPHIP_CHECK(hipStreamSynchronize(s));
-        for (int j = 0; j < m; ++j) live[j]->finishAsync(&res[liveAt[j]]);
     } catch (...) {
-        cleanup();
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
         throw;
     }
-    cleanup();
+    SPHIP_CHECK(hipStreamEndCapture(s, &graph));
+    SPHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+}
+
+void BatchSet::launch() {
+    Impl& b = *impl_;
+    if (b.members.empty()) throw InvalidArg("batched launch: no members");
+    if (b.inflight >= kAsyncDepth) throw InvalidArg("batched launch: ring full");
+    const int m = static_cast<int>(b.members.size());
+    const int k = (b.head + b.inflight) % kAsyncDepth;
+    SPHIP_CHECK(hipGraphLaunch(b.exec, b.s));
+    SPHIP_CHECK(hipMemcpyAsync(b.doneHost + static_cast<size_t>(k) * b.doneCap, b.doneDev,
+                               m * sizeof(int), hipMemcpyDeviceToHost, b.s));
+    SPHIP_CHECK(hipEventRecord(b.ev[k], b.s));
+    for (long& l : b.launches) ++l;
+    ++b.inflight;
+}
+
+int BatchSet::poll(bool wait, std::vector<int>* state) {
+    Impl& b = *impl_;
+    if (b.inflight == 0) throw InvalidArg("batched poll: nothing in flight");
+    const int k = b.head;
+    if (wait) {
+        SPHIP_CHECK(hipEventSynchronize(b.ev[k]));
+    } else {
+        const hipError_t q = hipEventQuery(b.ev[k]);
+        if (q == hipErrorNotReady) return -1;
+        SPHIP_CHECK(q);
+    }
+    b.head = (b.head + 1) % kAsyncDepth;
+    --b.inflight;
+    const int m = static_cast<int>(b.members.size());
+    // launches the member had when this launch ended
+    const long behind = b.inflight;
+    state->assign(m, 0);
+    for (int j = 0; j < m; ++j) {
+        if (b.doneHost[static_cast<size_t>(k) * b.doneCap + j] != 0) (*state)[j] = 1;
+        else if (b.launches[j] - behind >= b.bound[j]) (*state)[j] = 2;
+    }
+    return 0;
+}
+
+// Many FP32 stages advanced together by batched launches until every
+// instance's stage has ended; each instance's result equals its own run's.
+void runBatch(const std::vector<EngineBase*>& engines, const std::vector<char>& looping,
+              sp_hip_gp_result* res, int slots) {
+    std::vector<EngineBase*> live;
+    std::vector<int> liveAt;
+    for (size_t i = 0; i < engines.size(); ++i) {
+        if (!dynamic_cast<Engine<float>*>(engines[i]))
+            throw InvalidArg("batched runs take FP32 instances");
+        if (!looping[i]) continue;   // the stage ended before its loop
+        live.push_back(engines[i]);
+        liveAt.push_back(static_cast<int>(i));
+    }
+    if (live.empty()) return;
+    BatchSet set(slots);
+    set.assign(live);
+    for (int k = 0; k < kAsyncDepth; ++k) set.launch();
+    std::vector<int> state;
+    for (;;) {
+        set.poll(true, &state);
+        bool all = true;
+        for (int st : state) {
+            if (st == 2) throw std::runtime_error("batched loop did not terminate");
+            all = all && st == 1;
+        }
+        if (all) break;
+        set.launch();
+    }
+    set.assign({});   // waits for the launch still in flight
+    for (size_t j = 0; j < live.size(); ++j) live[j]->finishAsync(&res[liveAt[j]]);
 }
 
 std::unique_ptr<EngineBase> makeEngine(const sp_hip_design& d, const sp_hip_gp_config& cfg) {

@@ -3,16 +3,21 @@
 // and advances all of them together.  No counterpart in the reference (it
 // places one design per process).
 //
-// Each admitted job is one placement instance (EngineBase) on its own HIP
-// stream, driving its device-side stage loop in launches of a captured
-// graph.  The scheduler keeps kAsyncDepth launches of every running job in
-// flight and tops them up as they complete, so the jobs' kernels overlap on
-// the device; a job that arrives while others run is admitted as soon as a
-// slot and enough of the device budget are free.  The host set-ups of the
-// next jobs (design upload, initial evaluations, graph capture) run on worker
-// threads meanwhile, up to kSetupThreads at once.  Each job's result is the same as when it runs alone: an
-// instance's kernels touch only its own buffers and every kernel is
-// deterministic.
+// Each admitted job is one placement instance (EngineBase).  FP32 jobs on
+// power-of-two bin grids are batched across iterations: they advance
+// together in launches of one captured graph whose kernels each serve all of
+// them (BatchSet), two launches in flight.  When a batched job's stage ends,
+// or set-ups have finished in the meantime, the scheduler lets the launch in
+// flight end, retires the finished jobs, and rebuilds the batch with the
+// newcomers, so that a job arriving while others run joins them at the next
+// launch boundary.  Other jobs (FP64, grids that are not powers of two) run
+// on their own streams, each driving its device-side loop in launches of its
+// own graph, kAsyncDepth in flight.  Jobs are admitted as soon as a slot and
+// enough of the device budget are free; the host set-ups of the next jobs
+// (design upload, initial evaluations) run on worker threads meanwhile, up
+// to kSetupThreads at once.  Each job's result is the same as when it runs
+// alone: an instance's kernels touch only its own buffers and every kernel
+// is deterministic.
 #include <algorithm>
 #include <chrono>
 #include <future>
@@ -76,6 +81,7 @@ struct Job {
 };
 
 constexpr int kSetupThreads = 4;   // job set-ups in flight at once
+constexpr int kBatchSlots = 8;     // iterations per batched launch
 
 // What a set-up thread hands back.
 struct Admission {
@@ -140,6 +146,9 @@ public:
                 j.eng = std::move(a.eng);
                 if (a.finished) {
                     complete(j, false);
+                } else if (j.eng->batchable()) {
+                    j.info.state = SP_HIP_JOB_RUNNING;
+                    joining_.push_back(j.id);
                 } else {
                     j.info.state = SP_HIP_JOB_RUNNING;
                     running_.push_back(j.id);
@@ -147,6 +156,7 @@ public:
                 }
             }
         }
+        progress = pumpBatch() || progress;
         // top up the running jobs' launches, retire the finished ones
         for (size_t k = 0; k < running_.size();) {
             Job& j = *jobs_[running_[k]];
@@ -179,7 +189,8 @@ public:
                 continue;
             }
             if (!(j.info.arrival_s <= t) ||
-                static_cast<int>(running_.size() + admitting_.size()) >= maxActive_ ||
+                static_cast<int>(running_.size() + admitting_.size() + batched_.size() +
+                                 joining_.size()) >= maxActive_ ||
                 used_ + j.info.device_bytes > budget_)
                 break;
             queued_.erase(queued_.begin());
@@ -208,7 +219,9 @@ public:
     }
 
     void counts(int32_t* active, int32_t* queued, int32_t* finished) const {
-        if (active) *active = static_cast<int32_t>(running_.size() + admitting_.size());
+        if (active)
+            *active = static_cast<int32_t>(running_.size() + admitting_.size() + batched_.size() +
+                                           joining_.size());
         if (queued) *queued = static_cast<int32_t>(queued_.size());
         if (finished) *finished = finished_;
     }
@@ -219,7 +232,8 @@ public:
         while (finished_ < static_cast<int>(jobs_.size())) {
             if (now() > end) throw InvalidArg("drain: time limit reached");
             if (pump()) continue;
-            if (running_.empty() && admitting_.empty() && !queued_.empty()) {
+            if (running_.empty() && admitting_.empty() && batched_.empty() && joining_.empty() &&
+                !queued_.empty()) {
                 // idle until the next arrival
                 double next = end;
                 for (int id : queued_) next = std::min(next, jobs_[id]->info.arrival_s);
@@ -247,6 +261,75 @@ public:
     }
 
 private:
+    // The batched jobs: reads the oldest launch's done flags once it has
+    // ended; on a change of members (stages ended, set-ups joined) lets the
+    // other launch end, retires the finished jobs and rebuilds the batch;
+    // keeps kAsyncDepth launches in flight.  Returns whether anything changed.
+    bool pumpBatch() {
+        if (batched_.empty() && joining_.empty()) return false;
+        bool progress = false;
+        try {
+            if (!batch_) batch_ = std::make_unique<BatchSet>(kBatchSlots);
+            std::vector<int> ended(batched_.size(), 0);   // 1 done, 2 over its limits
+            auto merge = [&] {
+                for (size_t q = 0; q < ended.size(); ++q)
+                    if (state_[q] != 0 && ended[q] == 0) ended[q] = state_[q];
+            };
+            if (batch_->inFlight() > 0) {
+                if (batch_->poll(false, &state_) < 0) return false;
+                merge();
+                progress = true;
+            }
+            const bool change =
+                !joining_.empty() || std::any_of(ended.begin(), ended.end(), [](int e) { return e; });
+            if (change) {
+                while (batch_->inFlight() > 0) {
+                    batch_->poll(true, &state_);
+                    merge();
+                }
+                std::vector<int> keep;
+                for (size_t q = 0; q < batched_.size(); ++q) {
+                    Job& j = *jobs_[batched_[q]];
+                    if (ended[q] == 1) {
+                        try {
+                            complete(j, true);
+                        } catch (...) {
+                            failCurrent(j);
+                        }
+                    } else if (ended[q] == 2) {
+                        fail(j, SP_HIP_EINTERNAL, "batched loop did not terminate");
+                    } else {
+                        keep.push_back(batched_[q]);
+                    }
+                }
+                keep.insert(keep.end(), joining_.begin(), joining_.end());
+                joining_.clear();
+                batched_ = keep;
+                std::vector<EngineBase*> members;
+                for (int id : batched_) members.push_back(jobs_[id]->eng.get());
+                batch_->assign(members);
+                progress = true;
+            }
+            while (batch_->size() > 0 && batch_->inFlight() < kAsyncDepth) {
+                batch_->launch();
+                progress = true;
+            }
+        } catch (...) {
+            // a device or set-up error leaves the batch unusable: its jobs fail
+            std::vector<int> all(batched_);
+            all.insert(all.end(), joining_.begin(), joining_.end());
+            batched_.clear();
+            joining_.clear();
+            batch_.reset();
+            for (int id : all) {
+                Job& j = *jobs_[id];
+                if (j.info.state == SP_HIP_JOB_RUNNING) fail(j, SP_HIP_EDEVICE, "batched launch failed");
+            }
+            progress = true;
+        }
+        return progress;
+    }
+
     void topUp(Job& j) {
         while (j.eng->inFlight() < kAsyncDepth) j.eng->launchAsync();
     }
@@ -311,6 +394,9 @@ private:
     std::chrono::steady_clock::time_point t0_;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::vector<int> queued_, running_;
+    std::vector<int> batched_, joining_;   // members of batch_ (in its order), waiting to join
+    std::unique_ptr<BatchSet> batch_;
+    std::vector<int> state_;
     std::vector<std::pair<int, std::future<Admission>>> admitting_;   // (job, set-up)
     int64_t used_ = 0;
     int finished_ = 0;

@@ -436,19 +436,45 @@ def auto_bin_dim(d: Design, lo: int = 4, hi: int = 1024) -> int:
     return min(max(dim, lo), hi)
 
 
-def job_designs(count: int, lo: int, hi: int, seed0: int, dim_lo: int, dim_hi: int,
-                nets_per_cell: float = 1.05):
-    """The many-instance workloads: `count` seeded synthetic designs with cell
-    counts log-uniform in [lo, hi] (drawn from the splitmix64 stream of seed
-    1002 + seed0), the config-1 degree distribution, `nets_per_cell` nets per
-    cell, 57% utilisation at adaptec1's cell density; each with its bin
-    dimension by the automatic rule clamped to [dim_lo, dim_hi].  Returns
-    [(design, bins per side)]."""
+def job_cells(count: int, lo: int, hi: int, seed0: int, extremes: bool = False):
+    """Cell counts log-uniform in [lo, hi], drawn from the splitmix64 stream of
+    seed 1002 + seed0; with `extremes`, the smallest draw becomes lo and the
+    largest hi."""
     u = Rng(1002 + seed0).uniform(count)
+    cells = [int(round(lo * (hi / lo) ** float(x))) for x in u]
+    if extremes and count >= 2:
+        cells[int(np.argmin(u))] = lo
+        cells[int(np.argmax(u))] = hi
+    return cells
+
+
+def job_designs(count: int, lo: int, hi: int, seed0: int, dim_lo: int, dim_hi: int,
+                nets_per_cell: float = 1.05, extremes: bool = False):
+    """The many-instance workloads: `count` seeded synthetic designs with cell
+    counts from job_cells, the config-1 degree distribution, `nets_per_cell`
+    nets per cell, 57% utilisation at adaptec1's cell density; each with its
+    bin dimension by the automatic rule clamped to [dim_lo, dim_hi].  Returns
+    [(design, bins per side)]."""
     out = []
-    for k in range(count):
-        cells = int(round(lo * (hi / lo) ** float(u[k])))
+    for k, cells in enumerate(job_cells(count, lo, hi, seed0, extremes)):
         d = synth_design(cells, max(8, cells // 400), int(cells * nets_per_cell),
                          25.25 * np.sqrt(cells), 0.57, seed=seed0 + k + 1, name=f"job{k}")
         out.append((d, auto_bin_dim(d, dim_lo, dim_hi)))
     return out
+
+
+# Nets per cell of the serving stream: with the config-1 degree distribution
+# (3.28 pins per net on average) a 1M-cell job has about 2.1M pins.
+STREAM_NETS_PER_CELL = 0.64
+
+
+def serving_stream(count: int, rate: float, seed0: int = 100, lo: int = 10000,
+                   hi: int = 1000000, extremes: bool = False):
+    """The serving workload: `count` jobs (job_designs over [lo, hi] cells,
+    STREAM_NETS_PER_CELL nets per cell, bins 256-1024 by the automatic rule)
+    arriving as a Poisson process of `rate` jobs per second (exponential gaps
+    from the splitmix64 stream of seed 1002), the first at time 0.  Returns
+    ([(design, bins per side)], arrival times)."""
+    jobs = job_designs(count, lo, hi, seed0, 256, 1024, STREAM_NETS_PER_CELL, extremes)
+    gaps = -np.log(1.0 - Rng(1002).uniform(count)) / rate
+    return jobs, np.cumsum(gaps) - gaps[0]

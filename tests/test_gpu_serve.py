@@ -95,3 +95,38 @@ def test_admission_budget_arrivals_and_failures():
         srv.submit(jobs[0][0], hip.GpConfig(loop_mode=0))
     srv.release(ids[0])
     srv.close()
+
+
+def test_serving_stream_equals_alone():
+    """The serving workload: 12 jobs of 10k-1M cells (log-uniform, the smallest
+    10k and the largest 1M cells with 2.1M pins), bins 256-1024 by the
+    automatic rule, Poisson arrivals at 20 jobs/s, served together by the
+    scheduler (FP32 jobs batched across iterations).  Every job converges and
+    its result and positions equal the job run alone; p50/p99 latency (finish
+    - arrival) is reported."""
+    jobs, arrivals = D.serving_stream(12, 20.0, extremes=True)
+    cells = [d.num_movable for d, _ in jobs]
+    assert min(cells) == 10000 and max(cells) == 1000000
+    assert max(d.num_pins for d, _ in jobs) <= 2_200_000
+    assert {dim for _, dim in jobs} >= {256, 1024}
+    cfgs = [hip.GpConfig(gp_bin_dim=dim) for _, dim in jobs]
+    srv = serve.Server(hbm_budget=200 << 30, max_active=16)
+    for (d, _), c, t in zip(jobs, cfgs, arrivals):
+        srv.submit(d, c, arrival_s=float(t))
+    srv.drain(timeout_s=600)
+    infos = [srv.job(i) for i in range(len(jobs))]
+    srv.close()
+    lat = np.array([i["finish_s"] - i["arrival_s"] for i in infos])
+    print(f"latency p50 {np.percentile(lat, 50):.3f} s p99 {np.percentile(lat, 99):.3f} s",
+          flush=True)
+    overlap = sum(1 for a in infos for b in infos
+                  if a is not b and a["admit_s"] < b["admit_s"] < a["finish_s"])
+    assert overlap > 0
+    for (d, _), c, info in zip(jobs, cfgs, infos):
+        assert info["state"] == serve.DONE and info["status"] == 0, d.name
+        res, x, y, _ = alone(d, c)
+        print(d.name, d.num_movable, c.gp_bin_dim, res["iterations"], res["overflow"], flush=True)
+        assert res["converged"] and res["overflow"] <= 0.10
+        for k in RESULT_KEYS:
+            assert info["result"][k] == res[k], (d.name, k, info["result"][k], res[k])
+        assert np.array_equal(bits(info["x"]), bits(x)) and np.array_equal(bits(info["y"]), bits(y))
