@@ -463,6 +463,12 @@ int sp_filler_plan(const sp_hip_design* d, float target_density, sp_filler_info*
 int sp_filler_scatter(const sp_hip_design* d, int count, float width, float height,
                       uint64_t seed, float* x, float* y);
 
+/* `count` successive draws of the reference's Rng::uniform(lo, hi)
+ * (src/util/rng.h:15-18: std::mt19937 seeded with `seed`, one
+ * uniform_real_distribution<float>(lo, hi) draw each), the generator of the
+ * reference's test fixtures (test/test_eplace.cpp:45-121).  Host code. */
+int sp_rng_uniform(uint64_t seed, int64_t count, float lo, float hi, float* out);
+
 /* Batched C = A * B^T of row-major matrices (A m x k, B n x k, C m x n; dims
  * multiples of 64) on the matrix cores, the primitive of the Poisson
  * transforms.  Device pointers, float (or double with SP_HIP_FLAG_FP64);

@@ -143,6 +143,18 @@ int sp_filler_plan(const sp_hip_design* d, float target_density, sp_filler_info*
     });
 }
 
+int sp_rng_uniform(uint64_t seed, int64_t count, float lo, float hi, float* out) {
+    return guarded([&] {
+        if (count < 0) throw InvalidArg("negative count");
+        if (count > 0 && !out) throw InvalidArg("null output");
+        if (!(lo < hi) || !std::isfinite(lo) || !std::isfinite(hi))
+            throw InvalidArg("uniform range must be finite with lo < hi");
+        std::mt19937 gen(static_cast<std::mt19937::result_type>(seed));
+        std::uniform_real_distribution<float> dist(lo, hi);
+        for (int64_t i = 0; i < count; ++i) out[i] = dist(gen);
+    });
+}
+
 int sp_filler_scatter(const sp_hip_design* d, int count, float width, float height,
                       uint64_t seed, float* x, float* y) {
     return guarded([&] {

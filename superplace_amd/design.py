@@ -192,6 +192,17 @@ def add_fillers(d: Design, target_density: float, seed: int) -> Design:
     return out
 
 
+def mt_uniform(seed: int, count: int, lo: float, hi: float) -> np.ndarray:
+    """`count` successive Rng::uniform(lo, hi) draws of the reference's
+    generator (src/util/rng.h:15-18: std::mt19937 seeded with `seed`,
+    uniform_real_distribution<float>), through the library (sp_rng_uniform)."""
+    from . import hip
+    out = np.zeros(max(count, 1), np.float32)
+    hip.check(hip.lib().sp_rng_uniform(int(seed) & (2**64 - 1), int(count), float(lo), float(hi),
+                                       hip._ptr(out)))
+    return out[:count]
+
+
 # ----------------------------------------------------------------------------- generators
 def _degree_tail_powerlaw(mean_tail: float, dmin: int, dmax: int) -> float:
     """Exponent a of p(d) ~ d^-a on [dmin, dmax] whose mean is mean_tail (bisection)."""
@@ -219,6 +230,24 @@ def _sample_degrees(rng: Rng, n: int, tail_probs: np.ndarray, dmin_tail: int) ->
     cdf[-1] = 1.0
     u = rng.uniform(n)
     return degs[np.searchsorted(cdf, u, side="right")].astype(np.int64)
+
+
+def _fit_pin_count(degrees: np.ndarray, target: int, dmin_tail: int, dmax: int) -> np.ndarray:
+    """The sampled degrees moved to exactly `target` pins: one pin more (or
+    less) per tail net (degree >= dmin_tail), in net order, round after
+    round, within [dmin_tail, dmax]."""
+    deg = degrees.copy()
+    diff = int(target) - int(deg.sum())
+    tail = np.flatnonzero(deg >= dmin_tail)
+    while diff != 0:
+        step = 1 if diff > 0 else -1
+        cand = tail[deg[tail] < dmax] if step > 0 else tail[deg[tail] > dmin_tail]
+        if len(cand) == 0:
+            raise ValueError("pin count out of reach of the degree range")
+        k = min(abs(diff), len(cand))
+        deg[cand[:k]] += step
+        diff -= step * k
+    return deg
 
 
 def _netlist(rng: Rng, degrees: np.ndarray, num_movable: int, num_fixed: int,
@@ -280,7 +309,8 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
                  fixed_frac: float = 0.10, seed: int = 1002, init_sigma_frac: float = 0.05,
                  locality: float = 8.0, fixed_pin_frac: float = 0.01, name: str = "synth",
                  target_density: float = 1.0, with_fillers: bool = True,
-                 num_macros: int = 0, macro_area_frac: float = 0.0) -> Design:
+                 num_macros: int = 0, macro_area_frac: float = 0.0,
+                 num_pins: int | None = None) -> Design:
     """Seeded synthetic standard-cell design (the generator behind the
     benchmark workloads): num_cells movable cells of width U{1..8} sites and
     one row height, num_fixed fixed macros covering about fixed_frac of the
@@ -293,7 +323,10 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
     macro_area_frac of the movable area: areas U(0.5, 1.5) times the mean,
     aspect ratios U(0.5, 2), heights rounded to whole rows and at least two
     rows (the reader's macro rule is taller than 1.5 rows), widths keeping
-    the drawn area."""
+    the drawn area.
+
+    num_pins: the sampled degrees are moved to exactly this many pins
+    (_fit_pin_count) before the pins are placed."""
     rng = Rng(seed)
     nrows = int(die // row_height)
     rows = (np.arange(nrows, dtype=np.float32) * row_height,
@@ -335,6 +368,8 @@ def synth_design(num_cells: int, num_fixed: int, num_nets: int, die: float, util
         a = _degree_tail_powerlaw((4.27 - 0.55 * 2 - 0.2 * 3) / 0.25, 4, tail_max)
         tail_probs = np.arange(4, tail_max + 1, dtype=np.float64) ** -a
     degrees = _sample_degrees(rng, num_nets, tail_probs, 4)
+    if num_pins is not None:
+        degrees = _fit_pin_count(degrees, num_pins, 4, tail_max)
     pin2node, pin2net = _netlist(rng, degrees, n_mov, n_fixed, fixed_pin_frac, locality)
     node_w = np.concatenate([cw, fw])
     node_h = np.concatenate([ch, fh])
@@ -372,50 +407,81 @@ def config_mixed(seed: int = 1002, with_fillers: bool = True) -> Design:
 
 
 def config_adaptec1(seed: int = 1002, with_fillers: bool = True) -> Design:
-    """adaptec1-shaped: 211k cells, 543 fixed macros, 221k nets (~944k pins,
-    power-law degree tail up to 500), die 11,600 with rows of 12, 57%
+    """adaptec1-shaped: 211k cells, 543 fixed macros, 221k nets with 944k
+    pins (power-law degree tail up to 500), die 11,600 with rows of 12, 57%
     utilisation (512x512 bins in the workload)."""
     return synth_design(211_000, 543, 221_000, die=11600.0, util=0.57, tail="powerlaw",
                         tail_max=500, fixed_frac=0.10, seed=seed, name="synth-adaptec1",
-                        with_fillers=with_fillers, locality=12.0)
+                        with_fillers=with_fillers, locality=12.0, num_pins=944_000)
+
+
+_FIX_CORE, _FIX_ROW = 1200.0, 12.0
+
+
+def _fixture_netlist(w, h, n_mov: int, n_nodes: int):
+    """3-pin nets by the fixtures' rule (test/test_eplace.cpp:80-94): for each
+    movable i, nodes a=i, b=(7i+3) mod n_mov, c=(13i+11) mod n_nodes, skipped
+    when two coincide; pins at (+w/4, -h/4) of their node."""
+    i = np.arange(n_mov)
+    a, b, c = i, (i * 7 + 3) % n_mov, (i * 13 + 11) % n_nodes
+    keep = (a != b) & (b != c) & (a != c)
+    nodes = np.stack([a[keep], b[keep], c[keep]], axis=1).reshape(-1)
+    nets = int(keep.sum())
+    pin2net = np.repeat(np.arange(nets), 3)
+    offx = (np.float32(0.25) * w[nodes]).astype(np.float32)
+    offy = (np.float32(-0.25) * h[nodes]).astype(np.float32)
+    return nodes, pin2net, offx, offy, nets
+
+
+def _fixture_rows():
+    nrows = int(_FIX_CORE / _FIX_ROW)
+    return (np.arange(nrows) * _FIX_ROW, np.full(nrows, _FIX_ROW), np.zeros(nrows),
+            np.full(nrows, _FIX_CORE))
 
 
 def fixture_eplace(num_cells: int, num_macros: int, seed: int = 7) -> Design:
-    """The shape of the reference's mGP integration fixture
-    (test/test_eplace.cpp:45-113): 1200x1200 core of rows 12, cells 8x12 and
-    macros 100x108 clustered +-20 around the centre, four 60x60 fixed corner
-    terminals, 3-pin nets by the rule a=i, b=(7i+3) mod M, c=(13i+11) mod N.
-    Positions come from the splitmix64 stream instead of mt19937."""
-    core, rh = 1200.0, 12.0
-    rng = Rng(seed)
+    """The reference's mGP integration fixture (makeFixture,
+    test/test_eplace.cpp:45-113), node for node: a 1200x1200 core of rows 12;
+    cells 8x12, then macros 100x108, at the core centre plus
+    Rng(seed).uniform(-20, 20) for x then y of each node in order; four
+    60x60 fixed terminals at (40 | 1160, 40 | 1160); 3-pin nets by the
+    fixture rule."""
     n_mov = num_cells + num_macros
-    w = np.concatenate([np.full(num_cells, 8.0), np.full(num_macros, 100.0)])
-    h = np.concatenate([np.full(num_cells, rh), np.full(num_macros, 9 * rh)])
-    x = 0.5 * core + rng.uniform(n_mov, -20.0, 20.0)
-    y = 0.5 * core + rng.uniform(n_mov, -20.0, 20.0)
-    flags = np.concatenate([np.zeros(num_cells, np.uint8), np.full(num_macros, F_MACRO, np.uint8)])
-    tx = np.array([40.0, core - 40.0, 40.0, core - 40.0])
-    ty = np.array([40.0, 40.0, core - 40.0, core - 40.0])
-    x = np.concatenate([x, tx]); y = np.concatenate([y, ty])
-    w = np.concatenate([w, np.full(4, 60.0)]); h = np.concatenate([h, np.full(4, 60.0)])
-    flags = np.concatenate([flags, np.full(4, F_FIXED, np.uint8)])
-    n_nodes = n_mov + 4
-    pin2node, pin2net, offx, offy = [], [], [], []
-    nets = 0
-    for i in range(n_mov):
-        a, b, c = i, (i * 7 + 3) % n_mov, (i * 13 + 11) % n_nodes
-        if a == b or b == c or a == c:
-            continue
-        for nd in (a, b, c):
-            pin2node.append(nd)
-            offx.append(0.25 * w[nd])
-            offy.append(-0.25 * h[nd])
-            pin2net.append(nets)
-        nets += 1
-    nrows = int(core / rh)
-    rows = (np.arange(nrows) * rh, np.full(nrows, rh), np.zeros(nrows), np.full(nrows, core))
-    return make_design(x, y, w, h, flags, n_mov, pin2node, pin2net, offx, offy, nets, rows,
-                       name=f"fixture-{num_cells}-{num_macros}")
+    w = np.concatenate([np.full(num_cells, 8.0), np.full(num_macros, 100.0), np.full(4, 60.0)])
+    h = np.concatenate([np.full(num_cells, _FIX_ROW), np.full(num_macros, 9 * _FIX_ROW),
+                        np.full(4, 60.0)]).astype(np.float32)
+    w = w.astype(np.float32)
+    u = mt_uniform(seed, 2 * n_mov, -20.0, 20.0)
+    centre = np.float32(0.5 * _FIX_CORE)
+    x = np.concatenate([centre + u[0::2], [40.0, _FIX_CORE - 40.0, 40.0, _FIX_CORE - 40.0]])
+    y = np.concatenate([centre + u[1::2], [40.0, 40.0, _FIX_CORE - 40.0, _FIX_CORE - 40.0]])
+    flags = np.concatenate([np.zeros(num_cells, np.uint8), np.full(num_macros, F_MACRO, np.uint8),
+                            np.full(4, F_FIXED, np.uint8)])
+    nodes, pin2net, offx, offy, nets = _fixture_netlist(w, h, n_mov, n_mov + 4)
+    return make_design(x, y, w, h, flags, n_mov, nodes, pin2net, offx, offy, nets,
+                       _fixture_rows(), name=f"fixture-{num_cells}-{num_macros}")
+
+
+def fixture_overfull() -> Design:
+    """The reference's overfull fixture (makeOverfullFixture,
+    test/test_eplace.cpp:123-180): fixture_eplace(3000, 0) with 40 movable
+    macros of 208x208 appended to the movable nodes (centre plus
+    Rng(11).uniform(-20, 20), x then y), so that the movable area is about
+    1.4 times the core's; nets by the fixture rule over the new node order."""
+    base = fixture_eplace(3000, 0)
+    n = base.num_movable
+    u = mt_uniform(11, 80, -20.0, 20.0)
+    centre = np.float32(0.5 * _FIX_CORE)
+    cat = np.concatenate
+    x = cat([base.node_x[:n], centre + u[0::2], base.node_x[n:]])
+    y = cat([base.node_y[:n], centre + u[1::2], base.node_y[n:]])
+    w = cat([base.node_w[:n], np.full(40, 208.0, np.float32), base.node_w[n:]])
+    h = cat([base.node_h[:n], np.full(40, 208.0, np.float32), base.node_h[n:]])
+    flags = cat([base.node_flags[:n], np.full(40, F_MACRO, np.uint8), base.node_flags[n:]])
+    nm = n + 40
+    nodes, pin2net, offx, offy, nets = _fixture_netlist(w, h, nm, nm + 4)
+    return make_design(x, y, w, h, flags, nm, nodes, pin2net, offx, offy, nets, _fixture_rows(),
+                       name="fixture-overfull")
 
 
 def with_fillers(d: Design, target_density: float = 1.0, seed: int = 1002) -> Design:

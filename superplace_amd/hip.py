@@ -218,6 +218,7 @@ def lib():
     L.sp_metrics_destroy.argtypes = [_P]
     L.sp_filler_plan.argtypes = [_P, C.c_float, _P]
     L.sp_filler_scatter.argtypes = [_P, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, _P]
+    L.sp_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_float, C.c_float, _P]
     _lib = L
     return L
 

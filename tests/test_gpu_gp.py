@@ -161,8 +161,7 @@ def test_cgp_keeps_macros_fixed():
 
 
 def test_stagnation_guard_rolls_back():
-    from tests.test_oracle_eplace import overfull
-    d = D.with_fillers(overfull())
+    d = D.with_fillers(D.fixture_overfull())
     g, grows = hip.GpEngine(d, hip.GpConfig(gp_max_iter=500, gp_stagnation_window=40)).run()
     u, _ = hip.GpEngine(d, hip.GpConfig(gp_max_iter=500, gp_stagnation_window=0)).run()
     assert g["stagnated"] and not g["converged"] and g["overflow"] > 0.2
@@ -171,6 +170,18 @@ def test_stagnation_guard_rolls_back():
     rb = [r for r in grows if r["iter"] == g["rollback_iter"]][0]
     assert g["overflow"] == pytest.approx(rb["overflow"], rel=1e-6, abs=1e-7)
     assert g["hpwl"] == pytest.approx(rb["hpwl"], rel=1e-9)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_overfull_published_band(precision):
+    """The reference's published overfull answer (309 iterations, tau 0.2956,
+    HPWL 670765, taken at a 0.1% stagnation threshold) as a band."""
+    from tests.test_oracle_eplace import overfull_band
+    d = D.with_fillers(D.fixture_overfull())
+    r, _ = hip.GpEngine(d, hip.GpConfig(gp_max_iter=500, gp_stagnation_window=40,
+                                        gp_stagnation_min_gain=0.001, precision=precision)).run()
+    print(precision, r["iterations"], r["overflow"], r["hpwl"], flush=True)
+    overfull_band(r)
 
 
 def test_nonfinite_input_rejected():

@@ -62,34 +62,36 @@ def test_determinism():
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
 
 
-def overfull():
-    """test_eplace.cpp:115-180: 3000 cells + 40 macros 208x208 (1.4x core)."""
-    base = D.fixture_eplace(3000, 0)
-    rng = D.Rng(11)
-    n = base.num_movable
-    x = np.concatenate([base.node_x[:n], 600 + rng.uniform(40, -20, 20), base.node_x[n:]])
-    y = np.concatenate([base.node_y[:n], 600 + rng.uniform(40, -20, 20), base.node_y[n:]])
-    w = np.concatenate([base.node_w[:n], np.full(40, 208.0), base.node_w[n:]])
-    h = np.concatenate([base.node_h[:n], np.full(40, 208.0), base.node_h[n:]])
-    f = np.concatenate([base.node_flags[:n], np.full(40, D.F_MACRO, np.uint8), base.node_flags[n:]])
-    nm = n + 40
-    nn = nm + 4
-    p2n, p2e, ox, oy = [], [], [], []
-    nets = 0
-    for i in range(nm):
-        a, b, c = i, (i * 7 + 3) % nm, (i * 13 + 11) % nn
-        if a == b or b == c or a == c:
-            continue
-        for nd in (a, b, c):
-            p2n.append(nd); ox.append(0.25 * w[nd]); oy.append(-0.25 * h[nd]); p2e.append(nets)
-        nets += 1
-    rows = (base.row_ly, base.row_height, base.row_lx, base.row_hx)
-    return D.make_design(x, y, w, h, f, nm, p2n, p2e, ox, oy, nets, rows)
+# The reference's overfull stagnation case as published (test/test_eplace.cpp:
+# 509-515: 309 iterations, tau 0.2956, HPWL 670765) was measured while the
+# stagnation threshold was 0.1% (eplace.cpp:324-327 records its move to the
+# present 2% default): the band below runs that threshold.  The trajectory
+# of an overfull design is sensitive to rounding (the FP32 and FP64 oracles
+# differ by 8% in iterations), hence a band and not an equality.
+PUBLISHED_OVERFULL = dict(iterations=309, overflow=0.2956, hpwl=670765.0)
+
+
+def overfull_band(res):
+    p = PUBLISHED_OVERFULL
+    assert res["stagnated"] and not res["converged"]
+    assert abs(res["iterations"] / p["iterations"] - 1) <= 0.15, res["iterations"]
+    assert abs(res["overflow"] - p["overflow"]) <= 0.003, res["overflow"]
+    assert abs(res["hpwl"] / p["hpwl"] - 1) <= 0.15, res["hpwl"]
+    assert res["overflow"] >= 0.4 / 1.4   # the area floor
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_overfull_published_band(precision):
+    d = D.with_fillers(D.fixture_overfull())
+    r, rows, *_ = run(d, gp_max_iter=500, gp_stagnation_window=40, gp_stagnation_min_gain=0.001,
+                      precision=precision)
+    overfull_band(dict(stagnated=r.stagnated, converged=r.converged, iterations=r.iterations,
+                       overflow=r.overflow, hpwl=r.hpwl))
 
 
 @pytest.mark.slow
 def test_stagnation_guard_and_rollback():
-    d = D.with_fillers(overfull())
+    d = D.with_fillers(D.fixture_overfull())
     assert d.num_fillers == 0
     g, grows, *_ = run(d, gp_max_iter=500, gp_stagnation_window=40)
     u, urows, *_ = run(d, gp_max_iter=500, gp_stagnation_window=0)
