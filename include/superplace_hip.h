@@ -66,6 +66,17 @@ int sp_hip_poisson_solve(sp_hip_poisson* p, const float* density, float* phi, fl
  * No counterpart in the reference (its CUDA backend always crosses to the
  * host, cuda_poisson.cu:194-210); it is the data path of the engine below. */
 int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* out3_dev);
+/* sp_hip_poisson_solve_dev ordered on the caller's stream (a hipStream_t;
+ * NULL = the default stream) instead of synchronised: the solve runs after
+ * the work enqueued on `stream` so far, and work enqueued on `stream`
+ * afterwards runs after the solve; the host does not wait.  The device-span
+ * form of PoissonBackend::solve (src/numeric/poisson_backend.h:36-37) for a
+ * caller whose density and fields live on its own stream, where the
+ * reference's CUDA backend enqueues on a stream of its own and synchronises
+ * (src/numeric/cuda/cuda_poisson.cu:194-210).  One call at a time per
+ * handle, as every handle call. */
+int sp_hip_poisson_solve_async(sp_hip_poisson* p, const void* density_dev, void* out3_dev,
+                               void* stream);
 /* The FP32 solve of sp_hip_poisson_solve_dev without a handle and without
  * allocating: no counterpart in the reference; for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,

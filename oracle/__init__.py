@@ -100,6 +100,7 @@ def lib():
 
 
 CHECKER_PATH = HERE / "_ref" / "superplace-check"
+ADAPTER_CHECK_PATH = HERE / "_ref" / "adapter_check"
 
 
 def check_placement(aux_path, pl_path):
@@ -112,6 +113,21 @@ def check_placement(aux_path, pl_path):
     if p.returncode not in (0, 1):
         raise RuntimeError(f"superplace-check failed to run: {p.stderr.strip()}")
     return p.returncode == 0, p.stdout
+
+
+def adapter_check(aux_path, bin_dim, precision=32):
+    """The reference-side adapters (makeHipPoissonBackend, runHipStage) built
+    against the reference's headers and sources (oracle/_ref/adapter_check):
+    mGP of the BookShelf design with the reference's fillers, and a 96 x 64
+    Poisson solve; a dict of the results and hashes, or None if not built."""
+    import json
+    if not ADAPTER_CHECK_PATH.exists():
+        return None
+    p = subprocess.run([str(ADAPTER_CHECK_PATH), str(aux_path), str(int(bin_dim)),
+                        str(int(precision))], capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError(f"adapter_check failed ({p.returncode}): {p.stderr.strip()}")
+    return json.loads(p.stdout.strip().splitlines()[-1])
 
 
 def ref_lib():

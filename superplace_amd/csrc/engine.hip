@@ -1477,6 +1477,7 @@ public:
     virtual ~PoissonHandleBase() = default;
     virtual void solveHost(const float* rho, float* phi, float* ex, float* ey) = 0;
     virtual void solveDev(const void* rho, void* out3) = 0;
+    virtual void solveAsync(const void* rho, void* out3, hipStream_t caller) = 0;
     virtual const char* name() const = 0;
 };
 
@@ -1495,6 +1496,8 @@ public:
             SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hOut_), 3 * n * sizeof(T), hipHostMallocDefault));
             dIn_ = dalloc<T>(n);
             dOut_ = dalloc<T>(3 * n);
+            for (hipEvent_t* e : {&evIn_, &evOut_})
+                SPHIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
         } catch (...) {
             cleanup();
             throw;
@@ -1528,6 +1531,17 @@ public:
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
 
+    // Ordered on the caller's stream: the handle's stream waits for the
+    // caller's work so far, the caller's stream for the solve.
+    void solveAsync(const void* rho, void* out3, hipStream_t caller) override {
+        if (!rho || !out3) throw InvalidArg("Poisson: null device pointer");
+        SPHIP_CHECK(hipEventRecord(evIn_, caller));
+        SPHIP_CHECK(hipStreamWaitEvent(s_, evIn_, 0));
+        run(static_cast<const T*>(rho), static_cast<T*>(out3));
+        SPHIP_CHECK(hipEventRecord(evOut_, s_));
+        SPHIP_CHECK(hipStreamWaitEvent(caller, evOut_, 0));
+    }
+
     const char* name() const override {
         if (plan_->usesFft()) return sizeof(T) == 4 ? "hip-fft" : "hip-fft-fp64";
         return sizeof(T) == 4 ? "hip-mfma" : "hip-mfma-fp64";
@@ -1553,6 +1567,9 @@ private:
         if (hOut_) (void)hipHostFree(hOut_);
         if (dIn_) (void)hipFree(dIn_);
         if (dOut_) (void)hipFree(dOut_);
+        for (hipEvent_t e : {evIn_, evOut_})
+            if (e) (void)hipEventDestroy(e);
+        evIn_ = evOut_ = nullptr;
         if (s_) (void)hipStreamDestroy(s_);
         hIn_ = hOut_ = dIn_ = dOut_ = nullptr;
         s_ = nullptr;
@@ -1561,6 +1578,7 @@ private:
     hipStream_t s_ = nullptr;
     std::unique_ptr<PoissonPlan<T>> plan_;
     T *hIn_ = nullptr, *hOut_ = nullptr, *dIn_ = nullptr, *dOut_ = nullptr;
+    hipEvent_t evIn_ = nullptr, evOut_ = nullptr;
 };
 
 }  // namespace
@@ -1890,6 +1908,14 @@ int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* o
     return guarded([&] {
         if (!p) throw sphip::InvalidArg("null handle");
         p->impl->solveDev(density_dev, out3_dev);
+    });
+}
+
+int sp_hip_poisson_solve_async(sp_hip_poisson* p, const void* density_dev, void* out3_dev,
+                               void* stream) {
+    return guarded([&] {
+        if (!p) throw sphip::InvalidArg("null handle");
+        p->impl->solveAsync(density_dev, out3_dev, static_cast<hipStream_t>(stream));
     });
 }
 

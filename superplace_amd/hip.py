@@ -168,6 +168,7 @@ def lib():
                                         C.POINTER(_P)]
     L.sp_hip_poisson_solve.argtypes = [_P] * 5
     L.sp_hip_poisson_solve_dev.argtypes = [_P] * 3
+    L.sp_hip_poisson_solve_async.argtypes = [_P] * 4
     L.sp_hip_poisson_name.argtypes = [_P]
     L.sp_hip_poisson_destroy.argtypes = [_P]
     L.sp_hip_gp_create.argtypes = [_P, _P, C.POINTER(_P)]
@@ -287,6 +288,12 @@ class HipPoissonBackend:
 
     def solve_device(self, density_ptr: int, out3_ptr: int) -> None:
         check(lib().sp_hip_poisson_solve_dev(self._h, _P(density_ptr), _P(out3_ptr)))
+
+    def solve_async(self, density_ptr: int, out3_ptr: int, stream: int | None = None) -> None:
+        """solve_device ordered on the caller's HIP stream (None: the default
+        stream), without waiting (sp_hip_poisson_solve_async)."""
+        check(lib().sp_hip_poisson_solve_async(self._h, _P(density_ptr), _P(out3_ptr),
+                                               _P(stream) if stream else None))
 
     def name(self) -> str:
         return lib().sp_hip_poisson_name(self._h).decode()

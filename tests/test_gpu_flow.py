@@ -53,13 +53,29 @@ def test_mixed_size_flow(tmp_path):
     assert ok, report
 
 
-def test_small_flow_with_qp_init(tmp_path):
+_SMALL = {}
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_small_flow_with_qp_init(tmp_path, precision):
+    """The small design (2,000 cells, 128x128 bins) through QP init -> mGP ->
+    Abacus, on the FP64 path (as the small configuration runs) and on the
+    FP32 path; their global-placement HPWLs agree within 1% and the
+    legalized ones (Abacus moves cells by up to a site and a row, which
+    spreads a small difference) within 2%."""
     d = D.config_small(with_fillers=False)
-    cfg = flow.FlowConfig(gp=hip.GpConfig(gp_bin_dim=128), stages=("init", "gp", "lg"))
+    cfg = flow.FlowConfig(gp=hip.GpConfig(gp_bin_dim=128, precision=precision),
+                          stages=("init", "gp", "lg"))
     aux, out, s, pl = run_flow(tmp_path, d, "small", cfg)
+    print(precision, {k: s[k] for k in ("mgp_iterations", "mgp_overflow", "gp_hpwl", "lg_hpwl")},
+          flush=True)
     assert s["run_status"] == "completed"
     assert s["init_hpwl"] < s["init_hpwl_before"]
-    assert s["gp_movable_macros"] == 0 and s["mGP_converged"] == 1
+    assert s["gp_movable_macros"] == 0 and s["mGP_converged"] == 1 and s["mgp_overflow"] < 0.10
     assert s["lg_failed"] == 0
     ok, report = checked(aux, pl)
     assert ok, report
+    _SMALL[precision] = s
+    if len(_SMALL) == 2:
+        assert abs(_SMALL[32]["gp_hpwl"] / _SMALL[64]["gp_hpwl"] - 1) < 0.01
+        assert abs(_SMALL[32]["lg_hpwl"] / _SMALL[64]["lg_hpwl"] - 1) < 0.02

@@ -123,6 +123,45 @@ def test_determinism_and_reuse():
         assert np.array_equal(p, q) and np.array_equal(p, r)
 
 
+@pytest.mark.parametrize("nx,ny", [(512, 512), (100, 60)])
+def test_async_solve_is_ordered_on_the_callers_stream(nx, ny):
+    """sp_hip_poisson_solve_async on a caller's stream: the density copied in
+    on that stream, two solves in a row with no host wait between (the second
+    density overwrites the first only after the first solve has read it, by
+    stream order), the fields copied out on the stream; each equals the
+    synchronous device solve bit for bit."""
+    import ctypes as C
+    from tests.hiprt import DevArray, rt, H2D, D2H
+    R = rt()
+    R.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+    R.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    R.hipStreamSynchronize.argtypes = [C.c_void_p]
+    R.hipStreamDestroy.argtypes = [C.c_void_p]
+    rhos = [np.random.default_rng(20 + k).uniform(-1, 1, (nx, ny)).astype(np.float32)
+            for k in range(2)]
+    be = hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 3.0, 2.0))
+    want = []
+    for rho in rhos:
+        d_in, d_out = DevArray(rho), DevArray.zeros(3 * nx * ny, np.float32)
+        be.solve_device(d_in.ptr, d_out.ptr)
+        want.append(d_out.get())
+    s = C.c_void_p()
+    assert R.hipStreamCreateWithFlags(C.byref(s), 1) == 0
+    d_in, d_out = DevArray.zeros((nx, ny), np.float32), DevArray.zeros(3 * nx * ny, np.float32)
+    got = [np.zeros(3 * nx * ny, np.float32) for _ in rhos]
+    for rho, g in zip(rhos, got):
+        assert R.hipMemcpyAsync(C.c_void_p(d_in.ptr), rho.ctypes.data_as(C.c_void_p), rho.nbytes,
+                                H2D, s) == 0
+        be.solve_async(d_in.ptr, d_out.ptr, s.value)
+        assert R.hipMemcpyAsync(g.ctypes.data_as(C.c_void_p), C.c_void_p(d_out.ptr), g.nbytes,
+                                D2H, s) == 0
+    assert R.hipStreamSynchronize(s) == 0
+    assert R.hipStreamDestroy(s) == 0
+    be.close()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
 def test_bad_span_rejected():
     be = hip.HipPoissonBackend(hip.BackendConfig(16, 16, 1.0, 1.0))
     with pytest.raises(ValueError):
