@@ -286,7 +286,7 @@ def main():
         sys.exit(2)
 
     from superplace_amd import design as D
-    from superplace_amd import hip
+    from superplace_amd import hip, tune
 
     t0 = time.time()
     d = D.config_adaptec1()
@@ -378,6 +378,7 @@ def main():
         "evals_per_iter": evals_per_iter,
         "iteration_roofline": iteration,
         "kernel_us_per_launch": {f: round(v, 2) for f, v in fam_us.items()},
+        "launch_shapes": tune.get_shapes(512),
         "batch": batch,
         "serving": served,
     }

@@ -152,10 +152,35 @@ typedef struct sp_hip_gp_config {
     /* How the iteration loop is driven (no counterpart in the reference):
      * 0 = host decisions, the host reads the scalars after every evaluation;
      * 1 = decisions on the device, kernels launched slot by slot;
-     * 2 = decisions on the device, graph_slots iterations per hipGraph launch. */
+     * 2 = decisions on the device, graph_slots iterations per hipGraph launch
+     * (0 = the launch shapes' graph_slots, below). */
     int32_t loop_mode;
     int32_t graph_slots;
 } sp_hip_gp_config;
+
+/* Launch shapes of the device loop per bin-grid size: the choices the
+ * autotuner (superplace_amd/tune.py) makes from measured timings.  No shape
+ * changes a result bit: the scatter sums integers, the sort only reorders
+ * the density records, the column pass transforms every column pair alike
+ * and the graph only groups iterations.  The library holds a compiled-in
+ * table (the tuner's last output, superplace_amd/shapes.json); calls below
+ * override it for the instances created afterwards.  No counterpart in the
+ * reference (fixed CPU code paths). */
+typedef struct sp_hip_shapes {
+    int32_t scatter_chunk;   /* density records per scatter block: 256, 512, 1024 or 2048 */
+    int32_t sort_period;     /* slots between the sort passes' launches, 1-8 */
+    int32_t col_pairs;       /* column pairs per block of the Poisson column pass: 1, 2, 4
+                              * or 8; 0 = the column pass shares its launch with the
+                              * WA blocks (power-of-two grids) */
+    int32_t graph_slots;     /* iterations per graph launch where the configuration's
+                              * graph_slots is 0, 1-64 */
+} sp_hip_shapes;
+/* bin_dim 0 sets the shapes of every size without an entry of its own. */
+int sp_hip_shapes_set(int bin_dim, const sp_hip_shapes* shapes);
+/* The shapes an instance with this many bins per side takes. */
+int sp_hip_shapes_get(int bin_dim, sp_hip_shapes* shapes);
+/* Back to the compiled-in table. */
+int sp_hip_shapes_reset(void);
 
 /* One metrics.csv row (src/util/metrics.h:12-28). */
 typedef struct sp_hip_iter_metrics {
@@ -203,6 +228,7 @@ typedef struct sp_hip_gp_info {
     double step_x, step_y, origin_x, origin_y;
     double movable_area_scaled;   /* tau denominator (bin_grid.cpp:182-188) */
     int64_t device_bytes;         /* device memory the instance allocated so far */
+    sp_hip_shapes shapes;         /* the launch shapes it runs with */
 } sp_hip_gp_info;
 
 /* Kernel families timed with HIP events on the engine's stream in the

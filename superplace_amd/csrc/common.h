@@ -5,6 +5,9 @@
 #include <hip/hip_ext.h>
 
 #include <cstdint>
+#include <mutex>
+#include <utility>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
@@ -80,6 +83,27 @@ inline void launchKernel(void (*kernel)(Params...), dim3 grid, dim3 block, uint3
     } else {
         kernel<<<grid, block, shm, s>>>(static_cast<Params>(args)...);
     }
+}
+
+// Raises a kernel's dynamic LDS limit to at least `bytes`.  The attribute
+// belongs to the kernel, not to a launch, so instances with different
+// launch shapes only ever raise it: lowering it for a small grid would
+// break a larger grid's launches of the same kernel in another instance.
+inline void raiseLdsLimit(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> limits;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& kv : limits)
+        if (kv.first == kernel) {
+            if (bytes <= kv.second) return;
+            SPHIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(bytes)));
+            kv.second = bytes;
+            return;
+        }
+    SPHIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(bytes)));
+    limits.emplace_back(kernel, bytes);
 }
 
 // Pairs stored interleaved (x0, y0, x1, y1, ...) and moved as one vector access.

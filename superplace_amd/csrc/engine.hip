@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -104,6 +105,54 @@ int binDim(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
     return dim;
 }
 
+// ---------------------------------------------------------- launch shapes
+// The compiled-in shapes per bin-grid size: the autotuner's last table
+// (superplace_amd/shapes.json); dim 0 is the entry of sizes without one.
+struct DimShapes {
+    int dim;
+    sp_hip_shapes shapes;
+};
+constexpr DimShapes kTunedShapes[] = {
+    {0, {kTileChunk, kSortPeriod, 0, 8}},
+    {128, {256, 4, 0, 8}},
+    {256, {256, 4, 0, 8}},
+    {512, {512, 2, 0, 8}},
+    {1024, {512, 2, 2, 4}},
+};
+
+struct ShapeTable {
+    std::mutex mu;
+    std::vector<DimShapes> entries;   // overrides, then the compiled-in table
+};
+ShapeTable& shapeTable() {
+    static ShapeTable t;
+    return t;
+}
+
+void validateShapes(const sp_hip_shapes& s) {
+    if (!scatterChunkOk(s.scatter_chunk))
+        throw InvalidArg("shapes: scatter_chunk must be 256, 512, 1024 or 2048");
+    if (s.sort_period < 1 || s.sort_period > 8) throw InvalidArg("shapes: sort_period must be 1-8");
+    if (s.col_pairs != 0 && s.col_pairs != 1 && s.col_pairs != 2 && s.col_pairs != 4 &&
+        s.col_pairs != 8)
+        throw InvalidArg("shapes: col_pairs must be 0, 1, 2, 4 or 8");
+    if (s.graph_slots < 1 || s.graph_slots > 64) throw InvalidArg("shapes: graph_slots must be 1-64");
+}
+
+// An instance's shapes: an override of its size, the compiled-in entry of
+// its size, an override of size 0, the compiled-in entry of size 0.
+sp_hip_shapes shapesFor(int dim) {
+    std::lock_guard<std::mutex> lock(shapeTable().mu);
+    const auto& over = shapeTable().entries;
+    for (int want : {dim, 0}) {
+        for (const auto& e : over)
+            if (e.dim == want) return e.shapes;
+        for (const auto& e : kTunedShapes)
+            if (e.dim == want) return e.shapes;
+    }
+    return sp_hip_shapes{kTileChunk, kSortPeriod, 0, 8};
+}
+
 // ===========================================================================
 
 template <typename T>
@@ -136,6 +185,8 @@ public:
         o->origin_y = originY_;
         o->movable_area_scaled = movableAreaScaled_;
         o->device_bytes = static_cast<int64_t>(devBytes_);
+        o->shapes = sh_;
+        o->shapes.col_pairs = fuseWa_ ? 0 : poisson_->columnPairs();
     }
 
     void optIndex(int32_t* out) const override {
@@ -152,6 +203,8 @@ public:
     }
 
     void setProfiling(int mask) override { profMask_ = mask; }
+    // iterations per graph launch: the configuration's, else the shapes'
+    int loopSlots() const { return cfg_.graph_slots > 0 ? cfg_.graph_slots : sh_.graph_slots; }
     void kernelStats(double* ms, int64_t* launches, double* bytes, double* flops) const override {
         for (int f = 0; f < SP_HIP_K_COUNT; ++f) {
             if (ms) ms[f] = kms_[f];
@@ -286,7 +339,7 @@ public:
         counts[BK_SORT_COUNT] = static_cast<int>(
             std::min<long>((dd_.nDens + kBlock - 1) / kBlock, kSortBlocks));
         counts[BK_SORT_PLACE] = counts[BK_SORT_COUNT];
-        counts[BK_SCATTER] = tg.tilesX * tg.tilesY + (dd_.nDens + kTileChunk - 1) / kTileChunk;
+        counts[BK_SCATTER] = tg.tilesX * tg.tilesY + (dd_.nDens + tg.chunk - 1) / tg.chunk;
         counts[BK_FINALIZE] = static_cast<int>(
             (std::max<long>(static_cast<long>(nb_), tg.nKeys) + kBlock - 1) / kBlock);
         counts[BK_WA] = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
@@ -319,6 +372,7 @@ public:
         *p = poisson_->batchArgs(skip);
     }
     int gridDim() const { return dim_; }
+    int sortPeriod() const { return sh_.sort_period; }
     int gridLd() const { return poisson_->ld(); }
     long maxLoopLaunches(int slots) const {
         const int maxIter = cfg_.stage == 1 ? cfg_.filler_only_iters : cfg_.gp_max_iter;
@@ -430,7 +484,7 @@ private:
             launch();
             return;
         }
-        ensureEvents(std::max(1, cfg_.graph_slots));
+        ensureEvents(loopSlots());
         beginK(f, 0);
         launch();
         endK(f, 0);
@@ -442,6 +496,8 @@ private:
     hipStream_t s_ = nullptr;   // from the stream pool
     DevDesign<T> dd_{};
     DevArena arena_;            // the design, grids and optimizer state
+    sp_hip_shapes sh_{};        // launch shapes for this grid size
+    bool fuseWa_ = true;        // column pass and WA blocks in one launch
     void* pinned_ = nullptr;    // pooled pinned block: hScalars_, then hFlag_
     int total_ = 0, n_ = 0, dim_ = 0;
     size_t nb_ = 0;
@@ -542,6 +598,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     movableAreas(d, &cellArea, &macroArea);
     movableAreaScaled_ = cellArea + macroArea * static_cast<double>(target);
     dim_ = binDim(d, cfg_);
+    sh_ = shapesFor(dim_);
     if (dim_ < 2) throw InvalidArg("bin grid must be at least 2x2");
     const float coreW = d.core_hx - d.core_lx, coreH = d.core_hy - d.core_ly;
     if (sizeof(T) == 4) {
@@ -637,6 +694,8 @@ void Engine<T>::build(const sp_hip_design& d) {
 
     poisson_ = std::make_unique<PoissonPlan<T>>(dim_, dim_, stepX_, stepY_, s_, true, &arena_);
     geom_.ld = poisson_->ld();
+    // a column pass of its own where the shapes ask for one and the plan takes it
+    fuseWa_ = !(sh_.col_pairs > 0 && poisson_->setColumnPairs(sh_.col_pairs));
 
     // device design
     dd_.total = total;
@@ -701,6 +760,7 @@ void Engine<T>::build(const sp_hip_design& d) {
         while (side < tg.tilesX) side <<= 1;
         tg.nKeys = side * side;
         if (tg.nKeys > kTileKeys) throw InvalidArg("bin grid wider than 1024 bins");
+        tg.chunk = sh_.scatter_chunk;
         arena_.zeros(&tg.count, tg.nKeys);
         arena_.zeros(&tg.fill, tg.nKeys);
         arena_.zeros(&tg.start, tg.nKeys);
@@ -805,7 +865,8 @@ void Engine<T>::build(const sp_hip_design& d) {
     const long maxBlocks = std::max<long>(
         {(static_cast<long>(nb_) + kBlock - 1) / kBlock, (dd_.waWaveBase[kWaClasses + 1] + 3L) / 4,
          (n_ + kBlock - 1L) / kBlock,
-         static_cast<long>(dd_.tiles.tilesX) * dd_.tiles.tilesY + (dd_.nDens + kTileChunk - 1) / kTileChunk,
+         static_cast<long>(dd_.tiles.tilesX) * dd_.tiles.tilesY +
+             (dd_.nDens + dd_.tiles.chunk - 1) / dd_.tiles.chunk,
          1L});
     ws_.maxBlocks = static_cast<int>(maxBlocks) + 1;
     arena_.zeros(&ws_.partials, 4L * ws_.maxBlocks * 8);
@@ -1202,10 +1263,10 @@ void Engine<T>::launchSlot(bool events, int slot) {
                                                    offsetof(Ctl, done));
     auto b = [&](int f) { if (events) beginK(f, slot); };
     auto e = [&](int f) { if (events) endK(f, slot); };
-    // every kSortPeriod-th slot holds the sort passes; they return at once
+    // every sort_period-th slot holds the sort passes; they return at once
     // unless the previous scatter found too many nodes outside their tile
     // window (ctl->sortNext)
-    const bool sortSlot = slot % kSortPeriod == 0;
+    const bool sortSlot = slot % sh_.sort_period == 0;
     if (sortSlot) {
         b(SP_HIP_K_SORT);
         launchTileSort<T>(dd_, geom_, dCtl_, s_);
@@ -1232,7 +1293,7 @@ void Engine<T>::launchSlot(bool events, int slot) {
     // pass's launch (same blocks, same partials); profiled slots launch the
     // two apart so that each family has its own events.
     bool fused = false;
-    if (!events) {
+    if (!events && fuseWa_) {
         const int waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
         int densityBlocks = 0;
         fused = poisson_->solveWithWa(
@@ -1327,7 +1388,7 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
     // Profiling launches the slots directly (as loop_mode 1): the profiled
     // kernels carry event pairs, which a captured graph does not time.
     const bool graph = cfg_.loop_mode == 2 && profMask_ == 0;
-    const int slots = std::max(1, cfg_.graph_slots);
+    const int slots = loopSlots();
     if (profMask_) ensureEvents(slots);
     if (graph && !graphExec_) {
         // capture `slots` iterations
@@ -1685,6 +1746,11 @@ void BatchSet::Impl::capture() {
     std::vector<std::vector<int4>> maps(BK_KINDS);
     std::vector<int> dims, lds;
     std::vector<std::array<std::vector<int4>, 3>> fftMaps;   // per grid size, per pass
+    // the sort passes at the shortest period among the members (the sort
+    // only reorders records, so a member's results do not depend on it)
+    int sortPeriod = kSortPeriod;
+    for (int j = 0; j < m; ++j) sortPeriod = j == 0 ? members[j]->sortPeriod()
+                                                    : std::min(sortPeriod, members[j]->sortPeriod());
     for (int j = 0; j < m; ++j) {
         int counts[BK_KINDS];
         members[j]->batchShare(&args[j], &pargs[j], counts, doneDev);
@@ -1719,7 +1785,7 @@ void BatchSet::Impl::capture() {
     SPHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
         for (int slot = 0; slot < slots; ++slot) {
-            const int sorted = slot % kSortPeriod == 0 ? 1 : 0;
+            const int sorted = slot % sortPeriod == 0 ? 1 : 0;
             if (sorted)
                 for (int k : {BK_SORT_COUNT, BK_SORT_PLACE})
                     launchSlotBatch<float>(k, dArgs, dMaps[k], count(k), 1, s);
@@ -1908,6 +1974,37 @@ int sp_hip_poisson_solve_dev(sp_hip_poisson* p, const void* density_dev, void* o
     return guarded([&] {
         if (!p) throw sphip::InvalidArg("null handle");
         p->impl->solveDev(density_dev, out3_dev);
+    });
+}
+
+int sp_hip_shapes_set(int bin_dim, const sp_hip_shapes* shapes) {
+    return guarded([&] {
+        if (!shapes) throw sphip::InvalidArg("null shapes");
+        if (bin_dim < 0) throw sphip::InvalidArg("shapes: negative bin dimension");
+        sphip::validateShapes(*shapes);
+        auto& t = sphip::shapeTable();
+        std::lock_guard<std::mutex> lock(t.mu);
+        for (auto& e : t.entries)
+            if (e.dim == bin_dim) {
+                e.shapes = *shapes;
+                return;
+            }
+        t.entries.push_back({bin_dim, *shapes});
+    });
+}
+
+int sp_hip_shapes_get(int bin_dim, sp_hip_shapes* shapes) {
+    return guarded([&] {
+        if (!shapes) throw sphip::InvalidArg("null shapes");
+        *shapes = sphip::shapesFor(bin_dim);
+    });
+}
+
+int sp_hip_shapes_reset(void) {
+    return guarded([&] {
+        auto& t = sphip::shapeTable();
+        std::lock_guard<std::mutex> lock(t.mu);
+        t.entries.clear();
     });
 }
 

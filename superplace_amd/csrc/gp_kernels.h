@@ -20,6 +20,7 @@ struct TileGrid {
     int* size = nullptr;          // records of each key (last sort)
     int* extraStart = nullptr;    // nKeys + 1: first extra scatter block of each key
     unsigned long long* stage = nullptr;   // [tile][node | filler][kTileWin^2]
+    int chunk = 512;              // records per scatter block (a tile's owner takes the first)
 };
 
 template <typename T>
@@ -82,7 +83,9 @@ constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;
 constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 1;
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
-constexpr int kTileChunk = 512;      // nodes per scatter block (a tile's owner takes the first)
+constexpr int kTileChunk = 512;      // default records per scatter block (TileGrid::chunk)
+// The record chunks the scatter kernels are built for (TileGrid::chunk).
+inline bool scatterChunkOk(int c) { return c == 256 || c == 512 || c == 1024 || c == 2048; }
 constexpr double kSpillFrac = 0.05;  // re-sort once this fraction of nodes left their tile window
 constexpr int kSortPeriod = 2;       // ... checked at every second evaluation of the device loop
 constexpr int kSortBlocks = 512;     // grid of the sort passes (they stride over record chunks)

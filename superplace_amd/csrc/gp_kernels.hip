@@ -235,7 +235,7 @@ __device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g
         __syncthreads();
         for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
             const int c = tg.count[k];
-            prefix[k] = c > kTileChunk ? (c - 1) / kTileChunk : 0;
+            prefix[k] = c > tg.chunk ? (c - 1) / tg.chunk : 0;
         }
         __syncthreads();
         const int total = blockExclusiveScan(prefix, tg.nKeys, prefix, waveTot);
@@ -251,7 +251,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
 }
 
 // ------------------------------------------------------------------ (a) scatter
-// Blocks [0, tiles) own one tile each and take its first kTileChunk nodes;
+// Blocks [0, tiles) own one tile each and take its first CHUNK nodes
+// (CHUNK = the grid's TileGrid::chunk, which the sort's extra-block prefix
+// also uses);
 // blocks past that take chunk j >= 1 of a crowded tile (found in extraStart).
 // A block accumulates its nodes' fixed-point charges in the tile's window in
 // LDS; an owner stores the whole window to the stage (plain stores, every
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_place(DevDesign<T> d, GridGeom<
 // to the atomic grids.  Charges outside the window (nodes wider than the
 // halo) go to the atomic grids directly.  The block also copies its part of
 // the records back (A = B) for the next sort.
-template <typename T>
+template <typename T, int CHUNK>
 __device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                                    unsigned long long* nodeQ, unsigned long long* fillQ,
                                    int sorted, double* partials, const Ctl* ctl, int vb) {
@@ -267,7 +269,8 @@ __device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, 
         if (ctl->done) return;
         sorted = sorted && ctl->sortNext;
     }
-    constexpr int PER = kTileChunk / kBlock;
+    static_assert(CHUNK % kBlock == 0, "whole records per thread");
+    constexpr int PER = CHUNK / kBlock;
     constexpr int WIN2 = kTileWin * kTileWin;
     __shared__ unsigned long long win[2 * WIN2];   // node window, then filler window
     const TileGrid& tg = d.tiles;
@@ -297,8 +300,8 @@ __device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, 
     for (int e = threadIdx.x; e < 2 * WIN2; e += kBlock) win[e] = 0;
     __syncthreads();
     const int cnt = tg.size[key], st = tg.start[key];
-    const int lo = st + chunk * kTileChunk;
-    const int hi = st + min(cnt, (chunk + 1) * kTileChunk);
+    const int lo = st + chunk * CHUNK;
+    const int hi = st + min(cnt, (chunk + 1) * CHUNK);
     const int wx0 = tx * kTileBins - kTileHalo, wy0 = ty * kTileBins - kTileHalo;
     Footprint<T> fp[PER];
     uint8_t fl[PER];
@@ -364,12 +367,25 @@ __device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, 
     }
 }
 
-template <typename T>
+template <typename T, int CHUNK>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevDesign<T> d, GridGeom<T> g, T qscale,
                                                     unsigned long long* nodeQ,
                                                     unsigned long long* fillQ, int sorted,
                                                     double* partials, const Ctl* ctl) {
-    scatterBody<T>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, blockIdx.x);
+    scatterBody<T, CHUNK>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, blockIdx.x);
+}
+
+// scatterBody for the grid's chunk (one of scatterChunkOk's).
+template <typename T>
+__device__ inline void scatterAnyChunk(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
+                                       unsigned long long* nodeQ, unsigned long long* fillQ,
+                                       int sorted, double* partials, const Ctl* ctl, int vb) {
+    switch (d.tiles.chunk) {
+        case 256: scatterBody<T, 256>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, vb); break;
+        case 1024: scatterBody<T, 1024>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, vb); break;
+        case 2048: scatterBody<T, 2048>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, vb); break;
+        default: scatterBody<T, 512>(d, g, qscale, nodeQ, fillQ, sorted, partials, ctl, vb); break;
+    }
 }
 
 // ------------------------------------------------------------------ rho + tau
@@ -859,7 +875,8 @@ __global__ __launch_bounds__(kBlock) void k_slot_batch(const SlotArgs<T>* __rest
     } else if constexpr (KIND == BK_SORT_PLACE) {
         tilePlaceBody<T>(a.d, a.g, a.ctl, m.y, m.z);
     } else if constexpr (KIND == BK_SCATTER) {
-        scatterBody<T>(a.d, a.g, a.qscale, a.nodeQ, a.fillQ, sorted, a.ws.region(2), a.ctl, m.y);
+        scatterAnyChunk<T>(a.d, a.g, a.qscale, a.nodeQ, a.fillQ, sorted, a.ws.region(2), a.ctl,
+                           m.y);
     } else if constexpr (KIND == BK_FINALIZE) {
         finalizeBody<T>(a.din, a.rho, a.ctl, m.y);
     } else if constexpr (KIND == BK_WA) {
@@ -919,10 +936,15 @@ template <typename T>
 void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
                    unsigned long long* nodeQ, unsigned long long* fillQ, bool sorted,
                    ReduceWs& ws, const Ctl* ctl, hipStream_t s) {
-    const int blocks = d.tiles.tilesX * d.tiles.tilesY + (d.nDens + kTileChunk - 1) / kTileChunk;
+    const int c = d.tiles.chunk;
+    if (!scatterChunkOk(c)) throw std::logic_error("launchScatter: unsupported record chunk");
+    const int blocks = d.tiles.tilesX * d.tiles.tilesY + (d.nDens + c - 1) / c;
     ws.blocks[2] = blocks;
-    launchKernel(k_scatter<T>, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ,
-                 sorted ? 1 : 0, ws.region(2), ctl);
+    auto kern = c == 256 ? k_scatter<T, 256>
+              : c == 1024 ? k_scatter<T, 1024>
+              : c == 2048 ? k_scatter<T, 2048> : k_scatter<T, 512>;
+    launchKernel(kern, blocks, kBlock, 0, s, d, g, qscale, nodeQ, fillQ, sorted ? 1 : 0,
+                 ws.region(2), ctl);
     SPHIP_CHECK(hipGetLastError());
 }
 

@@ -119,6 +119,14 @@ private:
     size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0, ldsBW_ = 0;
     int colPairs_ = 1;   // column pairs per block of the column pass
     void setFftLds();
+
+public:
+    // Column pairs per block of the FFT column pass when it runs on its own
+    // (a launch shape: every pair is transformed alike, so the results do
+    // not depend on it).  False, with the plan unchanged, where the pairs
+    // do not divide the columns or their LDS would exceed 150 KB.
+    bool setColumnPairs(int pairs);
+    int columnPairs() const { return colPairs_; }
     void solveFft(bool withPhi, const int* skip);
     int passA(const DensityIn<T>* din, const int* skip);   // returns the tau partial count
     void passBC(const int* skip, bool withPhi = false);

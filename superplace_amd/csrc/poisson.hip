@@ -1046,17 +1046,23 @@ void PoissonPlan<T>::setFftLds() {
         const size_t lds = 3 * static_cast<size_t>(G) * nx_ * c;
         ldsBW_ = (kBlock % nt == 0 && (ny_ / 2) % G == 0 && lds <= kFusedLdsMax) ? lds : 0;
         if (ldsBW_)
-            SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols_wa<T>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, ldsBW_));
+            raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols_wa<T>), ldsBW_);
     }
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows_rho<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA_));
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsB_));
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_idct_rows<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsC_));
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows<T>), ldsA_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows_rho<T>), ldsA_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T>), ldsB_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_idct_rows<T>), ldsC_);
+}
+
+template <typename T>
+bool PoissonPlan<T>::setColumnPairs(int pairs) {
+    if (!fft_ || pairs < 1 || (pairs & (pairs - 1)) != 0 || (ny_ / 2) % pairs != 0) return false;
+    const size_t lds = (kFftLdsHead(nx_) + 4 * static_cast<size_t>(pairs) * nx_) * 2 * sizeof(T);
+    if (lds > 150 * 1024 || pairs * fftThreads(nx_) > 1024) return false;
+    colPairs_ = pairs;
+    ldsB_ = lds;
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T>), ldsB_);
+    return true;
 }
 
 template <typename T>
@@ -1233,12 +1239,9 @@ void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, fl
     const size_t ldsA = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
     const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
     const size_t ldsC = (kFftLdsHead(ny) + 4 * static_cast<size_t>(ny)) * c;
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_rows<float>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsA));
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct_cols<float>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsB));
-    SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_idct_rows<float>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, ldsC));
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows<float>), ldsA);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<float>), ldsB);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_idct_rows<float>), ldsC);
     const float invNorm = static_cast<float>(1.0 / (4.0 * nx * ny));
     launchKernel(k_dct_rows<float>, dim3(nx / 2), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsA),
                  s, rho, t0, nx, ny, ny, ty, static_cast<const int*>(nullptr));
@@ -1296,12 +1299,9 @@ void launchPoissonBatch(int pass, const PoissonBatchArgs<T>* args, const int4* m
     std::call_once(once, [] {
         // the largest grid's LDS (1024 per side) covers every smaller one
         const size_t big = (kFftLdsHead(1024) + 4 * 1024) * 2 * sizeof(T);
-        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 0>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 1>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
-        SPHIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fft_batch<T, 2>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 0>), big);
+        raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 1>), big);
+        raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 2>), big);
     });
     if (pass == 0)
         launchKernel(k_fft_batch<T, 0>, dim3(blocks), dim3(fftThreads(ny)),

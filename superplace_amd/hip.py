@@ -137,11 +137,20 @@ class SpHipEvalDebug(C.Structure):
                                   "field_x", "field_y", "force", "grad_wl", "grad")]
 
 
+class SpHipShapes(C.Structure):
+    _fields_ = [("scatter_chunk", C.c_int32), ("sort_period", C.c_int32),
+                ("col_pairs", C.c_int32), ("graph_slots", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SpHipGpInfo(C.Structure):
     _fields_ = [("dim", C.c_int32), ("n_opt", C.c_int32), ("frac_bits", C.c_int32),
                 ("precision", C.c_int32), ("step_x", C.c_double), ("step_y", C.c_double),
                 ("origin_x", C.c_double), ("origin_y", C.c_double),
-                ("movable_area_scaled", C.c_double), ("device_bytes", C.c_int64)]
+                ("movable_area_scaled", C.c_double), ("device_bytes", C.c_int64),
+                ("shapes", SpHipShapes)]
 
 
 _lib = None
@@ -150,7 +159,7 @@ _lib = None
 def declared_functions() -> list[str]:
     """Function names declared in include/superplace_hip.h."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics|abacus|macro)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sp_(?:hip|bookshelf|filler|qp|metrics|abacus|macro|rng)_\w+)\s*\(", text)))
 
 
 def lib():
@@ -350,7 +359,7 @@ class GpConfig:
     precision: int = 32
     max_backtrack: int = 10
     loop_mode: int = 2        # 0 host decisions, 1 device decisions, 2 device decisions + hipGraph
-    graph_slots: int = 8      # iterations per graph launch in loop_mode 2
+    graph_slots: int = 0      # iterations per graph launch in loop_mode 2 (0: the tuned shapes)
 
     def struct(self) -> SpHipGpConfig:
         return SpHipGpConfig(**dataclasses.asdict(self))
