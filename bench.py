@@ -264,6 +264,29 @@ def cpu_baseline(design, iters, min_seconds=10.0, max_runs=8):
                         f"{wall:.1f} s"))
 
 
+def copy_calibration():
+    """One device-to-device copy of 256 MiB (the runtime's copy kernel): the
+    known traffic that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    from superplace_amd import hip
+    hip.lib()   # the runtime the product library links
+    rt = C.CDLL("libamdhip64.so.7")
+    rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rt.hipFree.argtypes = [C.c_void_p]
+    n = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if rt.hipMalloc(C.byref(a), n) or rt.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if rt.hipMemcpy(b, a, n, 3) or rt.hipDeviceSynchronize():   # device to device
+            raise RuntimeError("copy failed")
+    finally:
+        rt.hipFree(a)
+        rt.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +301,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="arrivals per second")
     ap.add_argument("--roofline-only", default="", metavar="FAMILY",
                     help="run only the roofline leg of this kernel family")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -285,6 +310,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     from superplace_amd import design as D
     from superplace_amd import hip, tune
 
