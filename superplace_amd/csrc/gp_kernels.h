@@ -79,7 +79,7 @@ inline int waBlocksFor(int waves) {
     return b < 1 ? 1 : (b > kWaMaxBlocks ? kWaMaxBlocks : b);
 }
 constexpr int kTileBins = 16;        // bins per tile side
-constexpr int kTileHalo = 2;         // halo bins of a tile's scatter window
+constexpr int kTileHalo = 4;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;
 constexpr int kRecFiller = 1 << 30, kRecMacro = 1 << 29, kRecNode = (1 << 29) - 1;
 constexpr int kTileKeys = 4096;      // Morton keys of up to 64 x 64 tiles (1024^2 bins)
