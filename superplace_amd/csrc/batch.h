@@ -30,7 +30,7 @@ struct SlotArgs {
     ReduceWs ws;        // with the block counts of the batched slot
     Ctl* ctl;
     DevRow* rows;
-    T* const* bufs;
+    BufTable<T> bufs;
     double denom;       // tau denominator
     WaTask<T> wa;
     int* doneOut;       // the batched reduce copies ctl->done to doneOut[instance]

@@ -365,7 +365,7 @@ public:
         a->ws = ws;
         a->ctl = dCtl_;
         a->rows = dRows_;
-        a->bufs = dBufs_;
+        a->bufs = bufTab_;
         a->denom = movableAreaScaled_;
         a->wa = WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws.region(1), dCtl_};
         a->doneOut = doneOut;
@@ -520,7 +520,7 @@ private:
     std::vector<T*> snapBufs_;
     // device-resident loop: state, buffer table, metrics rows, snapshot pool, graph
     Ctl* dCtl_ = nullptr;
-    T** dBufs_ = nullptr;
+    BufTable<T> bufTab_{};   // the state buffers of the device loop, by role index
     DevRow* dRows_ = nullptr;
     int rowsCap_ = 0;
     T* snapPool_[kMaxSnaps] = {};
@@ -727,6 +727,17 @@ void Engine<T>::build(const sp_hip_design& d) {
     arena_.upload(&dd_.flags, fl);
     arena_.upload(&dd_.optIdx, optIdxHost_);
     {
+        // optIdx = k below a split and k + shift from it on: the kernels compute it
+        int split = 0;
+        while (split < n_ && optIdxHost_[split] == split) ++split;
+        const int shift = split < n_ ? optIdxHost_[split] - split : 0;
+        bool affine = true;
+        for (int k = split; k < n_ && affine; ++k) affine = optIdxHost_[k] == k + shift;
+        dd_.optAffine = affine ? 1 : 0;
+        dd_.optSplit = split;
+        dd_.optShift = shift;
+    }
+    {
         // density records in node order to start with; the first sort orders them
         std::vector<int> recId(dens.size());
         std::vector<float> recWH(2 * dens.size());
@@ -840,10 +851,6 @@ void Engine<T>::build(const sp_hip_design& d) {
         dd_.waWaveBase[kWaClasses + 1] =
             dd_.waWaveBase[kWaClasses] + (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]);
     }
-    arena_.upload(&dd_.netStart, netStart);
-    arena_.upload(&dd_.slotNode, slotNode);
-    arena_.upload(&dd_.slotOffX, slotOffX);
-    arena_.upload(&dd_.slotOffY, slotOffY);
     // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
     // of pins that no net pass writes (no net, degree < 2 or above
     // ignore_net_degree) stay zero.
@@ -854,7 +861,45 @@ void Engine<T>::build(const sp_hip_design& d) {
         if (pinSlot[p] >= 0) slotNp[pinSlot[p]] = q;
     }
     arena_.upload(&dd_.nodeStart, nodeStart);
-    arena_.upload(&dd_.slotNp, slotNp);
+    // classes below kWaClasses in lane order: net g of class c (W = 2^(c+1)
+    // lanes per net, 64 / W nets per wavefront task) takes lanes
+    // (g % (64 / W)) W + j, j < degree, of task waWaveBase[c] + g / (64 / W)
+    {
+        const long lanes = 64L * dd_.waWaveBase[kWaClasses];
+        std::vector<int> laneNode(lanes, -1), laneNp(lanes, 0);
+        std::vector<float> laneOffX(lanes, 0.f), laneOffY(lanes, 0.f);
+        for (int c = 0; c < kWaClasses; ++c) {
+            const int W = 2 << c, perWave = 64 / W;
+            for (int k = dd_.waNetLo[c]; k < dd_.waNetLo[c + 1]; ++k) {
+                const int g = k - dd_.waNetLo[c];
+                const long base = 64L * (dd_.waWaveBase[c] + g / perWave) + (g % perWave) * W;
+                for (int t = netStart[k]; t < netStart[k + 1]; ++t) {
+                    const long e = base + (t - netStart[k]);
+                    laneNode[e] = slotNode[t];
+                    laneOffX[e] = slotOffX[t];
+                    laneOffY[e] = slotOffY[t];
+                    laneNp[e] = slotNp[t];
+                }
+            }
+        }
+        arena_.upload(&dd_.laneNode, laneNode);
+        arena_.upload(&dd_.laneOffX, laneOffX);
+        arena_.upload(&dd_.laneOffY, laneOffY);
+        arena_.upload(&dd_.laneNp, laneNp);
+    }
+    // the tail class keeps per-net slot ranges, rebased to its first slot
+    {
+        const int k0 = dd_.waNetLo[kWaClasses], t0 = netStart[k0];
+        std::vector<int> tailStart(netStart.begin() + k0, netStart.end());
+        for (int& t : tailStart) t -= t0;
+        arena_.upload(&dd_.tailStart, tailStart);
+        auto tail = [&](auto& v) { return std::vector<typename std::decay_t<decltype(v)>::value_type>(
+                                       v.begin() + t0, v.end()); };
+        arena_.upload(&dd_.slotNode, tail(slotNode));
+        arena_.upload(&dd_.slotOffX, tail(slotOffX));
+        arena_.upload(&dd_.slotOffY, tail(slotOffY));
+        arena_.upload(&dd_.slotNp, tail(slotNp));
+    }
     arena_.zeros(&dd_.pinGrad, 2 * static_cast<size_t>(std::max(1, d.num_pins)));
 
     arena_.zeros(&nodeQ_, nb_);
@@ -881,7 +926,6 @@ void Engine<T>::build(const sp_hip_design& d) {
     // optimizer state
     const size_t n2 = 2 * static_cast<size_t>(std::max(n_, 1));
     arena_.zeros(&dCtl_, 1);
-    arena_.zeros(&dBufs_, kNumBufs);
     if (cfg_.loop_mode != 0)
         for (int i = 0; i < kMaxSnaps; ++i) arena_.zeros(&snapPool_[i], n2);
     for (hipEvent_t& ev : loopEv_) SPHIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -890,7 +934,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     arena_.place(s_);
 
     // algorithmic bytes / flops per launch (the roofline accounting)
-    const double P = d.num_pins, E = d.num_nets, ts = sizeof(T), nd = dd_.nDens;
+    const double P = d.num_pins, ts = sizeof(T), nd = dd_.nDens;
     const double stageBytes = static_cast<double>(dd_.tiles.tilesX) * dd_.tiles.tilesY * 2 *
                               kTileWin * kTileWin * 8;
     // sort: pass 1 reads each record's node index and (x, y); pass 2 reads the
@@ -908,9 +952,11 @@ void Engine<T>::build(const sp_hip_design& d) {
         bytesPer_[SP_HIP_K_FINALIZE] = 0;
     }
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
-    // WA net pass: per slot node id, offsets, the node's x/y, entry id and
-    // the gradient write; netStart per net
-    bytesPer_[SP_HIP_K_WA_NETS] = P * (4 + 2 * ts + 2 * ts + 4 + 2 * ts) + E * 4;
+    // WA net pass: per lane of the class tasks its node id, offsets and entry
+    // id; per pin the node's x/y and the gradient write; tail slot ranges
+    bytesPer_[SP_HIP_K_WA_NETS] = 64.0 * dd_.waWaveBase[kWaClasses] * (4 + 2 * ts + 4) +
+                                  P * (2 * ts + 2 * ts) +
+                                  (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]) * 4.0;
     // node pass: per node its index, x/y, w/h, flags, pin range, the
     // preconditioner terms, u/up/gp reads and the grad write, four field bins
     // (x and y); per pin entry its gradient
@@ -945,7 +991,7 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
     });
     hostTimed(SP_HIP_K_NODE_GRAD, [&] {
         launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), gamma, lambda, u, up, gp, grad,
-                          dbgForce, dbgWl, ws_, nullptr, nullptr, s_);
+                          dbgForce, dbgWl, ws_, nullptr, BufTable<T>{}, s_);
     });
     hostTimed(SP_HIP_K_REDUCE, [&] {
         launchReduce<T>(ws_, movableAreaScaled_, true, nullptr, nullptr, s_);
@@ -961,8 +1007,8 @@ EvalScalars Engine<T>::evaluate(const T* u, const T* up, const T* gp, T* grad, f
 template <typename T>
 void Engine<T>::scatterPos(const T* pos) {
     hostTimed(SP_HIP_K_UPDATE, [&] {
-        launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr,
-                        s_);
+        launchUpdate<T>(dd_, pos, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr,
+                        BufTable<T>{}, s_);
     });
     curPos_ = pos;
 }
@@ -1014,7 +1060,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         }
         SPHIP_CHECK(hipMemcpyAsync(probe_, pos.data(), bytes2n, hipMemcpyHostToDevice, s_));
         // clamp into u (v' = clamp(A) with vOut = u)
-        launchUpdate<T>(dd_, probe_, nullptr, nullptr, 0.0, 0.0, u_, nullptr, nullptr, nullptr, s_);
+        launchUpdate<T>(dd_, probe_, nullptr, nullptr, 0.0, 0.0, u_, nullptr, nullptr,
+                        BufTable<T>{}, s_);
         curPos_ = u_;
     }
 
@@ -1080,7 +1127,7 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
             const double scale = 0.1 * (posNorm > 0.0 ? posNorm : 1.0) / gNorm;
             hostTimed(SP_HIP_K_UPDATE, [&] {
                 launchUpdate<T>(dd_, u_, gA_, nullptr, static_cast<double>(static_cast<T>(scale)),
-                                0.0, probe_, nullptr, nullptr, nullptr, s_);
+                                0.0, probe_, nullptr, nullptr, BufTable<T>{}, s_);
             });
             curPos_ = probe_;
             EvalScalars sB = evaluate(probe_, u_, gA_, gB_, lambda, gamma);
@@ -1133,8 +1180,8 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
                 const double lip = dx / dg;
                 if (lip > 0.95 * alpha) break;
                 alpha = static_cast<double>(static_cast<T>(lip));
-                launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, nullptr, nullptr,
-                                s_);
+                launchUpdate<T>(dd_, uPrev_, gPrev_, vPrev_, alpha, momPrev, v_, u_, nullptr,
+                                BufTable<T>{}, s_);
                 curPos_ = u_;
                 sc = evaluate(u_, uPrev_, gPrev_, g_, lambda, gamma);
                 checkFinite(sc);
@@ -1164,7 +1211,7 @@ void Engine<T>::run(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRow
         const T aNext = T(0.5) * (T(1) + std::sqrt(T(4) * aT * aT + T(1)));
         const T mom = (aT - T(1)) / aNext;
         launchUpdate<T>(dd_, u_, g_, v_, alpha, static_cast<double>(mom), vNew_, uNew_, nullptr,
-                        nullptr, s_);
+                        BufTable<T>{}, s_);
         // rotate (nesterov.cpp:109-118)
         std::swap(uPrev_, u_);
         std::swap(gPrev_, g_);
@@ -1315,13 +1362,13 @@ void Engine<T>::launchSlot(bool events, int slot) {
     }
     b(SP_HIP_K_NODE_GRAD);
     launchNodeGrad<T>(dd_, geom_, poisson_->fieldXY(), 0.f, 0.f, nullptr,
-                      nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, dBufs_, s_);
+                      nullptr, nullptr, nullptr, nullptr, nullptr, ws_, dCtl_, bufTab_, s_);
     e(SP_HIP_K_NODE_GRAD);
     b(SP_HIP_K_REDUCE);
     launchReduce<T>(ws_, movableAreaScaled_, true, dCtl_, dRows_, s_);
     e(SP_HIP_K_REDUCE);
     b(SP_HIP_K_UPDATE);
-    launchUpdate<T>(dd_, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, dCtl_, dBufs_, s_);
+    launchUpdate<T>(dd_, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, dCtl_, bufTab_, s_);
     e(SP_HIP_K_UPDATE);
 }
 
@@ -1330,8 +1377,9 @@ template <typename T>
 void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int maxRows,
                           float lambda, float gamma, double alpha0, double effRef, float targetTau,
                           int maxIter, int maxBt, float stepX) {
-    T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
-    for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
+    static_assert(kNumBufs == 8 + 4, "the table lists the eight roles and four snapshots");
+    const BufTable<T> table{{u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_, snapPool_[0],
+                             snapPool_[1], snapPool_[2], snapPool_[3]}};
     Ctl c{};
     for (int r = 0; r < B_ROLES; ++r) c.cur[r] = c.nxt[r] = r;   // table order = role order
     c.mode = kModeNone;
@@ -1381,7 +1429,15 @@ void Engine<T>::runDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, int 
         }
     }
     SPHIP_CHECK(hipMemcpyAsync(dCtl_, &c, sizeof(Ctl), hipMemcpyHostToDevice, s_));
-    SPHIP_CHECK(hipMemcpyAsync(dBufs_, table, sizeof(table), hipMemcpyHostToDevice, s_));
+    // the loop kernels take the table by value: a graph captured with other
+    // buffers is dropped
+    if (graphExec_ && std::memcmp(&table, &bufTab_, sizeof(table)) != 0) {
+        SPHIP_CHECK(hipGraphExecDestroy(graphExec_));
+        SPHIP_CHECK(hipGraphDestroy(graph_));
+        graphExec_ = nullptr;
+        graph_ = nullptr;
+    }
+    bufTab_ = table;
     if (curPos_ != u_) scatterPos(u_);
     SPHIP_CHECK(hipStreamSynchronize(s_));
 
@@ -1484,8 +1540,6 @@ void Engine<T>::finishDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, i
     inflight_ = 0;
     res->seconds_loop =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - loopT0_).count();
-    T* table[kNumBufs] = {u_, v_, uPrev_, vPrev_, uNew_, vNew_, g_, gPrev_};
-    for (int i = 0; i < kMaxSnaps; ++i) table[8 + i] = snapPool_[i];
     Ctl c{};
     SPHIP_CHECK(hipMemcpyAsync(&c, dCtl_, sizeof(Ctl), hipMemcpyDeviceToHost, s_));
     SPHIP_CHECK(hipStreamSynchronize(s_));
@@ -1527,7 +1581,7 @@ void Engine<T>::finishDevice(sp_hip_gp_result* res, sp_hip_iter_metrics* rows, i
     }
     if (c.finalBuf < 0 || c.finalBuf >= kNumBufs) throw std::runtime_error("device loop: no result");
     double hp = 0;
-    scatterPos(table[c.finalBuf]);
+    scatterPos(bufTab_.p[c.finalBuf]);
     res->overflow = static_cast<float>(deliveredMetrics(&hp));
     res->hpwl = hp;
 }
@@ -1906,8 +1960,10 @@ size_t footprintEstimate(const sp_hip_design& d, const sp_hip_gp_config& cfg) {
     // per optimised node (at most every node): preconditioner, the state
     // vectors and the snapshot pool
     b += total * (12 + (2 * 11 + 2 * kMaxSnaps) * t);
-    // per pin: slot node, offsets, node-order entry, gradient; per net
-    b += pins * (8 + 4 * t) + 4 * (nets + 1);
+    // per pin: node, offsets and node-order entry in lane order (at most two
+    // lanes per pin: a net of degree d takes fewer than 2d lanes) or as a
+    // tail slot, and its gradient; per net
+    b += pins * (2 * (8 + 2 * t) + 2 * t) + 4 * (nets + 1);
     // per bin: fixed-point grids, static densities, tile staging, and the
     // transform buffers of the padded grid
     b += bins * 32 + tiles * tiles * 2 * kTileWin * kTileWin * 8 + padded * padded * 8 * t;

@@ -34,6 +34,10 @@ struct DevDesign {
     const T* h = nullptr;
     const uint8_t* flags = nullptr;
     const int* optIdx = nullptr;     // pos index k -> node i
+    // When optIdx is k for k < optSplit and k + optShift beyond (the usual
+    // opt set: movable nodes, then the fillers past the fixed nodes),
+    // optAffine is set and the kernels compute i instead of loading it.
+    int optAffine = 0, optSplit = 0, optShift = 0;
     // The density nodes in spatial order (by tile of their centre bin, tiles
     // in Morton order) as records: node index with the filler / macro flags
     // in bits 30 / 29 (recId), (w, h) and (x, y) interleaved (recWH, recXY).
@@ -51,13 +55,22 @@ struct DevDesign {
     TileGrid tiles;
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
-    // net -> pin slots.  Nets are renumbered by degree (ascending), so each
-    // WA size class below is a contiguous range of nets and of slots.
-    const int* netStart = nullptr;
+    // Nets are renumbered by degree (ascending), so each WA size class below
+    // is a contiguous range of nets.  Classes below kWaClasses keep their
+    // pins in lane order: entry 64 gw + l is lane l of wavefront task gw
+    // (node -1: an empty lane), with its pin offsets and its node-pin entry
+    // (where its gradient goes).
+    const int* laneNode = nullptr;
+    const T* laneOffX = nullptr;
+    const T* laneOffY = nullptr;
+    const int* laneNp = nullptr;
+    // Tail nets (class kWaClasses): net waNetLo[kWaClasses] + k owns slots
+    // [tailStart[k], tailStart[k + 1]) of the slot arrays.
+    const int* tailStart = nullptr;
     const int* slotNode = nullptr;
     const T* slotOffX = nullptr;
     const T* slotOffY = nullptr;
-    const int* slotNp = nullptr;      // node-pin entry of each slot (where its gradient goes)
+    const int* slotNp = nullptr;
     // WA size classes: class c < kWaClasses holds the nets of degree in
     // (2^c, 2^(c+1)] (class 0: degree 2), 2^(c+1) lanes per net; class
     // kWaClasses the nets of degree > 2^kWaClasses, one wavefront each.
@@ -72,6 +85,7 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
+constexpr int kWaTailRegs = 8;       // pins per lane per batch of a tail net's extrema pass
 constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
 // Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each).
 inline int waBlocksFor(int waves) {
@@ -114,6 +128,21 @@ constexpr int kMaxSnaps = 4;      // tau-ladder snapshots (the ladder holds <= 3
 constexpr int kNumBufs = 8 + kMaxSnaps;
 // roles in the state-buffer table (then kMaxSnaps snapshot buffers)
 enum : int { B_U = 0, B_V, B_UP, B_VP, B_UN, B_VN, B_G, B_GP, B_ROLES };
+
+// The state-buffer table, passed to the loop kernels by value: a role's
+// buffer is then picked from kernel arguments by the index in Ctl, with no
+// dependent load of the table itself.
+template <typename T>
+struct BufTable {
+    T* p[kNumBufs];
+};
+template <typename T>
+__host__ __device__ inline T* pickBuf(const BufTable<T>& t, int i) {
+    T* r = t.p[0];
+#pragma unroll
+    for (int j = 1; j < kNumBufs; ++j) r = i == j ? t.p[j] : r;
+    return r;
+}
 
 enum : int { kModeNone = 0, kModeStep = 1, kModeRedo = 2 };
 enum : int { kRunning = 0, kConverged = 1, kStagnated = 2, kIterLimit = 3, kNonFinite = 4 };
@@ -193,7 +222,7 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
+                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, const BufTable<T>& bufs,
                     hipStream_t s);
 
 // Sums the regions written by this evaluation into EvalScalars (tau is the
@@ -209,6 +238,7 @@ void launchReduce(ReduceWs& ws, double denom, bool withNode, Ctl* ctl, DevRow* r
 // operands, alpha and m come from ctl->mode (step or redo) and the table.
 template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
-                  double m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs, hipStream_t s);
+                  double m, T* vOut, T* uOut, const Ctl* ctl, const BufTable<T>& bufs,
+                  hipStream_t s);
 
 }  // namespace sphip

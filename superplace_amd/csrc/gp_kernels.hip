@@ -432,26 +432,36 @@ __global__ __launch_bounds__(kBlock) void k_wa_pins(WaTask<T> w) {
 //  * FP64 partial sums: sum|dWL|, sum|F| (eplace.cpp:168-190) and the
 //    Nesterov step statistics ||u-up||^2, ||g-gp||^2, (u-up).(g-gp), ||g||^2
 //    (nesterov.cpp:25-44), plus a count of non-finite gradients.
+// Node index of optimised node k (optIdx, computed when it is affine).
+template <typename T>
+__device__ inline int optNode(const DevDesign<T>& d, int k) {
+    if (d.optAffine) return k < d.optSplit ? k : k + d.optShift;
+    return d.optIdx[k];
+}
+
+// Loads are ordered by dependence: the node's own data and the Nesterov
+// vectors first (one memory round trip), then its footprint's field bins and
+// its pin entries (a second; pin entries eight at a time).  The sums run in
+// the same order as a plain sequential loop over pins and bins.
 template <typename T>
 __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                                     T invG, float lambda, const T* u, const T* up, const T* gp,
                                     T* grad, T* dbgForce, T* dbgWl, double* partials,
-                                    const Ctl* ctl, T* const* bufs, int vb) {
+                                    const Ctl* ctl, const BufTable<T>& bufs, int vb) {
     if (ctl) {
         if (ctl->done) return;
         lambda = ctl->lambda;
         invG = invGammaOf<T>(ctl->gamma);
-        u = bufs[ctl->nxt[B_U]];
-        up = bufs[ctl->nxt[B_UP]];
-        gp = bufs[ctl->nxt[B_GP]];
-        grad = bufs[ctl->nxt[B_G]];
+        u = pickBuf(bufs, ctl->nxt[B_U]);
+        up = pickBuf(bufs, ctl->nxt[B_UP]);
+        gp = pickBuf(bufs, ctl->nxt[B_GP]);
+        grad = pickBuf(bufs, ctl->nxt[B_G]);
     }
     const int k = vb * kBlock + threadIdx.x;
     const int n = d.nOpt;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     if (k < n) {
-        // every load that does not depend on another is issued first
-        const int i = d.optIdx[k];
+        const int i = optNode(d, k);
         const double u0 = u[k], u1 = u[n + k];
         const double up0 = up ? static_cast<double>(up[k]) : 0.0;
         const double up1 = up ? static_cast<double>(up[n + k]) : 0.0;
@@ -464,48 +474,71 @@ __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g,
         const bool hasWl = i < d.numMov;
         const int t0 = hasWl ? d.nodeStart[i] : 0, t1 = hasWl ? d.nodeStart[i + 1] : 0;
         const T xi = pi.x, yi = pi.y;
-        // WA gradient: the node's pin entries, four loads in flight at a time
-        T gx = 0, gy = 0;
-        for (int t = t0; t < t1; t += 4) {
-            typename Vec2<T>::type pg[4];
+        // the footprint's first four bins (x-major order), loads issued now
+        const bool dens = !(f & (F_FIXED | F_NI));
+        Footprint<T> fp{};
+        int nby = 1, nbin = 0;
+        if (dens) {
+            fp = footprintOf<T>(xi, yi, wi, hi, f, g);
+            nby = fp.by1 - fp.by0 + 1;
+            nbin = (fp.bx1 - fp.bx0 + 1) * nby;
+        }
+        T q[4], fxv[4], fyv[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) {
+            q[j] = T(0);
+            if (j < nbin) {
+                const int bx = fp.bx0 + j / nby, by = fp.by0 + j % nby;
+                q[j] = binCharge<T>(bx, by, fp, g);
+                if (q[j] > T(0)) {
+                    const auto fe = load2(exy, static_cast<long>(bx) * g.ld + by);
+                    fxv[j] = fe.x;
+                    fyv[j] = fe.y;
+                }
+            }
+        }
+        // WA gradient: the node's pin entries, eight loads in flight at a time
+        T gx = 0, gy = 0;
+        for (int t = t0; t < t1; t += 8) {
+            typename Vec2<T>::type pg[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
                 if (t + j < t1) pg[j] = load2(d.pinGrad, t + j);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 8; ++j)
                 if (t + j < t1) {
                     gx += pg[j].x;
                     gy += pg[j].y;
                 }
         }
-        // density force over the footprint (bins in x-major order)
+        // density force over the footprint
         T fx = 0, fy = 0;
-        if (!(f & (F_FIXED | F_NI))) {
-            const Footprint<T> fp = footprintOf<T>(xi, yi, wi, hi, f, g);
-            const int nby = fp.by1 - fp.by0 + 1;
-            const int nbin = (fp.bx1 - fp.bx0 + 1) * nby;
-            for (int e0 = 0; e0 < nbin; e0 += 4) {
-                T q[4], fxv[4], fyv[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    q[j] = T(0);
-                    if (e0 + j < nbin) {
-                        const int bx = fp.bx0 + (e0 + j) / nby, by = fp.by0 + (e0 + j) % nby;
-                        q[j] = binCharge<T>(bx, by, fp, g);
-                        if (q[j] > T(0)) {
-                            const auto fe = load2(exy, static_cast<long>(bx) * g.ld + by);
-                            fxv[j] = fe.x;
-                            fyv[j] = fe.y;
-                        }
+        for (int j = 0; j < 4; ++j)
+            if (q[j] > T(0)) {
+                fx += q[j] * fxv[j];
+                fy += q[j] * fyv[j];
+            }
+        for (int e0 = 4; e0 < nbin; e0 += 4) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                q[j] = T(0);
+                if (e0 + j < nbin) {
+                    const int bx = fp.bx0 + (e0 + j) / nby, by = fp.by0 + (e0 + j) % nby;
+                    q[j] = binCharge<T>(bx, by, fp, g);
+                    if (q[j] > T(0)) {
+                        const auto fe = load2(exy, static_cast<long>(bx) * g.ld + by);
+                        fxv[j] = fe.x;
+                        fyv[j] = fe.y;
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (q[j] > T(0)) {
-                        fx += q[j] * fxv[j];
-                        fy += q[j] * fyv[j];
-                    }
             }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (q[j] > T(0)) {
+                    fx += q[j] * fxv[j];
+                    fy += q[j] * fyv[j];
+                }
         }
         const T lam = static_cast<T>(lambda);
         const T precond = static_cast<T>(1.0 / fmax(1.0, pdeg + static_cast<double>(lambda) * parea));
@@ -534,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
                                                       T invG, float lambda,
                                                       const T* u, const T* up, const T* gp, T* grad,
                                                       T* dbgForce, T* dbgWl, double* partials,
-                                                      const Ctl* ctl, T* const* bufs) {
+                                                      const Ctl* ctl, BufTable<T> bufs) {
     nodeGradBody<T>(d, g, exy, invG, lambda, u, up, gp, grad, dbgForce, dbgWl, partials, ctl,
                     bufs, blockIdx.x);
 }
@@ -795,8 +828,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
 // :156-165) with EPlace's clamp (eplace.cpp:117-133) and scatter (:97-105).
 template <typename T>
 __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G, const T* B,
-                                  T alpha, T m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs,
-                                  int vb) {
+                                  T alpha, T m, T* vOut, T* uOut, const Ctl* ctl,
+                                  const BufTable<T>& bufs, int vb) {
     const int k = vb * kBlock + threadIdx.x;
     const int n = d.nOpt;
     if (ctl) {
@@ -805,8 +838,8 @@ __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G,
         const int mode = ctl->mode;
         if (mode == kModeNone) return;
         if (k < n && ctl->snapWrite >= 0) {   // tau-ladder snapshot of the evaluated point
-            const T* src = bufs[ctl->snapSrc];
-            T* dst = bufs[ctl->snapWrite];
+            const T* src = pickBuf(bufs, ctl->snapSrc);
+            T* dst = pickBuf(bufs, ctl->snapWrite);
             dst[k] = src[k];
             dst[n + k] = src[n + k];
         }
@@ -814,15 +847,15 @@ __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G,
         m = static_cast<T>(ctl->m);
         const int* c = ctl->cur;
         if (mode == kModeStep) {
-            A = bufs[c[B_U]]; G = bufs[c[B_G]]; B = bufs[c[B_V]];
-            vOut = bufs[c[B_VN]]; uOut = bufs[c[B_UN]];
+            A = pickBuf(bufs, c[B_U]); G = pickBuf(bufs, c[B_G]); B = pickBuf(bufs, c[B_V]);
+            vOut = pickBuf(bufs, c[B_VN]); uOut = pickBuf(bufs, c[B_UN]);
         } else {   // redo the previous step from u_prev (nesterov.cpp:156-165)
-            A = bufs[c[B_UP]]; G = bufs[c[B_GP]]; B = bufs[c[B_VP]];
-            vOut = bufs[c[B_V]]; uOut = bufs[c[B_U]];
+            A = pickBuf(bufs, c[B_UP]); G = pickBuf(bufs, c[B_GP]); B = pickBuf(bufs, c[B_VP]);
+            vOut = pickBuf(bufs, c[B_V]); uOut = pickBuf(bufs, c[B_U]);
         }
     }
     if (k >= n) return;
-    const int i = d.optIdx[k];
+    const int i = optNode(d, k);
     const T hw = T(0.5) * d.w[i], hh = T(0.5) * d.h[i];
     const T eps = static_cast<T>(1e-3f);
     const T xlo = d.coreLx + hw + eps;
@@ -850,7 +883,7 @@ __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
                                                    const T* B, T alpha, T m, T* vOut, T* uOut,
-                                                   const Ctl* ctl, T* const* bufs) {
+                                                   const Ctl* ctl, BufTable<T> bufs) {
     updateBody<T>(d, A, G, B, alpha, m, vOut, uOut, ctl, bufs, blockIdx.x);
 }
 
@@ -992,7 +1025,7 @@ void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& w
 template <typename T>
 void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                     float gamma, float lambda, const T* u, const T* up, const T* gp, T* grad,
-                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, T* const* bufs,
+                    T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, const BufTable<T>& bufs,
                     hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
     ws.blocks[3] = static_cast<int>(gridFor(d.nOpt));
@@ -1010,7 +1043,8 @@ void launchReduce(ReduceWs& ws, double denom, bool withNode, Ctl* ctl, DevRow* r
 
 template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
-                  double m, T* vOut, T* uOut, const Ctl* ctl, T* const* bufs, hipStream_t s) {
+                  double m, T* vOut, T* uOut, const Ctl* ctl, const BufTable<T>& bufs,
+                  hipStream_t s) {
     launchKernel(k_update<T>, gridFor(d.nOpt), kBlock, 0, s, d, A, G, B, static_cast<T>(alpha),
                                                    static_cast<T>(m), vOut, uOut, ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
@@ -1031,10 +1065,10 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
                                   hipStream_t);                                                  \
     template void launchNodeGrad<T>(const DevDesign<T>&, const GridGeom<T>&, const T*,           \
                                     float, float, const T*, const T*, const T*, T*, T*, T*,      \
-                                    ReduceWs&, const Ctl*, T* const*, hipStream_t);              \
+                                    ReduceWs&, const Ctl*, const BufTable<T>&, hipStream_t);     \
     template void launchReduce<T>(ReduceWs&, double, bool, Ctl*, DevRow*, hipStream_t);         \
     template void launchUpdate<T>(const DevDesign<T>&, const T*, const T*, const T*, double,     \
-                                  double, T*, T*, const Ctl*, T* const*, hipStream_t);
+                                  double, T*, T*, const Ctl*, const BufTable<T>&, hipStream_t);
 SPHIP_INST(float)
 SPHIP_INST(double)
 template void launchSlotBatch<float>(int, const SlotArgs<float>*, const int4*, int, int,

@@ -126,26 +126,27 @@ __device__ inline double waOf(T mxX, T mnX, T mxY, T mnY, const WaSums<T>& s) {
 
 // Size class with W = 2^LW lanes per net: one pin per lane, so a pin is
 // loaded once and its exponentials are computed once, for both the net's
-// sums and the pin's gradient.  Lanes past the degree carry neutral values.
+// sums and the pin's gradient.  The class's pins sit in lane order (entry
+// 64 gw + lane of wavefront task gw; node -1 marks an empty lane), so the
+// lane's node, offsets and node-pin entry are one coalesced load with no
+// per-net index before it.  Empty lanes carry neutral values.
 template <int LW, typename T>
-__device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lane, T invG,
+__device__ inline void waGroupPass(const DevDesign<T>& d, int gw, int lane, T invG,
                                    int ignoreDeg, double (&v)[2]) {
     constexpr int W = 1 << LW;
-    const int net = d.waNetLo[c] + ((gw - d.waWaveBase[c]) << (6 - LW)) + (lane >> LW);
+    const long e = static_cast<long>(gw) * 64 + lane;
     const int j = lane & (W - 1);
-    int t = 0, deg = 0;
-    if (net < d.waNetLo[c + 1]) {
-        t = d.netStart[net];
-        deg = d.netStart[net + 1] - t;
-        t += j;
-    }
-    const bool pin = j < deg;
+    const int node = d.laneNode[e];
+    const T ox = d.laneOffX[e], oy = d.laneOffY[e];
+    const int q = d.laneNp[e];
+    const bool pin = node >= 0;
     T px = 0, py = 0;
     if (pin) {
-        const auto p = load2(d.xy, d.slotNode[t]);
-        px = p.x + d.slotOffX[t];
-        py = p.y + d.slotOffY[t];
+        const auto p = load2(d.xy, node);
+        px = p.x + ox;
+        py = p.y + oy;
     }
+    const int deg = groupSum<W>(pin ? 1 : 0);
     const T mxX = groupMax<W>(pin ? px : -tmax<T>()), mnX = groupMin<W>(pin ? px : tmax<T>());
     const T mxY = groupMax<W>(pin ? py : -tmax<T>()), mnY = groupMin<W>(pin ? py : tmax<T>());
     const T dxp = px - mxX, dxn = px - mnX, dyp = py - mxY, dyn = py - mnY;
@@ -164,21 +165,16 @@ __device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lan
     if (wa && pin) {
         T gx, gy;
         pinGradOf<T>(s, dxp, dxn, ax, anx, dyp, dyn, ay, any, invG, gx, gy);
-        const int q = d.slotNp[t];
-        d.pinGrad[2L * q] = gx;
-        d.pinGrad[2L * q + 1] = gy;
+        store2(d.pinGrad, q, gx, gy);
     }
 }
 
-// Nets above the largest class (the power-law tail): one wavefront per net,
-// lane-strided pins, shuffle trees; the gradient loop recomputes the pins'
-// exponentials.
+// A tail net of more than kWaTailRegs * 64 pins: lane-strided loops over
+// its slots (extrema, sums, gradients), the exponentials recomputed in the
+// gradient loop.
 template <typename T>
-__device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T invG, int ignoreDeg,
-                                  double (&v)[2]) {
-    const int net = d.waNetLo[kWaClasses] + (gw - d.waWaveBase[kWaClasses]);
-    if (net >= d.waNetLo[kWaClasses + 1]) return;
-    const int b = d.netStart[net], e = d.netStart[net + 1];
+__device__ inline void waWaveLoop(const DevDesign<T>& d, int b, int e, int lane, T invG,
+                                  int ignoreDeg, double (&v)[2]) {
     T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
     for (int t = b + lane; t < e; t += 64) {
         const auto p = load2(d.xy, d.slotNode[t]);
@@ -211,9 +207,109 @@ __device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T inv
         T gx, gy;
         pinGradOf<T>(s, dxp, dxn, ax, anx, dyp, dyn, ay, any, invG, gx, gy);
         const int q = d.slotNp[t];
-        d.pinGrad[2L * q] = gx;
-        d.pinGrad[2L * q + 1] = gy;
+        store2(d.pinGrad, q, gx, gy);
     }
+}
+
+// Nets above the largest class (the power-law tail): one wavefront per net,
+// lane l taking the net's pins l, l + 64, ...  The slot loads of a batch of
+// pins are issued together, then its coordinate gathers together, so a batch
+// costs two memory round trips:
+//  - a net above ignoreDeg needs only its extrema (HPWL): batches of
+//    kWaTailRegs pins per lane, folded into the extrema as they arrive;
+//  - a net of at most 2 x 64 pins (within ignoreDeg) keeps its pins in
+//    registers for the extrema, the sums and the gradients;
+//  - larger nets within ignoreDeg run waWaveLoop.
+// Extrema do not depend on order, and the sums run over a lane's pins in the
+// order of waWaveLoop's loops, so every path gives waWaveLoop's bits.
+template <typename T>
+__device__ inline void waWavePass(const DevDesign<T>& d, int gw, int lane, T invG, int ignoreDeg,
+                                  double (&v)[2]) {
+    const int k = gw - d.waWaveBase[kWaClasses];
+    if (d.waNetLo[kWaClasses] + k >= d.waNetLo[kWaClasses + 1]) return;
+    const int b = d.tailStart[k], e = d.tailStart[k + 1];
+    if (e - b > ignoreDeg) {
+        T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+        constexpr int R = kWaTailRegs;
+        for (int t0 = b + lane; t0 < e; t0 += 64 * R) {
+            int nd[R];
+            T ox[R], oy[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int t = t0 + 64 * r;
+                const bool in = t < e;
+                nd[r] = in ? d.slotNode[t] : -1;
+                ox[r] = in ? d.slotOffX[t] : T(0);
+                oy[r] = in ? d.slotOffY[t] : T(0);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (nd[r] >= 0) {
+                    const auto p = load2(d.xy, nd[r]);
+                    const T px = p.x + ox[r], py = p.y + oy[r];
+                    mxX = fmax(mxX, px); mnX = fmin(mnX, px);
+                    mxY = fmax(mxY, py); mnY = fmin(mnY, py);
+                }
+        }
+        mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
+        if (lane == 0) v[1] += static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+        return;
+    }
+    if (e - b > 128) {
+        waWaveLoop<T>(d, b, e, lane, invG, ignoreDeg, v);
+        return;
+    }
+    constexpr int R = 2;
+    int nd[R], q[R];
+    T px[R], py[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int t = b + lane + 64 * r;
+        const bool in = t < e;
+        nd[r] = in ? d.slotNode[t] : -1;
+        px[r] = in ? d.slotOffX[t] : T(0);
+        py[r] = in ? d.slotOffY[t] : T(0);
+        q[r] = in ? d.slotNp[t] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (nd[r] >= 0) {
+            const auto p = load2(d.xy, nd[r]);
+            px[r] = p.x + px[r];
+            py[r] = p.y + py[r];
+        }
+    T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (nd[r] >= 0) {
+            mxX = fmax(mxX, px[r]); mnX = fmin(mnX, px[r]);
+            mxY = fmax(mxY, py[r]); mnY = fmin(mnY, py[r]);
+        }
+    mxX = waveMax(mxX); mnX = waveMin(mnX); mxY = waveMax(mxY); mnY = waveMin(mnY);
+    if (lane == 0) v[1] += static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+    WaSums<T> s{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (nd[r] >= 0) {
+            const T dxp = px[r] - mxX, dxn = px[r] - mnX, dyp = py[r] - mxY, dyn = py[r] - mnY;
+            const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
+            const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
+            s.bpx += ax; s.cpx += dxp * ax; s.bnx += anx; s.cnx += dxn * anx;
+            s.bpy += ay; s.cpy += dyp * ay; s.bny += any; s.cny += dyn * any;
+        }
+    s.bpx = waveSum(s.bpx); s.cpx = waveSum(s.cpx); s.bnx = waveSum(s.bnx); s.cnx = waveSum(s.cnx);
+    s.bpy = waveSum(s.bpy); s.cpy = waveSum(s.cpy); s.bny = waveSum(s.bny); s.cny = waveSum(s.cny);
+    if (lane == 0) v[0] += waOf<T>(mxX, mnX, mxY, mnY, s);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (nd[r] >= 0) {
+            const T dxp = px[r] - mxX, dxn = px[r] - mnX, dyp = py[r] - mxY, dyn = py[r] - mnY;
+            const T ax = texp<T>(dxp * invG), anx = texp<T>(-dxn * invG);
+            const T ay = texp<T>(dyp * invG), any = texp<T>(-dyn * invG);
+            T gx, gy;
+            pinGradOf<T>(s, dxp, dxn, ax, anx, dyp, dyn, ay, any, invG, gx, gy);
+            store2(d.pinGrad, q[r], gx, gy);
+        }
 }
 
 
@@ -240,18 +336,21 @@ __device__ inline void waBlockTask(const WaTask<T>& w, int block, int nblocks) {
     double v[2] = {0.0, 0.0};
     // wavefront tasks are strided over the blocks (a bounded number of block
     // partials); each task belongs to one size class
+    // tasks are taken from the end (the tail nets and the largest classes
+    // first), so the longest tasks start in the first round
     const int tasks = d.waWaveBase[kWaClasses + 1];
-    for (int gw = block * (kBlock / 64) + (threadIdx.x >> 6); gw < tasks;
-         gw += nblocks * (kBlock / 64)) {
+    for (int q = block * (kBlock / 64) + (threadIdx.x >> 6); q < tasks;
+         q += nblocks * (kBlock / 64)) {
+        const int gw = tasks - 1 - q;
         int c = 0;
         while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
         switch (c) {
-            case 0: waGroupPass<1, T>(d, 0, gw, lane, invG, ignoreDeg, v); break;
-            case 1: waGroupPass<2, T>(d, 1, gw, lane, invG, ignoreDeg, v); break;
-            case 2: waGroupPass<3, T>(d, 2, gw, lane, invG, ignoreDeg, v); break;
-            case 3: waGroupPass<4, T>(d, 3, gw, lane, invG, ignoreDeg, v); break;
-            case 4: waGroupPass<5, T>(d, 4, gw, lane, invG, ignoreDeg, v); break;
-            case 5: waGroupPass<6, T>(d, 5, gw, lane, invG, ignoreDeg, v); break;
+            case 0: waGroupPass<1, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 1: waGroupPass<2, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 2: waGroupPass<3, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 3: waGroupPass<4, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 4: waGroupPass<5, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 5: waGroupPass<6, T>(d, gw, lane, invG, ignoreDeg, v); break;
             case kWaClasses: waWavePass<T>(d, gw, lane, invG, ignoreDeg, v); break;
             default: break;
         }
