@@ -534,13 +534,25 @@ __global__ __launch_bounds__(kRhoThreads) void k_dct_rows_rho(DensityIn<T> in, T
         clearTileKeys(in.tg, t);
     const int r0 = 2 * blockIdx.x;
     double over[kRhoRounds] = {0.0, 0.0, 0.0, 0.0};
+    // rounds in pairs: both rounds' loads in flight before the first
+    // clearing store
 #pragma unroll
-    for (int j = 0; j < kRhoRounds; ++j) {
-        if (j < rounds) {
-            const int e = j * kRhoThreads + threadIdx.x;
-            const int row = e / L, i = e - row * L;
-            const T val = finalizeBin<T>(in, r0 * L + e, &over[j]);
-            reinterpret_cast<T*>(a[0] + dctPos(i, L))[row] = val;
+    for (int j0 = 0; j0 < kRhoRounds; j0 += 2) {
+        if (j0 >= rounds) break;
+        BinLoads bl[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (j0 + h < rounds)
+                finalizeLoad<T>(in, r0 * L + (j0 + h) * kRhoThreads + threadIdx.x, bl[h]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int j = j0 + h;
+            if (j < rounds) {
+                const int e = j * kRhoThreads + threadIdx.x;
+                const int row = e / L, i = e - row * L;
+                const T val = finalizeFinish<T>(in, r0 * L + e, bl[h], &over[j]);
+                reinterpret_cast<T*>(a[0] + dctPos(i, L))[row] = val;
+            }
         }
     }
     __syncthreads();
@@ -717,8 +729,11 @@ __device__ inline int xcdBlock(int b, int n) {
     return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
 }
 
-template <typename T>
-__global__ __launch_bounds__(1024) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
+// MAXT: the largest block the instance is launched with (G teams of
+// fftThreads(nx) threads); blocks of at most 256 threads take the 256-thread
+// instance, whose register budget holds the transforms without spilling.
+template <typename T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
                                                    T* __restrict__ Hs, int G, int nx, int ny,
                                                    int ld, FftTab<T> tab, const T* __restrict__ u,
                                                    const T* __restrict__ v, T invNorm,
@@ -770,11 +785,12 @@ __device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restri
     cplx<T>* b = a + L;
     cplx<T>* c = phi ? b + L : nullptr;
     cplx<T>* d = phi ? c + L : nullptr;
-    loadTables<T>(f, tab.tw, tab.hf, v, L);
     const int x = block * teams + team;
-    // the row pair (Hs[x][j], Hc[x][j]) into b
+    // the row pair (Hs[x][j], Hc[x][j]) into b; the first batch's loads are
+    // issued ahead of the table loads
     const T* hs = Hs + static_cast<long>(x) * ld;
     const T* hc = Hc + static_cast<long>(x) * ld;
+    bool tables = false;
     for (int i0 = tid; i0 < L; i0 += nt * kFftBatch) {
         T vsv[kFftBatch], vcv[kFftBatch];
 #pragma unroll
@@ -783,12 +799,17 @@ __device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restri
             vsv[q] = i < L ? hs[i] : T(0);
             vcv[q] = i < L ? hc[i] : T(0);
         }
+        if (!tables) {
+            loadTables<T>(f, tab.tw, tab.hf, v, L);
+            tables = true;
+        }
 #pragma unroll
         for (int q = 0; q < kFftBatch; ++q) {
             const int i = i0 + nt * q;
             if (i < L) b[i] = cmk<T>(vsv[q], vcv[q]);
         }
     }
+    if (!tables) loadTables<T>(f, tab.tw, tab.hf, v, L);
     __syncthreads();
     for (int j = tid; j <= L / 2; j += nt) {
         const int jm = (L - j) & (L - 1);
@@ -1050,7 +1071,8 @@ void PoissonPlan<T>::setFftLds() {
     }
     raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows<T>), ldsA_);
     raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows_rho<T>), ldsA_);
-    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T>), ldsB_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T, 256>), ldsB_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T, 1024>), ldsB_);
     raiseLdsLimit(reinterpret_cast<const void*>(k_idct_rows<T>), ldsC_);
 }
 
@@ -1061,7 +1083,8 @@ bool PoissonPlan<T>::setColumnPairs(int pairs) {
     if (lds > 150 * 1024 || pairs * fftThreads(nx_) > 1024) return false;
     colPairs_ = pairs;
     ldsB_ = lds;
-    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T>), ldsB_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T, 256>), ldsB_);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<T, 1024>), ldsB_);
     return true;
 }
 
@@ -1078,9 +1101,10 @@ void PoissonPlan<T>::passBC(const int* skip, bool withPhi) {
     const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
     const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
     const int ld = my_;
-    launchKernel(k_dct_cols<T>, dim3(ny_ / (2 * colPairs_)), dim3(colPairs_ * fftThreads(nx_)),
-                 static_cast<uint32_t>(ldsB_), stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx,
-                 u_, v_, invNorm_, skip);
+    const int colThreads = colPairs_ * fftThreads(nx_);
+    launchKernel(colThreads <= 256 ? k_dct_cols<T, 256> : k_dct_cols<T, 1024>,
+                 dim3(ny_ / (2 * colPairs_)), dim3(colThreads), static_cast<uint32_t>(ldsB_),
+                 stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx, u_, v_, invNorm_, skip);
     SPHIP_CHECK(hipGetLastError());
     T* fx = interleaved_ ? exy_ : ex_;
     T* fy = interleaved_ ? exy_ + 1 : ey_;
@@ -1240,12 +1264,12 @@ void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, fl
     const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
     const size_t ldsC = (kFftLdsHead(ny) + 4 * static_cast<size_t>(ny)) * c;
     raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows<float>), ldsA);
-    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<float>), ldsB);
+    raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<float, 256>), ldsB);
     raiseLdsLimit(reinterpret_cast<const void*>(k_idct_rows<float>), ldsC);
     const float invNorm = static_cast<float>(1.0 / (4.0 * nx * ny));
     launchKernel(k_dct_rows<float>, dim3(nx / 2), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsA),
                  s, rho, t0, nx, ny, ny, ty, static_cast<const int*>(nullptr));
-    launchKernel(k_dct_cols<float>, dim3(ny / 2), dim3(fftThreads(nx)), static_cast<uint32_t>(ldsB),
+    launchKernel(k_dct_cols<float, 256>, dim3(ny / 2), dim3(fftThreads(nx)), static_cast<uint32_t>(ldsB),
                  s, t0, t1, t2, 1, nx, ny, ny, tx, static_cast<const float*>(u),
                  static_cast<const float*>(v), invNorm, static_cast<const int*>(nullptr));
     launchKernel(k_idct_rows<float>, dim3(nx), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsC),
