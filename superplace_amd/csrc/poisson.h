@@ -39,10 +39,9 @@ struct PoissonBatchArgs {
     T* t1;
     T* t2;
     T* exy;             // interleaved field out
-    const T* tabX;      // per axis: twiddles, then half-angle factors (complex)
-    const T* tabY;
-    const T* u;
-    const T* v;
+    const double* tabX;   // per axis: the FP64 twiddle factor tables (complex)
+    const double* tabY;
+    double stepX, stepY;   // bin steps (the spectral scales are computed from them)
     T invNorm;
     const int* skip;    // the instance's done flag
 };
@@ -112,10 +111,11 @@ private:
       *ax_ = nullptr;
     T *u_ = nullptr, *v_ = nullptr;
     T invNorm_;
-    // FFT path: per axis a table of L twiddles then L half-angle factors
-    // (complex, interleaved); LDS bytes of the three passes
+    double stepX_, stepY_;
+    // FFT path: per axis the twiddle factor tables (complex, interleaved);
+    // LDS bytes of the three passes
     bool fft_ = false;
-    T *tabX_ = nullptr, *tabY_ = nullptr;
+    double *tabX_ = nullptr, *tabY_ = nullptr;
     size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0, ldsBW_ = 0;
     int colPairs_ = 1;   // column pairs per block of the column pass
     void setFftLds();

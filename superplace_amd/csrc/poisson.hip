@@ -285,11 +285,39 @@ __device__ inline void dft8(cplx<T>* v) {
     v[4] = u2; v[5] = d2; v[6] = u3; v[7] = d3;
 }
 
+// exp(-2 pi i t / 4L), t < 4L, as the product of two small tables:
+// lo[t mod 2^S] hi[t >> S] with S = ceil(log2(4L) / 2), together at most 128
+// entries for L <= 1024 (a full table of L twiddles and L half-angle
+// factors read by every block of a pass made its loads the slowest phase
+// of the pass).  The FFT twiddle exp(-2 pi i t / L) is entry 4t, the
+// half-angle factor exp(-i pi k / 2L) of the cosine transforms entry k.
+// The tables hold FP64 values and the product is taken in FP64, so an FP32
+// factor is the FP64 value rounded once, as a full FP32 table holds it.
+__host__ __device__ constexpr int twShift(int L) {
+    int m = 0;
+    while ((1 << m) < 4 * L) ++m;
+    return (m + 1) / 2;
+}
+__host__ __device__ constexpr int twLo(int L) { return 1 << twShift(L); }
+__host__ __device__ constexpr int twCount(int L) { return twLo(L) + ((4 * L) >> twShift(L)); }
+
+template <typename T>
+struct Twid {
+    const cplx<double>* fac;   // lo (nLo entries), then hi
+    int s, nLo;
+    __device__ cplx<T> at(int t) const {
+        const cplx<double> w = cmul<double>(fac[nLo + (t >> s)], fac[t & (nLo - 1)]);
+        return cmk<T>(static_cast<T>(w.x), static_cast<T>(w.y));
+    }
+    __device__ cplx<T> tw(int t) const { return at(4 * t); }
+    __device__ cplx<T> hf(int k) const { return at(k); }
+};
+
 // One Stockham stage of radix R for K transforms at once, butterflies spread
 // over the block's threads: butterfly j reads a[j + r L/R], turns input r by
 // tw[r (j mod ns) L/(ns R)] and writes b[(j - j mod ns) R + j mod ns + r ns].
 template <typename T, bool INV, int R, int K>
-__device__ inline void fftStage(cplx<T>* const* a, cplx<T>* const* b, const cplx<T>* tw, int L,
+__device__ inline void fftStage(cplx<T>* const* a, cplx<T>* const* b, const Twid<T>& tw, int L,
                                 int ns, int tid, int nt) {
     const int nb = L / R, st = L / (ns * R);
     for (int j = tid; j < nb; j += nt) {
@@ -302,7 +330,7 @@ __device__ inline void fftStage(cplx<T>* const* a, cplx<T>* const* b, const cplx
         if (jm) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cplx<T> w = tw[r * jm * st];
+                cplx<T> w = tw.tw(r * jm * st);
                 if (INV) w = cconj<T>(w);
 #pragma unroll
                 for (int k = 0; k < K; ++k) v[k][r] = cmul<T>(v[k][r], w);
@@ -338,7 +366,7 @@ __device__ inline void fftStage(cplx<T>* const* a, cplx<T>* const* b, const cplx
 // (exp(+2 pi i nk/L)).  Stages end in block barriers, so every team of the
 // block runs the same L.
 template <typename T, bool INV, int K>
-__device__ void blockFft(cplx<T>** a, cplx<T>** b, const cplx<T>* tw, int L, int tid, int nt) {
+__device__ void blockFft(cplx<T>** a, cplx<T>** b, const Twid<T>& tw, int L, int tid, int nt) {
     for (int ns = 1; ns < L;) {
         const int q = L / ns;
         int R;
@@ -359,50 +387,45 @@ __device__ void blockFft(cplx<T>** a, cplx<T>** b, const cplx<T>* tw, int L, int
     }
 }
 
-// LDS layout of the FFT passes: the twiddle table (L entries
-// exp(-2 pi i t/L)), the half-angle table (L entries exp(-i pi k/2L)), the
-// pass's spectral scale (u or v, L reals), then the waves' buffers.
+// LDS layout of the FFT passes: the twiddle factor tables (Twid), then the
+// waves' buffers.
 template <typename T>
 struct FftLds {
-    cplx<T>* tw;
-    cplx<T>* hf;
-    T* sc;
+    cplx<double>* fac;
     cplx<T>* waves;
+    __device__ Twid<T> twid(int L) const { return Twid<T>{fac, twShift(L), twLo(L)}; }
 };
+// Complex T entries of LDS ahead of the waves' buffers: the FP64 factor tables.
+template <typename T>
+constexpr int kFftLdsHead(int L) {
+    return twCount(L) * static_cast<int>(sizeof(cplx<double>) / sizeof(cplx<T>));
+}
 template <typename T>
 __device__ inline FftLds<T> fftLds(unsigned char* smem, int L) {
     FftLds<T> f;
-    f.tw = reinterpret_cast<cplx<T>*>(smem);
-    f.hf = f.tw + L;
-    f.sc = reinterpret_cast<T*>(f.hf + L);
-    f.waves = f.hf + L + L / 2;
+    f.fac = reinterpret_cast<cplx<double>*>(smem);
+    f.waves = reinterpret_cast<cplx<T>*>(smem) + kFftLdsHead<T>(L);
     return f;
 }
-// Complex entries of LDS ahead of the waves' buffers.
-constexpr int kFftLdsHead(int L) { return 2 * L + L / 2; }
 
-// Copies the tables (and the scale array, when given) into LDS; the caller
+// Copies the factor tables (at most 128 entries) into LDS; the caller
 // synchronises before use.
 template <typename T>
-__device__ inline void loadTables(const FftLds<T>& f, const cplx<T>* gtw, const cplx<T>* ghf,
-                                  const T* gsc, int L) {
-    for (int t = threadIdx.x; t < L; t += blockDim.x) {
-        f.tw[t] = gtw[t];
-        f.hf[t] = ghf[t];
-        if (gsc) f.sc[t] = gsc[t];
-    }
+__device__ inline void loadTables(const FftLds<T>& f, const cplx<double>* gfac, int L) {
+    const int n = twCount(L);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) f.fac[t] = gfac[t];
 }
 
 // REDFT10 outputs for the pair (k, L-k) from the transform Z of two packed
 // lines: X_a, X_b at k and (when 0 < k < L/2) at L-k.
 template <typename T>
-__device__ inline void dct2Pair(const cplx<T>* Z, const cplx<T>* hf, int k, int L, T* xa, T* xb,
+__device__ inline void dct2Pair(const cplx<T>* Z, const Twid<T>& tw, int k, int L, T* xa, T* xb,
                                 T* xaM, T* xbM) {
     const cplx<T> zk = Z[k], zm = cconj<T>(Z[(L - k) & (L - 1)]);
     const cplx<T> va = cmk<T>(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y + zm.y));
     const cplx<T> dv = csub<T>(zk, zm);
     const cplx<T> vb = cmk<T>(T(0.5) * dv.y, T(-0.5) * dv.x);   // -i (Z[k] - conj Z[L-k]) / 2
-    const cplx<T> h = hf[k];
+    const cplx<T> h = tw.hf(k);
     *xa = T(2) * (h.x * va.x - h.y * va.y);
     *xb = T(2) * (h.x * vb.x - h.y * vb.y);
     // at L-k: V[L-k] = conj(V[k]) and exp(-i pi (L-k)/2L) = -i conj(h)
@@ -421,11 +444,30 @@ __device__ inline void dct3InPair(T aj, T aM, cplx<T> h, cplx<T>* zj, cplx<T>* z
     *zM = cmul<T>(cmk<T>(aM, -aj), gm);
 }
 
+// An axis: its FP64 factor tables in device memory (lo, then hi; see
+// Twid), its length and bin step.
 template <typename T>
 struct FftTab {
-    const cplx<T>* tw;   // exp(-2 pi i t / L), t < L
-    const cplx<T>* hf;   // exp(-i pi k / 2L), k < L
+    const cplx<double>* fac;
+    int n;
+    double step;
 };
+
+// The spectral scale of frequency j along an axis, u_j = pi j / (n step)
+// (cpu_poisson.cpp:163-173), computed where it is used, in the arithmetic
+// of the plan's scale tables: the reference's float arithmetic in FP32,
+// double in FP64.
+template <typename T>
+__device__ inline T specScale(int j, const FftTab<T>& a);
+template <>
+__device__ inline float specScale<float>(int j, const FftTab<float>& a) {
+    return static_cast<float>(3.14159265358979323846f) * static_cast<float>(j) /
+           (static_cast<float>(a.n) * static_cast<float>(a.step));
+}
+template <>
+__device__ inline double specScale<double>(int j, const FftTab<double>& a) {
+    return 3.14159265358979323846 * j / (a.n * a.step);
+}
 
 // Global loads are issued kFftBatch at a time into registers before their
 // LDS stores, so their latencies overlap.
@@ -447,7 +489,8 @@ __device__ inline void dctRowsBlock(const T* __restrict__ rho, T* __restrict__ o
     const FftLds<T> f = fftLds<T>(smem, L);
     cplx<T>* a[1] = {f.waves + static_cast<long>(team) * 2 * L};
     cplx<T>* b[1] = {a[0] + L};
-    loadTables<T>(f, tab.tw, tab.hf, nullptr, L);
+    loadTables<T>(f, tab.fac, L);
+    const Twid<T> tw = f.twid(L);
     const int r0 = 2 * (block * teams + team);
     const T* xa = rho + static_cast<long>(r0) * ld;
     const T* xb = xa + ld;
@@ -466,13 +509,13 @@ __device__ inline void dctRowsBlock(const T* __restrict__ rho, T* __restrict__ o
         }
     }
     __syncthreads();
-    blockFft<T, false, 1>(a, b, f.tw, L, tid, nt);
+    blockFft<T, false, 1>(a, b, tw, L, tid, nt);
     const cplx<T>* Z = a[0];
     T* oa = out + static_cast<long>(r0) * ld;
     T* ob = oa + ld;
     for (int k = tid; k <= L / 2; k += nt) {
         T ra, rb, raM, rbM;
-        dct2Pair<T>(Z, f.hf, k, L, &ra, &rb, &raM, &rbM);
+        dct2Pair<T>(Z, tw, k, L, &ra, &rb, &raM, &rbM);
         oa[k] = ra;
         ob[k] = rb;
         if (k > 0 && k < L / 2) {
@@ -529,7 +572,8 @@ __global__ __launch_bounds__(kRhoThreads) void k_dct_rows_rho(DensityIn<T> in, T
     const FftLds<T> f = fftLds<T>(smem, L);
     cplx<T>* a[1] = {f.waves};
     cplx<T>* b[1] = {f.waves + L};
-    loadTables<T>(f, tab.tw, tab.hf, nullptr, L);
+    loadTables<T>(f, tab.fac, L);
+    const Twid<T> tw = f.twid(L);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < in.tg.nKeys; t += gridDim.x * blockDim.x)
         clearTileKeys(in.tg, t);
     const int r0 = 2 * blockIdx.x;
@@ -559,13 +603,13 @@ __global__ __launch_bounds__(kRhoThreads) void k_dct_rows_rho(DensityIn<T> in, T
     // threads past the team take no butterflies (their index is past every
     // stage's count) but meet every barrier
     const int tid = static_cast<int>(threadIdx.x) < nt ? static_cast<int>(threadIdx.x) : L;
-    blockFft<T, false, 1>(a, b, f.tw, L, tid, nt);
+    blockFft<T, false, 1>(a, b, tw, L, tid, nt);
     const cplx<T>* Z = a[0];
     T* oa = out + static_cast<long>(r0) * ld;
     T* ob = oa + ld;
     for (int k = tid; k <= L / 2; k += nt) {
         T ra, rb, raM, rbM;
-        dct2Pair<T>(Z, f.hf, k, L, &ra, &rb, &raM, &rbM);
+        dct2Pair<T>(Z, tw, k, L, &ra, &rb, &raM, &rbM);
         oa[k] = ra;
         ob[k] = rb;
         if (k > 0 && k < L / 2) {
@@ -590,33 +634,24 @@ __global__ __launch_bounds__(kBlock) void k_dct_rows(const T* __restrict__ rho, 
 // spectral divide (cpu_poisson.cpp:163-173), then both inverse transforms
 // along x at once (cosine -> Hc, shifted sine of u P -> Hs).  The block
 // loads and stores 2G adjacent values per row.
-// LDS_TABLES: the twiddle, half-angle and u tables are copied into LDS
-// ahead of the teams' buffers; otherwise they are read from global memory
-// (through the caches) and the buffers start at LDS offset 0.
-template <typename T, bool LDS_TABLES>
+// Three line buffers per team: the two inverse transforms
+// run one after the other, sharing the third buffer (two at once need a
+// fourth buffer and, holding both transforms' butterflies, spill registers
+// in a 256-thread block), which also leaves LDS room for the WA blocks that
+// share the fused launch.
+template <typename T>
 __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc,
                                     T* __restrict__ Hs, int G, int nx, int ny, int ld,
-                                    const FftTab<T>& tab, const T* __restrict__ u,
-                                    const T* __restrict__ v, T invNorm, int block) {
+                                    const FftTab<T>& tab, const FftTab<T>& taby, T invNorm,
+                                    int block) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int L = nx, nt = blockDim.x / G;
     const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
-    FftLds<T> f;
-    if (LDS_TABLES) {
-        f = fftLds<T>(smem, L);
-        loadTables<T>(f, tab.tw, tab.hf, u, L);
-    } else {
-        f.tw = const_cast<cplx<T>*>(tab.tw);
-        f.hf = const_cast<cplx<T>*>(tab.hf);
-        f.sc = const_cast<T*>(u);
-        f.waves = reinterpret_cast<cplx<T>*>(smem);
-    }
-    const cplx<T>* hf = f.hf;
-    const T* us = f.sc;
-    // team g owns region[NB L g, NB L (g + 1)): four line buffers, or three
-    // when the tables are not in LDS (the inverse transforms then run one
-    // after the other, sharing the third buffer)
-    constexpr int NB = LDS_TABLES ? 4 : 3;
+    const FftLds<T> f = fftLds<T>(smem, L);
+    loadTables<T>(f, tab.fac, L);
+    const Twid<T> tw = f.twid(L);
+    // team g owns region[NB L g, NB L (g + 1))
+    constexpr int NB = 3;
     cplx<T>* region = f.waves;
     cplx<T>* b0 = region + static_cast<long>(team) * NB * L;
     cplx<T>* b1 = b0 + L;
@@ -643,11 +678,11 @@ __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc
         }
     }
     const int ka = c0 + 2 * team;
-    const T va = v[ka], vb = v[ka + 1];
+    const T va = specScale<T>(ka, taby), vb = specScale<T>(ka + 1, taby);
     __syncthreads();
     cplx<T>* fa[1] = {b0};
     cplx<T>* fb[1] = {b1};
-    blockFft<T, false, 1>(fa, fb, f.tw, L, tid, nt);
+    blockFft<T, false, 1>(fa, fb, tw, L, tid, nt);
     cplx<T>* r = fa[0];
     cplx<T>* o = fb[0];
     // each thread owns the pairs (j, L-j): REDFT10 outputs, spectral divide,
@@ -655,19 +690,19 @@ __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc
     for (int j = tid; j <= L / 2; j += nt) {
         const int jm = (L - j) & (L - 1);
         T ga, gb, gaM, gbM;
-        dct2Pair<T>(r, hf, j, L, &ga, &gb, &gaM, &gbM);
+        dct2Pair<T>(r, tw, j, L, &ga, &gb, &gaM, &gbM);
         if (j == 0 || j == L / 2) {
             gaM = ga;
             gbM = gb;
         }
-        const T uj = us[j], um = us[jm];
+        const T uj = specScale<T>(j, tab), um = specScale<T>(jm, tab);
         auto spec = [&](T g, T uu, T vv) {
             const T den = uu * uu + vv * vv;
             return den > T(0) ? g * invNorm / den : T(0);
         };
         const T pa = spec(ga, uj, va), pb = spec(gb, uj, vb);
         const T paM = j == 0 ? T(0) : spec(gaM, um, va), pbM = j == 0 ? T(0) : spec(gbM, um, vb);
-        const cplx<T> h = hf[j];
+        const cplx<T> h = tw.hf(j);
         cplx<T> za, zaM, zb, zbM;
         dct3InPair<T>(pa, paM, h, &za, &zaM);
         dct3InPair<T>(pb, pbM, h, &zb, &zbM);
@@ -688,22 +723,14 @@ __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc
     __syncthreads();
     const cplx<T>* res1;
     const cplx<T>* res2;
-    if (LDS_TABLES) {
-        cplx<T>* in[2] = {o, r};
-        cplx<T>* sc2[2] = {b2, b2 + L};
-        blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
-        res1 = in[0];
-        res2 = in[1];
-    } else {
-        cplx<T>* i1[1] = {o};
-        cplx<T>* s1[1] = {b2};
-        blockFft<T, true, 1>(i1, s1, f.tw, L, tid, nt);
-        cplx<T>* i2[1] = {r};
-        cplx<T>* s2[1] = {i1[0] == o ? b2 : o};
-        blockFft<T, true, 1>(i2, s2, f.tw, L, tid, nt);
-        res1 = i1[0];
-        res2 = i2[0];
-    }
+    cplx<T>* i1[1] = {o};
+    cplx<T>* s1[1] = {b2};
+    blockFft<T, true, 1>(i1, s1, tw, L, tid, nt);
+    cplx<T>* i2[1] = {r};
+    cplx<T>* s2[1] = {i1[0] == o ? b2 : o};
+    blockFft<T, true, 1>(i2, s2, tw, L, tid, nt);
+    res1 = i1[0];
+    res2 = i2[0];
     // the results sit at the same offsets in every team's region
     const long off1 = res1 - b0, off2 = res2 - b0;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -735,17 +762,16 @@ __device__ inline int xcdBlock(int b, int n) {
 template <typename T, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_dct_cols(const T* __restrict__ F1, T* __restrict__ Hc,
                                                    T* __restrict__ Hs, int G, int nx, int ny,
-                                                   int ld, FftTab<T> tab, const T* __restrict__ u,
-                                                   const T* __restrict__ v, T invNorm,
-                                                   const int* skip) {
+                                                   int ld, FftTab<T> tab, FftTab<T> taby,
+                                                   T invNorm, const int* skip) {
     if (skip && *skip) return;
-    dctColsBlock<T, true>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm,
+    dctColsBlock<T>(F1, Hc, Hs, G, nx, ny, ld, tab, taby, invNorm,
                           xcdBlock(blockIdx.x, gridDim.x));
 }
 
 // Pass B with the WA pass's blocks in the same launch: blocks below
-// colBlocks run the column pass (kBlock threads = G teams, tables read from
-// global memory so that its LDS leaves room for the WA blocks), the rest WA
+// colBlocks run the column pass (kBlock threads = G teams, three line
+// buffers each so that its LDS leaves room for the WA blocks), the rest WA
 // blocks waFirst, waFirst + 1, ... of a WA pass of waTotal blocks.  The WA
 // pass depends only on the positions, so it fills the CUs that the column
 // pass leaves idle.
@@ -753,13 +779,13 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dct_cols_wa(const T* __restrict__ F1,
                                                         T* __restrict__ Hc, T* __restrict__ Hs,
                                                         int G, int nx, int ny, int ld,
-                                                        FftTab<T> tab, const T* __restrict__ u,
-                                                        const T* __restrict__ v, T invNorm,
+                                                        FftTab<T> tab, FftTab<T> taby,
+                                                        T invNorm,
                                                         const int* skip, WaTask<T> wa,
                                                         int colBlocks, int waFirst, int waTotal) {
     if (skip && *skip) return;
     if (static_cast<int>(blockIdx.x) < colBlocks)
-        dctColsBlock<T, false>(F1, Hc, Hs, G, nx, ny, ld, tab, u, v, invNorm,
+        dctColsBlock<T>(F1, Hc, Hs, G, nx, ny, ld, tab, taby, invNorm,
                                xcdBlock(blockIdx.x, colBlocks));
     else
         waBlockTask<T>(wa, waFirst + (blockIdx.x - colBlocks), waTotal);
@@ -774,23 +800,21 @@ template <typename T>
 __device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restrict__ Hs,
                                      T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ phi,
                                      int cstep, int nx, int ny, int ld, const FftTab<T>& tab,
-                                     const T* __restrict__ v, int block) {
+                                     int block) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int L = ny, nt = fftThreads(L), teams = blockDim.x / nt;
     const int team = threadIdx.x / nt, tid = threadIdx.x - team * nt;
     const FftLds<T> f = fftLds<T>(smem, L);
-    const cplx<T>* hf = f.hf;
-    const T* vs = f.sc;
     cplx<T>* a = f.waves + static_cast<long>(team) * (phi ? 4 : 2) * L;
     cplx<T>* b = a + L;
     cplx<T>* c = phi ? b + L : nullptr;
     cplx<T>* d = phi ? c + L : nullptr;
     const int x = block * teams + team;
-    // the row pair (Hs[x][j], Hc[x][j]) into b; the first batch's loads are
-    // issued ahead of the table loads
+    // the row pair (Hs[x][j], Hc[x][j]) into b; the tables load alongside
     const T* hs = Hs + static_cast<long>(x) * ld;
     const T* hc = Hc + static_cast<long>(x) * ld;
-    bool tables = false;
+    loadTables<T>(f, tab.fac, L);
+    const Twid<T> tw = f.twid(L);
     for (int i0 = tid; i0 < L; i0 += nt * kFftBatch) {
         T vsv[kFftBatch], vcv[kFftBatch];
 #pragma unroll
@@ -799,26 +823,22 @@ __device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restri
             vsv[q] = i < L ? hs[i] : T(0);
             vcv[q] = i < L ? hc[i] : T(0);
         }
-        if (!tables) {
-            loadTables<T>(f, tab.tw, tab.hf, v, L);
-            tables = true;
-        }
 #pragma unroll
         for (int q = 0; q < kFftBatch; ++q) {
             const int i = i0 + nt * q;
             if (i < L) b[i] = cmk<T>(vsv[q], vcv[q]);
         }
     }
-    if (!tables) loadTables<T>(f, tab.tw, tab.hf, v, L);
     __syncthreads();
     for (int j = tid; j <= L / 2; j += nt) {
         const int jm = (L - j) & (L - 1);
         const cplx<T> pj = b[j], pm = j == 0 ? cmk<T>(0, 0) : b[jm];
-        const cplx<T> h = hf[j];
+        const cplx<T> h = tw.hf(j);
         cplx<T> za, zaM, zb, zbM;
         dct3InPair<T>(pj.x, pm.x, h, &za, &zaM);
         // shifted sine of v Hc: a2[m] = v[L-m] Hc[L-m] (m >= 1), a2[0] = 0
-        const T bj = j == 0 ? T(0) : vs[jm] * pm.y, bm = j == 0 ? T(0) : vs[j] * pj.y;
+        const T bj = j == 0 ? T(0) : specScale<T>(jm, tab) * pm.y;
+        const T bm = j == 0 ? T(0) : specScale<T>(j, tab) * pj.y;
         dct3InPair<T>(bj, bm, h, &zb, &zbM);
         a[j] = cmk<T>(za.x - zb.y, za.y + zb.x);
         if (j > 0 && j < L / 2) a[L - j] = cmk<T>(zaM.x - zbM.y, zaM.y + zbM.x);
@@ -833,13 +853,13 @@ __device__ inline void idctRowsBlock(const T* __restrict__ Hc, const T* __restri
     if (phi) {
         cplx<T>* in[2] = {a, c};
         cplx<T>* sc2[2] = {b, d};
-        blockFft<T, true, 2>(in, sc2, f.tw, L, tid, nt);
+        blockFft<T, true, 2>(in, sc2, tw, L, tid, nt);
         for (int y = tid; y < L; y += nt) phi[row + y] = in[1][dctPos(y, L)].x;
         a = in[0];
     } else {
         cplx<T>* in[1] = {a};
         cplx<T>* sc1[1] = {b};
-        blockFft<T, true, 1>(in, sc1, f.tw, L, tid, nt);
+        blockFft<T, true, 1>(in, sc1, tw, L, tid, nt);
         a = in[0];
     }
     for (int y = tid; y < L; y += nt) {
@@ -856,9 +876,9 @@ __global__ __launch_bounds__(kBlock) void k_idct_rows(const T* __restrict__ Hc,
                                                       T* __restrict__ fx, T* __restrict__ fy,
                                                       T* __restrict__ phi, int cstep, int nx,
                                                       int ny, int ld, FftTab<T> tab,
-                                                      const T* __restrict__ v, const int* skip) {
+                                                      const int* skip) {
     if (skip && *skip) return;
-    idctRowsBlock<T>(Hc, Hs, fx, fy, phi, cstep, nx, ny, ld, tab, v, blockIdx.x);
+    idctRowsBlock<T>(Hc, Hs, fx, fy, phi, cstep, nx, ny, ld, tab, blockIdx.x);
 }
 
 template <typename T, int EPI, bool TRANS>
@@ -916,7 +936,7 @@ template <typename T>
 PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
                             bool interleaved, DevArena* arena)
     : nx_(nx), ny_(ny), mx_(roundUp(nx, 64)), my_(roundUp(ny, 64)), stream_(stream),
-      interleaved_(interleaved), ownsMem_(arena == nullptr) {
+      interleaved_(interleaved), ownsMem_(arena == nullptr), stepX_(stepX), stepY_(stepY) {
     if (nx < 2 || ny < 2) throw std::invalid_argument("Poisson grid must be at least 2x2");
     if (!(stepX > 0) || !(stepY > 0)) throw std::invalid_argument("bin step must be positive");
     const double pi = 3.14159265358979323846;
@@ -936,19 +956,24 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
     };
     fft_ = fftShape(nx) && fftShape(ny);
     if (fft_) {
-        // twiddles exp(-2 pi i t/L) and half-angle factors exp(-i pi k/2L)
+        // the factor tables of exp(-2 pi i t / 4L) (Twid): lo, then hi
         auto tables = [&](int L) {
-            std::vector<double> t(4 * static_cast<size_t>(L));
-            for (int k = 0; k < L; ++k) {
-                t[2 * k] = std::cos(2 * pi * k / L);
-                t[2 * k + 1] = -std::sin(2 * pi * k / L);
-                t[2 * L + 2 * k] = std::cos(pi * k / (2.0 * L));
-                t[2 * L + 2 * k + 1] = -std::sin(pi * k / (2.0 * L));
+            const int s = twShift(L), nLo = twLo(L), n = twCount(L);
+            std::vector<double> t(2 * static_cast<size_t>(n));
+            for (int k = 0; k < n; ++k) {
+                const double e = k < nLo ? k : static_cast<double>((k - nLo) << s);
+                t[2 * k] = std::cos(2 * pi * e / (4.0 * L));
+                t[2 * k + 1] = -std::sin(2 * pi * e / (4.0 * L));
             }
             return t;
         };
-        put(&tabX_, tables(nx));
-        put(&tabY_, tables(ny));
+        if (arena) {
+            arena->upload(&tabX_, tables(nx));
+            arena->upload(&tabY_, tables(ny));
+        } else {
+            tabX_ = upload<double>(tables(nx), stream);
+            tabY_ = upload<double>(tables(ny), stream);
+        }
         setFftLds();
     }
     std::vector<double> u(mx_, 0.0), v(my_, 0.0);
@@ -1019,7 +1044,9 @@ template <typename T>
 PoissonPlan<T>::~PoissonPlan() {
     if (!ownsMem_) return;
     for (T* b : {rho_, phi_, ex_, ey_, exy_, t0_, p_, t1_, t2_, cyT_, cx_, ayT_, bsv_, sxu_, ax_, u_,
-                 v_, tabX_, tabY_})
+                 v_})
+        if (b) (void)hipFree(b);
+    for (double* b : {tabX_, tabY_})
         if (b) (void)hipFree(b);
 }
 
@@ -1047,24 +1074,24 @@ bool PoissonPlan<T>::fftShape(int n) {
 template <typename T>
 void PoissonPlan<T>::setFftLds() {
     const size_t c = 2 * sizeof(T);
-    ldsA_ = (kFftLdsHead(ny_) + 2 * static_cast<size_t>(ny_)) * c;
+    ldsA_ = (kFftLdsHead<T>(ny_) + 2 * static_cast<size_t>(ny_)) * c;
     // column pairs per block of the column pass: one (so that the pass runs
     // on ny / 2 blocks; measured faster at 512^2 than gathering 4 or 8 pairs
     // per block for longer row segments), within 150 KB of LDS
     colPairs_ = 1;
     while (colPairs_ > 1 && (colPairs_ > ny_ / 2 ||
-                             (kFftLdsHead(nx_) + 4 * static_cast<size_t>(colPairs_) * nx_) * c >
+                             (kFftLdsHead<T>(nx_) + 3 * static_cast<size_t>(colPairs_) * nx_) * c >
                                  150 * 1024))
         colPairs_ >>= 1;
-    ldsB_ = (kFftLdsHead(nx_) + 4 * static_cast<size_t>(colPairs_) * nx_) * c;
-    ldsC_ = (kFftLdsHead(ny_) + 4 * static_cast<size_t>(ny_)) * c;
+    ldsB_ = (kFftLdsHead<T>(nx_) + 3 * static_cast<size_t>(colPairs_) * nx_) * c;
+    ldsC_ = (kFftLdsHead<T>(ny_) + 4 * static_cast<size_t>(ny_)) * c;
     // the column pass sharing its launch with the WA blocks: kBlock threads,
-    // so kBlock / team column pairs per block, its tables read from global
-    // memory (every block of the launch reserves the same LDS, and the WA
-    // blocks need the room); only where that LDS stays within kFusedLdsMax
+    // so kBlock / team column pairs per block, three line buffers per team
+    // (every block of the launch reserves the same LDS, and the WA blocks
+    // need the room); only where that LDS stays within kFusedLdsMax
     {
         const int nt = fftThreads(nx_), G = kBlock / nt;
-        const size_t lds = 3 * static_cast<size_t>(G) * nx_ * c;
+        const size_t lds = (kFftLdsHead<T>(nx_) + 3 * static_cast<size_t>(G) * nx_) * c;
         ldsBW_ = (kBlock % nt == 0 && (ny_ / 2) % G == 0 && lds <= kFusedLdsMax) ? lds : 0;
         if (ldsBW_)
             raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols_wa<T>), ldsBW_);
@@ -1079,7 +1106,7 @@ void PoissonPlan<T>::setFftLds() {
 template <typename T>
 bool PoissonPlan<T>::setColumnPairs(int pairs) {
     if (!fft_ || pairs < 1 || (pairs & (pairs - 1)) != 0 || (ny_ / 2) % pairs != 0) return false;
-    const size_t lds = (kFftLdsHead(nx_) + 4 * static_cast<size_t>(pairs) * nx_) * 2 * sizeof(T);
+    const size_t lds = (kFftLdsHead<T>(nx_) + 3 * static_cast<size_t>(pairs) * nx_) * 2 * sizeof(T);
     if (lds > 150 * 1024 || pairs * fftThreads(nx_) > 1024) return false;
     colPairs_ = pairs;
     ldsB_ = lds;
@@ -1097,28 +1124,26 @@ void PoissonPlan<T>::solveFft(bool withPhi, const int* skip) {
 // Passes B and C of the FFT path.
 template <typename T>
 void PoissonPlan<T>::passBC(const int* skip, bool withPhi) {
-    using C2 = cplx<T>;
-    const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
-    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    const FftTab<T> tx{reinterpret_cast<const cplx<double>*>(tabX_), nx_, stepX_};
+    const FftTab<T> ty{reinterpret_cast<const cplx<double>*>(tabY_), ny_, stepY_};
     const int ld = my_;
     const int colThreads = colPairs_ * fftThreads(nx_);
     launchKernel(colThreads <= 256 ? k_dct_cols<T, 256> : k_dct_cols<T, 1024>,
                  dim3(ny_ / (2 * colPairs_)), dim3(colThreads), static_cast<uint32_t>(ldsB_),
-                 stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx, u_, v_, invNorm_, skip);
+                 stream_, t0_, t1_, t2_, colPairs_, nx_, ny_, ld, tx, ty, invNorm_, skip);
     SPHIP_CHECK(hipGetLastError());
     T* fx = interleaved_ ? exy_ : ex_;
     T* fy = interleaved_ ? exy_ + 1 : ey_;
     const size_t lds = withPhi ? ldsC_ : ldsC_ - 2 * static_cast<size_t>(ny_) * 2 * sizeof(T);
     launchKernel(k_idct_rows<T>, dim3(nx_), dim3(fftThreads(ny_)), static_cast<uint32_t>(lds),
                  stream_, t1_, t2_, fx, fy, withPhi ? phi_ : nullptr, interleaved_ ? 2 : 1, nx_,
-                 ny_, ld, ty, v_, skip);
+                 ny_, ld, ty, skip);
     SPHIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 int PoissonPlan<T>::passA(const DensityIn<T>* din, const int* skip) {
-    using C2 = cplx<T>;
-    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    const FftTab<T> ty{reinterpret_cast<const cplx<double>*>(tabY_), ny_, stepY_};
     if (din) {
         launchKernel(k_dct_rows_rho<T>, dim3(nx_ / 2), dim3(kRhoThreads),
                      static_cast<uint32_t>(ldsA_), stream_, *din, t0_, nx_, ny_, my_, ty, skip);
@@ -1152,23 +1177,22 @@ template <typename T>
 bool PoissonPlan<T>::solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip,
                                  const DensityIn<T>* din, int* densityBlocks) {
     if (!fft_ || ldsBW_ == 0 || (din && (!takesDensity() || din->g.dim != ny_))) return false;
-    using C2 = cplx<T>;
-    const FftTab<T> tx{reinterpret_cast<const C2*>(tabX_), reinterpret_cast<const C2*>(tabX_) + nx_};
-    const FftTab<T> ty{reinterpret_cast<const C2*>(tabY_), reinterpret_cast<const C2*>(tabY_) + ny_};
+    const FftTab<T> tx{reinterpret_cast<const cplx<double>*>(tabX_), nx_, stepX_};
+    const FftTab<T> ty{reinterpret_cast<const cplx<double>*>(tabY_), ny_, stepY_};
     const int ld = my_;
     const int blocksA = passA(din, skip);
     if (densityBlocks) *densityBlocks = blocksA;
     const int G = kBlock / fftThreads(nx_), colBlocks = ny_ / (2 * G);
     launchKernel(k_dct_cols_wa<T>, dim3(colBlocks + waBlocks), dim3(kBlock),
-                 static_cast<uint32_t>(ldsBW_), stream_, t0_, t1_, t2_, G, nx_, ny_, ld, tx, u_,
-                 v_, invNorm_, skip, wa, colBlocks, 0, waBlocks);
+                 static_cast<uint32_t>(ldsBW_), stream_, t0_, t1_, t2_, G, nx_, ny_, ld, tx, ty,
+                 invNorm_, skip, wa, colBlocks, 0, waBlocks);
     SPHIP_CHECK(hipGetLastError());
     T* fx = interleaved_ ? exy_ : ex_;
     T* fy = interleaved_ ? exy_ + 1 : ey_;
     const size_t lds = ldsC_ - 2 * static_cast<size_t>(ny_) * 2 * sizeof(T);
     launchKernel(k_idct_rows<T>, dim3(nx_), dim3(fftThreads(ny_)), static_cast<uint32_t>(lds),
                  stream_, t1_, t2_, fx, fy, static_cast<T*>(nullptr), interleaved_ ? 2 : 1, nx_,
-                 ny_, ld, ty, v_, skip);
+                 ny_, ld, ty, skip);
     SPHIP_CHECK(hipGetLastError());
     return true;
 }
@@ -1216,21 +1240,17 @@ void PoissonPlan<T>::solve(bool withPhi, const int* skip) {
 }
 
 // ------------------------------------------------------------ no allocation
-// The tables of one axis: L twiddles exp(-2 pi i t/L), then L half-angle
-// factors exp(-i pi k/2L) (complex, interleaved), in FP64 rounded once to T,
-// the values PoissonPlan uploads; and the FP32 spectral scales
-// u_j = pi j / (n step), in the reference's float arithmetic.
-__global__ __launch_bounds__(kBlock) void k_fft_tables(float* tab, float* scale, int L,
-                                                       float step) {
+// The factor tables of one axis: the twCount(L) entries of exp(-2 pi i t/4L)
+// (Twid; complex, interleaved, FP64), the values PoissonPlan uploads.
+__global__ __launch_bounds__(kBlock) void k_fft_tables(double* tab, int L) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= L) return;
     const double pi = 3.14159265358979323846;
-    tab[2 * k] = static_cast<float>(cos(2 * pi * k / L));
-    tab[2 * k + 1] = static_cast<float>(-sin(2 * pi * k / L));
-    tab[2 * L + 2 * k] = static_cast<float>(cos(pi * k / (2.0 * L)));
-    tab[2 * L + 2 * k + 1] = static_cast<float>(-sin(pi * k / (2.0 * L)));
-    scale[k] = static_cast<float>(3.14159265358979323846f) * static_cast<float>(k) /
-               (static_cast<float>(L) * step);
+    if (k < twCount(L)) {
+        const int nLo = twLo(L);
+        const double e = k < nLo ? k : static_cast<double>((k - nLo) << twShift(L));
+        tab[2 * k] = cos(2 * pi * e / (4.0 * L));
+        tab[2 * k + 1] = -sin(2 * pi * e / (4.0 * L));
+    }
 }
 
 size_t fftSolveWorkFloats(int nx, int ny) {
@@ -1242,27 +1262,27 @@ void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, fl
     if (!PoissonPlan<float>::fftShape(nx) || !PoissonPlan<float>::fftShape(ny) || nx < 64 ||
         ny < 64)
         throw std::invalid_argument("fftSolveInto: grid sides must be powers of two, 64 to 1024");
-    using C2 = cplx<float>;
     const size_t n = static_cast<size_t>(nx) * ny;
     float* t0 = work;
     float* t1 = t0 + n;
     float* t2 = t1 + n;
-    float* tabX = t2 + n;
-    float* tabY = tabX + 4 * static_cast<size_t>(nx);
-    float* u = tabY + 4 * static_cast<size_t>(ny);
-    float* v = u + nx;
-    launchKernel(k_fft_tables, dim3((nx + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tabX, u, nx,
-                 stepX);
-    launchKernel(k_fft_tables, dim3((ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tabY, v, ny,
-                 stepY);
+    // each axis's FP64 factor tables in the 4 n floats the work array gives it
+    double* tabX = reinterpret_cast<double*>(t2 + n);
+    double* tabY = reinterpret_cast<double*>(t2 + n + 4 * static_cast<size_t>(nx));
+    static_assert(4 * twCount(64) <= 4 * 64 && 4 * twCount(1024) <= 4 * 1024,
+                  "an axis's factor tables fit the 4 n floats the work array gives them");
+    launchKernel(k_fft_tables, dim3((twCount(nx) + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                 tabX, nx);
+    launchKernel(k_fft_tables, dim3((twCount(ny) + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                 tabY, ny);
     SPHIP_CHECK(hipGetLastError());
-    const FftTab<float> tx{reinterpret_cast<const C2*>(tabX), reinterpret_cast<const C2*>(tabX) + nx};
-    const FftTab<float> ty{reinterpret_cast<const C2*>(tabY), reinterpret_cast<const C2*>(tabY) + ny};
+    const FftTab<float> tx{reinterpret_cast<const cplx<double>*>(tabX), nx, stepX};
+    const FftTab<float> ty{reinterpret_cast<const cplx<double>*>(tabY), ny, stepY};
     // the LDS of the plan's passes with one column pair per block (setFftLds)
     const size_t c = 2 * sizeof(float);
-    const size_t ldsA = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
-    const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
-    const size_t ldsC = (kFftLdsHead(ny) + 4 * static_cast<size_t>(ny)) * c;
+    const size_t ldsA = (kFftLdsHead<float>(ny) + 2 * static_cast<size_t>(ny)) * c;
+    const size_t ldsB = (kFftLdsHead<float>(nx) + 3 * static_cast<size_t>(nx)) * c;
+    const size_t ldsC = (kFftLdsHead<float>(ny) + 4 * static_cast<size_t>(ny)) * c;
     raiseLdsLimit(reinterpret_cast<const void*>(k_dct_rows<float>), ldsA);
     raiseLdsLimit(reinterpret_cast<const void*>(k_dct_cols<float, 256>), ldsB);
     raiseLdsLimit(reinterpret_cast<const void*>(k_idct_rows<float>), ldsC);
@@ -1270,11 +1290,10 @@ void fftSolveInto(int nx, int ny, float stepX, float stepY, const float* rho, fl
     launchKernel(k_dct_rows<float>, dim3(nx / 2), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsA),
                  s, rho, t0, nx, ny, ny, ty, static_cast<const int*>(nullptr));
     launchKernel(k_dct_cols<float, 256>, dim3(ny / 2), dim3(fftThreads(nx)), static_cast<uint32_t>(ldsB),
-                 s, t0, t1, t2, 1, nx, ny, ny, tx, static_cast<const float*>(u),
-                 static_cast<const float*>(v), invNorm, static_cast<const int*>(nullptr));
+                 s, t0, t1, t2, 1, nx, ny, ny, tx, ty, invNorm, static_cast<const int*>(nullptr));
     launchKernel(k_idct_rows<float>, dim3(nx), dim3(fftThreads(ny)), static_cast<uint32_t>(ldsC),
                  s, t1, t2, out3 + n, out3 + 2 * n, out3, 1, nx, ny, ny, ty,
-                 static_cast<const float*>(v), static_cast<const int*>(nullptr));
+                 static_cast<const int*>(nullptr));
     SPHIP_CHECK(hipGetLastError());
 }
 
@@ -1288,17 +1307,16 @@ __global__ __launch_bounds__(kBlock) void k_fft_batch(const PoissonBatchArgs<T>*
     const int4 m = map[blockIdx.x];
     const PoissonBatchArgs<T>& a = args[m.x];
     if (a.skip && *a.skip) return;
-    using C2 = cplx<T>;
-    const FftTab<T> tx{reinterpret_cast<const C2*>(a.tabX), reinterpret_cast<const C2*>(a.tabX) + nx};
-    const FftTab<T> ty{reinterpret_cast<const C2*>(a.tabY), reinterpret_cast<const C2*>(a.tabY) + ny};
+    const FftTab<T> tx{reinterpret_cast<const cplx<double>*>(a.tabX), nx, a.stepX};
+    const FftTab<T> ty{reinterpret_cast<const cplx<double>*>(a.tabY), ny, a.stepY};
     if constexpr (PASS == 0)
         dctRowsBlock<T>(a.rho, a.t0, nx, ny, ld, ty, m.y);
     else if constexpr (PASS == 1)
-        dctColsBlock<T, true>(a.t0, a.t1, a.t2, 1, nx, ny, ld, tx, a.u, a.v, a.invNorm,
+        dctColsBlock<T>(a.t0, a.t1, a.t2, 1, nx, ny, ld, tx, ty, a.invNorm,
                               xcdBlock(m.y, m.z));
     else
         idctRowsBlock<T>(a.t1, a.t2, a.exy, a.exy + 1, static_cast<T*>(nullptr), 2, nx, ny, ld, ty,
-                         a.v, m.y);
+                         m.y);
 }
 
 }  // namespace
@@ -1306,7 +1324,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_batch(const PoissonBatchArgs<T>*
 template <typename T>
 PoissonBatchArgs<T> PoissonPlan<T>::batchArgs(const int* skip) const {
     if (!fft_ || !interleaved_) throw std::logic_error("batched solve: FFT plan with interleaved field only");
-    return PoissonBatchArgs<T>{rho_, t0_, t1_, t2_, exy_, tabX_, tabY_, u_, v_, invNorm_, skip};
+    return PoissonBatchArgs<T>{rho_, t0_, t1_, t2_, exy_, tabX_, tabY_, stepX_, stepY_, invNorm_,
+                               skip};
 }
 
 template <typename T>
@@ -1316,13 +1335,13 @@ void launchPoissonBatch(int pass, const PoissonBatchArgs<T>* args, const int4* m
     const size_t c = 2 * sizeof(T);
     // the plan's LDS with one column pair per block (setFftLds), the inverse
     // rows without phi
-    const size_t ldsA = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
-    const size_t ldsB = (kFftLdsHead(nx) + 4 * static_cast<size_t>(nx)) * c;
-    const size_t ldsC = (kFftLdsHead(ny) + 2 * static_cast<size_t>(ny)) * c;
+    const size_t ldsA = (kFftLdsHead<T>(ny) + 2 * static_cast<size_t>(ny)) * c;
+    const size_t ldsB = (kFftLdsHead<T>(nx) + 3 * static_cast<size_t>(nx)) * c;
+    const size_t ldsC = (kFftLdsHead<T>(ny) + 2 * static_cast<size_t>(ny)) * c;
     static std::once_flag once;
     std::call_once(once, [] {
         // the largest grid's LDS (1024 per side) covers every smaller one
-        const size_t big = (kFftLdsHead(1024) + 4 * 1024) * 2 * sizeof(T);
+        const size_t big = (kFftLdsHead<T>(1024) + 4 * 1024) * 2 * sizeof(T);
         raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 0>), big);
         raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 1>), big);
         raiseLdsLimit(reinterpret_cast<const void*>(k_fft_batch<T, 2>), big);
