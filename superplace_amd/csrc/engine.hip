@@ -342,7 +342,7 @@ public:
         counts[BK_SCATTER] = tg.tilesX * tg.tilesY + (dd_.nDens + tg.chunk - 1) / tg.chunk;
         counts[BK_FINALIZE] = static_cast<int>(
             (std::max<long>(static_cast<long>(nb_), tg.nKeys) + kBlock - 1) / kBlock);
-        counts[BK_WA] = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
+        counts[BK_WA] = dd_.waBlocks;
         counts[BK_NODE] = (n_ + kBlock - 1) / kBlock;
         counts[BK_REDUCE] = 1;
         counts[BK_UPDATE] = counts[BK_NODE];
@@ -843,13 +843,26 @@ void Engine<T>::build(const sp_hip_design& d) {
         }
         dd_.waNetLo[kWaClasses + 1] = d.num_nets;
         dd_.waWaveBase[0] = 0;
+        dd_.waLaneBase[0] = 0;
         for (int c = 0; c < kWaClasses; ++c) {
-            const int perWave = 64 >> (c + 1);
             const int nets = dd_.waNetLo[c + 1] - dd_.waNetLo[c];
-            dd_.waWaveBase[c + 1] = dd_.waWaveBase[c] + (nets + perWave - 1) / perWave;
+            const int tasks = (nets + waNetsPerTask(c) - 1) / waNetsPerTask(c);
+            dd_.waWaveBase[c + 1] = dd_.waWaveBase[c] + tasks;
+            dd_.waLaneBase[c + 1] = dd_.waLaneBase[c] + static_cast<long>(tasks) * waEntriesPerTask(c);
         }
         dd_.waWaveBase[kWaClasses + 1] =
             dd_.waWaveBase[kWaClasses] + (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]);
+        // WA blocks: no more than the GPU holds at once, in the WA pass's own
+        // launch and, where the loop fuses it, beside the column blocks (a
+        // block that waits for a free slot starts a second round and sets the
+        // launch's length); every WA pass of the instance takes this count,
+        // so its partials sum in one order
+        int resident = waResidentBlocks<T>();
+        if (fuseWa_) {
+            const int fused = poisson_->fusedWaResidentBlocks();
+            if (fused > 0) resident = std::min(resident, fused);
+        }
+        dd_.waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1], resident);
     }
     // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
     // of pins that no net pass writes (no net, degree < 2 or above
@@ -861,20 +874,24 @@ void Engine<T>::build(const sp_hip_design& d) {
         if (pinSlot[p] >= 0) slotNp[pinSlot[p]] = q;
     }
     arena_.upload(&dd_.nodeStart, nodeStart);
-    // classes below kWaClasses in lane order: net g of class c (W = 2^(c+1)
-    // lanes per net, 64 / W nets per wavefront task) takes lanes
-    // (g % (64 / W)) W + j, j < degree, of task waWaveBase[c] + g / (64 / W)
+    // classes below kWaClasses in lane order, from entry waLaneBase[c]:
+    //  - c < kWaLaneClasses: net g is lane g % 64 of task g / 64, its pin j
+    //    entry (task P + j) 64 + lane (P = 2^(c+1) pins per lane);
+    //  - otherwise net g (W = 2^(c+1) lanes per net, 64 / W nets per task)
+    //    takes lanes (g % (64 / W)) W + j, j < degree, of task g / (64 / W)
     {
-        const long lanes = 64L * dd_.waWaveBase[kWaClasses];
+        const long lanes = dd_.waLaneBase[kWaClasses];
         std::vector<int> laneNode(lanes, -1), laneNp(lanes, 0);
         std::vector<float> laneOffX(lanes, 0.f), laneOffY(lanes, 0.f);
         for (int c = 0; c < kWaClasses; ++c) {
             const int W = 2 << c, perWave = 64 / W;
             for (int k = dd_.waNetLo[c]; k < dd_.waNetLo[c + 1]; ++k) {
                 const int g = k - dd_.waNetLo[c];
-                const long base = 64L * (dd_.waWaveBase[c] + g / perWave) + (g % perWave) * W;
                 for (int t = netStart[k]; t < netStart[k + 1]; ++t) {
-                    const long e = base + (t - netStart[k]);
+                    const int j = t - netStart[k];
+                    const long e = c < kWaLaneClasses
+                        ? dd_.waLaneBase[c] + (static_cast<long>(g / 64) * W + j) * 64 + g % 64
+                        : dd_.waLaneBase[c] + 64L * (g / perWave) + (g % perWave) * W + j;
                     laneNode[e] = slotNode[t];
                     laneOffX[e] = slotOffX[t];
                     laneOffY[e] = slotOffY[t];
@@ -954,7 +971,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     flopsPer_[SP_HIP_K_POISSON] = poisson_->flops(false);
     // WA net pass: per lane of the class tasks its node id, offsets and entry
     // id; per pin the node's x/y and the gradient write; tail slot ranges
-    bytesPer_[SP_HIP_K_WA_NETS] = 64.0 * dd_.waWaveBase[kWaClasses] * (4 + 2 * ts + 4) +
+    bytesPer_[SP_HIP_K_WA_NETS] = static_cast<double>(dd_.waLaneBase[kWaClasses]) * (4 + 2 * ts + 4) +
                                   P * (2 * ts + 2 * ts) +
                                   (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]) * 4.0;
     // node pass: per node its index, x/y, w/h, flags, pin range, the
@@ -967,7 +984,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     // reduce: the block partials of the evaluation (1, 2 and 7 doubles)
     bytesPer_[SP_HIP_K_REDUCE] =
         8.0 * ((nb_ + kBlock - 1) / kBlock +
-               2.0 * std::min(kWaMaxBlocks, (dd_.waWaveBase[kWaClasses + 1] + 3) / 4) +
+               2.0 * dd_.waBlocks +
                7.0 * ((n_ + kBlock - 1) / kBlock));
 }
 
@@ -1341,7 +1358,7 @@ void Engine<T>::launchSlot(bool events, int slot) {
     // two apart so that each family has its own events.
     bool fused = false;
     if (!events && fuseWa_) {
-        const int waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1]);
+        const int waBlocks = dd_.waBlocks;
         int densityBlocks = 0;
         fused = poisson_->solveWithWa(
             WaTask<T>{dd_, T(0), cfg_.ignore_net_degree, ws_.region(1), dCtl_}, waBlocks, skip,

@@ -57,9 +57,8 @@ struct DevDesign {
     const float* precDeg = nullptr;    // pin count of opt node k
     // Nets are renumbered by degree (ascending), so each WA size class below
     // is a contiguous range of nets.  Classes below kWaClasses keep their
-    // pins in lane order: entry 64 gw + l is lane l of wavefront task gw
-    // (node -1: an empty lane), with its pin offsets and its node-pin entry
-    // (where its gradient goes).
+    // pins in lane order (waLaneBase; node -1: an empty slot), with their pin
+    // offsets and node-pin entries (where their gradients go).
     const int* laneNode = nullptr;
     const T* laneOffX = nullptr;
     const T* laneOffY = nullptr;
@@ -72,12 +71,16 @@ struct DevDesign {
     const T* slotOffY = nullptr;
     const int* slotNp = nullptr;
     // WA size classes: class c < kWaClasses holds the nets of degree in
-    // (2^c, 2^(c+1)] (class 0: degree 2), 2^(c+1) lanes per net; class
-    // kWaClasses the nets of degree > 2^kWaClasses, one wavefront each.
+    // (2^c, 2^(c+1)] (class 0: degree 2), one lane per net (2^(c+1) pin
+    // slots, 64 nets per wavefront) below kWaLaneClasses, else 2^(c+1) lanes
+    // per net; class kWaClasses the nets of degree > 2^kWaClasses, one
+    // wavefront each.
     // Nets of class c: [waNetLo[c], waNetLo[c+1]); its wavefronts start at
     // waWaveBase[c]; waWaveBase[kWaClasses + 1] is the total.
     int waNetLo[8];
     int waWaveBase[8];
+    int waBlocks = 1;     // blocks of every WA pass of the instance (fixes its partials' order)
+    long waLaneBase[8];   // first lane-order entry of class c (c <= kWaClasses)
     // node -> pin entries; pinGrad holds the WA gradient (x, y) of each entry,
     // written by the net pass (zero for pins of nets it skips)
     const int* nodeStart = nullptr;
@@ -85,13 +88,23 @@ struct DevDesign {
 };
 
 constexpr int kWaClasses = 6;        // lanes per net 2, 4, ..., 64; then one wave per net
+constexpr int kWaLaneClasses = 2;    // classes 0, 1 (degree 2, 3-4): one lane per net
+// Nets of class c per wavefront task, and lane-order entries per task.
+__host__ __device__ inline int waNetsPerTask(int c) { return c < kWaLaneClasses ? 64 : 64 >> (c + 1); }
+__host__ __device__ inline int waEntriesPerTask(int c) { return c < kWaLaneClasses ? 64 * (2 << c) : 64; }
 constexpr int kWaTailRegs = 8;       // pins per lane per batch of a tail net's extrema pass
 constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
-// Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each).
-inline int waBlocksFor(int waves) {
+// Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each),
+// at most `resident` (the blocks the GPU holds at once, so that no block
+// waits for another to end before it starts).
+inline int waBlocksFor(int waves, int resident = kWaMaxBlocks) {
     const int b = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    return b < 1 ? 1 : (b > kWaMaxBlocks ? kWaMaxBlocks : b);
+    const int cap = resident < kWaMaxBlocks ? resident : kWaMaxBlocks;
+    return b < 1 ? 1 : (b > cap ? (cap < 1 ? 1 : cap) : b);
 }
+// Blocks of the WA pass's kernel (k_wa_pins) one GPU holds at once.
+template <typename T>
+int waResidentBlocks();
 constexpr int kTileBins = 16;        // bins per tile side
 constexpr int kTileHalo = 4;         // halo bins of a tile's scatter window
 constexpr int kTileWin = kTileBins + 2 * kTileHalo;

@@ -1013,11 +1013,20 @@ void launchFinalize(const GridGeom<T>& g, const TileGrid& tg, unsigned long long
 }
 
 template <typename T>
+int waResidentBlocks() {
+    int dev = 0, cus = 0, perCu = 0;
+    SPHIP_CHECK(hipGetDevice(&dev));
+    SPHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SPHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &perCu, reinterpret_cast<const void*>(k_wa_pins<T>), kBlock, 0));
+    return perCu * cus;
+}
+
+template <typename T>
 void launchWaNets(const DevDesign<T>& d, float gamma, int ignoreDeg, ReduceWs& ws,
                   const Ctl* ctl, hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
-    const int waves = d.waWaveBase[kWaClasses + 1];
-    ws.blocks[1] = waBlocksFor(waves);
+    ws.blocks[1] = d.waBlocks;
     launchKernel(k_wa_pins<T>, ws.blocks[1], kBlock, 0, s, WaTask<T>{d, invG, ignoreDeg, ws.region(1), ctl});
     SPHIP_CHECK(hipGetLastError());
 }
@@ -1051,6 +1060,7 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
 }
 
 #define SPHIP_INST(T)                                                                            \
+    template int waResidentBlocks<T>();                                                          \
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
                                    unsigned long long*, unsigned long long*, bool, ReduceWs&,    \
                                    const Ctl*, hipStream_t);                                                 \

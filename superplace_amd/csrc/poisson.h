@@ -71,6 +71,10 @@ public:
     // With din, the first pass finalizes the density grids itself (rho is
     // then not written): one tau partial per block of that pass into
     // din->partials, their count into *densityBlocks.
+    // WA blocks that fit beside the column blocks in the fused launch (the
+    // blocks the GPU holds at once, less the column blocks); 0 when the plan
+    // has no fused launch
+    int fusedWaResidentBlocks() const;
     bool solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip,
                      const DensityIn<T>* din = nullptr, int* densityBlocks = nullptr);
     // solve(false, skip) from the integer density grids (takesDensity()

@@ -1174,6 +1174,18 @@ int PoissonPlan<T>::solveFromDensity(const DensityIn<T>& din, const int* skip) {
 }
 
 template <typename T>
+int PoissonPlan<T>::fusedWaResidentBlocks() const {
+    if (!fft_ || ldsBW_ == 0) return 0;
+    int dev = 0, cus = 0, perCu = 0;
+    SPHIP_CHECK(hipGetDevice(&dev));
+    SPHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SPHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &perCu, reinterpret_cast<const void*>(k_dct_cols_wa<T>), kBlock, ldsBW_));
+    const int colBlocks = ny_ / (2 * (kBlock / fftThreads(nx_)));
+    return perCu * cus - colBlocks;
+}
+
+template <typename T>
 bool PoissonPlan<T>::solveWithWa(const WaTask<T>& wa, int waBlocks, const int* skip,
                                  const DensityIn<T>* din, int* densityBlocks) {
     if (!fft_ || ldsBW_ == 0 || (din && (!takesDensity() || din->g.dim != ny_))) return false;

@@ -124,17 +124,78 @@ __device__ inline double waOf(T mxX, T mnX, T mxY, T mnY, const WaSums<T>& s) {
            static_cast<double>((mxY - mnY) + s.cpy / s.bpy - s.cny / s.bny);
 }
 
+// Size class of nets of at most P pins with one lane per net: the lane
+// holds its net's pins in registers, so extrema, sums and gradients need no
+// cross-lane step, and the sums run over the pins in net order as the
+// reference's loops do (wa_wirelength.cpp:119-225).  Pin j of the lane's net
+// is lane-order entry (task P + j) 64 + lane: every load is coalesced and
+// needs no per-net index first.  Empty slots (node -1) carry no pin.
+template <int P, typename T>
+__device__ inline void waLanePass(const DevDesign<T>& d, int c, int gw, int lane, T invG,
+                                  int ignoreDeg, double (&v)[2]) {
+    const long e0 = d.waLaneBase[c] + static_cast<long>(gw - d.waWaveBase[c]) * P * 64 + lane;
+    int nd[P], q[P];
+    T px[P], py[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const long e = e0 + 64L * j;
+        nd[j] = d.laneNode[e];
+        px[j] = d.laneOffX[e];
+        py[j] = d.laneOffY[e];
+        q[j] = d.laneNp[e];
+    }
+    int deg = 0;
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+        if (nd[j] >= 0) {
+            const auto p = load2(d.xy, nd[j]);
+            px[j] = p.x + px[j];
+            py[j] = p.y + py[j];
+            ++deg;
+        }
+    if (deg < 2) return;
+    T mxX = -tmax<T>(), mnX = tmax<T>(), mxY = -tmax<T>(), mnY = tmax<T>();
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+        if (nd[j] >= 0) {
+            mxX = fmax(mxX, px[j]); mnX = fmin(mnX, px[j]);
+            mxY = fmax(mxY, py[j]); mnY = fmin(mnY, py[j]);
+        }
+    v[1] += static_cast<double>(mxX - mnX) + static_cast<double>(mxY - mnY);
+    if (deg > ignoreDeg) return;
+    WaSums<T> s{0, 0, 0, 0, 0, 0, 0, 0};
+    T ax[P], anx[P], ay[P], any[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+        if (nd[j] >= 0) {
+            const T dxp = px[j] - mxX, dxn = px[j] - mnX, dyp = py[j] - mxY, dyn = py[j] - mnY;
+            ax[j] = texp<T>(dxp * invG); anx[j] = texp<T>(-dxn * invG);
+            ay[j] = texp<T>(dyp * invG); any[j] = texp<T>(-dyn * invG);
+            s.bpx += ax[j]; s.cpx += dxp * ax[j]; s.bnx += anx[j]; s.cnx += dxn * anx[j];
+            s.bpy += ay[j]; s.cpy += dyp * ay[j]; s.bny += any[j]; s.cny += dyn * any[j];
+        }
+    v[0] += waOf<T>(mxX, mnX, mxY, mnY, s);
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+        if (nd[j] >= 0) {
+            const T dxp = px[j] - mxX, dxn = px[j] - mnX, dyp = py[j] - mxY, dyn = py[j] - mnY;
+            T gx, gy;
+            pinGradOf<T>(s, dxp, dxn, ax[j], anx[j], dyp, dyn, ay[j], any[j], invG, gx, gy);
+            store2(d.pinGrad, q[j], gx, gy);
+        }
+}
+
 // Size class with W = 2^LW lanes per net: one pin per lane, so a pin is
 // loaded once and its exponentials are computed once, for both the net's
 // sums and the pin's gradient.  The class's pins sit in lane order (entry
-// 64 gw + lane of wavefront task gw; node -1 marks an empty lane), so the
+// waLaneBase[c] + 64 task + lane; node -1 marks an empty lane), so the
 // lane's node, offsets and node-pin entry are one coalesced load with no
 // per-net index before it.  Empty lanes carry neutral values.
 template <int LW, typename T>
-__device__ inline void waGroupPass(const DevDesign<T>& d, int gw, int lane, T invG,
+__device__ inline void waGroupPass(const DevDesign<T>& d, int c, int gw, int lane, T invG,
                                    int ignoreDeg, double (&v)[2]) {
     constexpr int W = 1 << LW;
-    const long e = static_cast<long>(gw) * 64 + lane;
+    const long e = d.waLaneBase[c] + static_cast<long>(gw - d.waWaveBase[c]) * 64 + lane;
     const int j = lane & (W - 1);
     const int node = d.laneNode[e];
     const T ox = d.laneOffX[e], oy = d.laneOffY[e];
@@ -344,13 +405,14 @@ __device__ inline void waBlockTask(const WaTask<T>& w, int block, int nblocks) {
         const int gw = tasks - 1 - q;
         int c = 0;
         while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
+        static_assert(kWaLaneClasses == 2, "classes 0 and 1 take one lane per net");
         switch (c) {
-            case 0: waGroupPass<1, T>(d, gw, lane, invG, ignoreDeg, v); break;
-            case 1: waGroupPass<2, T>(d, gw, lane, invG, ignoreDeg, v); break;
-            case 2: waGroupPass<3, T>(d, gw, lane, invG, ignoreDeg, v); break;
-            case 3: waGroupPass<4, T>(d, gw, lane, invG, ignoreDeg, v); break;
-            case 4: waGroupPass<5, T>(d, gw, lane, invG, ignoreDeg, v); break;
-            case 5: waGroupPass<6, T>(d, gw, lane, invG, ignoreDeg, v); break;
+            case 0: waLanePass<2, T>(d, 0, gw, lane, invG, ignoreDeg, v); break;
+            case 1: waLanePass<4, T>(d, 1, gw, lane, invG, ignoreDeg, v); break;
+            case 2: waGroupPass<3, T>(d, 2, gw, lane, invG, ignoreDeg, v); break;
+            case 3: waGroupPass<4, T>(d, 3, gw, lane, invG, ignoreDeg, v); break;
+            case 4: waGroupPass<5, T>(d, 4, gw, lane, invG, ignoreDeg, v); break;
+            case 5: waGroupPass<6, T>(d, 5, gw, lane, invG, ignoreDeg, v); break;
             case kWaClasses: waWavePass<T>(d, gw, lane, invG, ignoreDeg, v); break;
             default: break;
         }
