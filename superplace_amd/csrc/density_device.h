@@ -89,8 +89,9 @@ __device__ inline void finalizeLoad(const DensityIn<T>& in, int b, BinLoads& L) 
 template <typename T>
 __device__ inline T finalizeFinish(const DensityIn<T>& in, int b, const BinLoads& L,
                                    double* over) {
-    in.nodeQ[b] = 0;
-    in.fillQ[b] = 0;
+    // only entries a spill wrote are cleared (most stay zero between sorts)
+    if (L.sn) in.nodeQ[b] = 0;
+    if (L.sf) in.fillQ[b] = 0;
     unsigned long long sn = L.sn, sf = L.sf;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
