@@ -165,7 +165,7 @@ public:
         s_ = takeStream(device_);
         const size_t before = g_devBytes;
         try {
-            build(d);
+            build(fillerOrder(d));
         } catch (...) {
             release();
             throw;
@@ -190,15 +190,15 @@ public:
     }
 
     void optIndex(int32_t* out) const override {
-        std::copy(optIdxHost_.begin(), optIdxHost_.end(), out);
+        for (int k = 0; k < n_; ++k) out[k] = extOf(optIdxHost_[k]);
     }
 
     void positions(float* x, float* y) const override {
         std::vector<T> hx(total_), hy(total_);
         downloadXY(hx.data(), hy.data());
         for (int i = 0; i < total_; ++i) {
-            x[i] = static_cast<float>(hx[i]);
-            y[i] = static_cast<float>(hy[i]);
+            x[extOf(i)] = static_cast<float>(hx[i]);
+            y[extOf(i)] = static_cast<float>(hy[i]);
         }
     }
 
@@ -217,7 +217,11 @@ public:
     // One evaluation at host positions of all nodes (EPlace scatter + objAndGrad).
     void evalHost(const float* x, const float* y, float lambda, float gamma,
                   sp_hip_eval_scalars* out, sp_hip_eval_debug* dbg) override {
-        std::vector<T> hx(x, x + total_), hy(y, y + total_);
+        std::vector<T> hx(total_), hy(total_);
+        for (int i = 0; i < total_; ++i) {
+            hx[i] = x[extOf(i)];
+            hy[i] = y[extOf(i)];
+        }
         uploadXY(hx.data(), hy.data());
         std::vector<T> pos(2 * static_cast<size_t>(n_));
         for (int k = 0; k < n_; ++k) {
@@ -410,6 +414,18 @@ private:
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
     std::vector<T> xyStage_;
+    // Fillers carry no pins, so their order is free: the engine keeps them in
+    // Morton order of their start positions (a 1024 x 1024 lattice over the
+    // core), so that neighbouring threads of the node pass and the update
+    // read neighbouring field bins and write neighbouring density records,
+    // and the first spatial sort starts from a nearly sorted order.  ext_
+    // maps the engine's node index to the caller's (empty: the identity);
+    // positions and node indices cross the C ABI in the caller's order.
+    std::vector<int> ext_;
+    std::vector<float> ordX_, ordY_, ordW_, ordH_;
+    std::vector<uint8_t> ordF_;
+    int extOf(int i) const { return ext_.empty() ? i : ext_[i]; }
+    sp_hip_design fillerOrder(const sp_hip_design& d);
     void build(const sp_hip_design& d);
     void release();
     EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
@@ -571,6 +587,43 @@ void Engine<T>::release() {
     }
     giveStream(device_, s_);
     s_ = nullptr;
+}
+
+template <typename T>
+sp_hip_design Engine<T>::fillerOrder(const sp_hip_design& d) {
+    const int nn = d.num_nodes, total = d.num_nodes + d.num_fillers;
+    if (d.num_fillers < 2) return d;
+    const double lx = d.core_lx, ly = d.core_ly;
+    const double w = std::max(1e-30, static_cast<double>(d.core_hx) - lx);
+    const double h = std::max(1e-30, static_cast<double>(d.core_hy) - ly);
+    auto lattice = [](double v) {
+        return static_cast<uint32_t>(std::clamp(static_cast<int>(v * 1024.0), 0, 1023));
+    };
+    auto morton = [](uint32_t ix, uint32_t iy) {
+        uint64_t k = 0;
+        for (int b = 0; b < 10; ++b)
+            k |= static_cast<uint64_t>(((ix >> b) & 1u) | (((iy >> b) & 1u) << 1)) << (2 * b);
+        return k;
+    };
+    std::vector<uint64_t> key(d.num_fillers);
+    for (int f = 0; f < d.num_fillers; ++f)
+        key[f] = morton(lattice((d.node_x[nn + f] - lx) / w), lattice((d.node_y[nn + f] - ly) / h));
+    ext_.resize(total);
+    std::iota(ext_.begin(), ext_.end(), 0);
+    std::stable_sort(ext_.begin() + nn, ext_.end(),
+                     [&](int a, int b) { return key[a - nn] < key[b - nn]; });
+    auto gather = [&](auto& dst, const auto* src) {
+        dst.resize(total);
+        for (int i = 0; i < total; ++i) dst[i] = src[ext_[i]];
+        return dst.data();
+    };
+    sp_hip_design p = d;
+    p.node_x = gather(ordX_, d.node_x);
+    p.node_y = gather(ordY_, d.node_y);
+    p.node_w = gather(ordW_, d.node_w);
+    p.node_h = gather(ordH_, d.node_h);
+    p.node_flags = gather(ordF_, d.node_flags);
+    return p;
 }
 
 template <typename T>
