@@ -410,7 +410,6 @@ private:
         }
         SPHIP_CHECK(hipMemcpyAsync(dd_.xy, xyStage_.data(), xyStage_.size() * sizeof(T),
                                    hipMemcpyHostToDevice, s_));
-        launchRecRefresh<T>(dd_, s_);
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
     std::vector<T> xyStage_;
@@ -806,16 +805,6 @@ void Engine<T>::build(const sp_hip_design& d) {
         arena_.upload(&dd_.recIdB, recId);
         arena_.upload(&dd_.recWHA, recWH);
         arena_.upload(&dd_.recWHB, recWH);
-        std::vector<float> recXY(2 * dens.size());
-        std::vector<int> recPos(total, -1);
-        for (size_t t = 0; t < dens.size(); ++t) {
-            recXY[2 * t] = d.node_x[dens[t]];
-            recXY[2 * t + 1] = d.node_y[dens[t]];
-            recPos[dens[t]] = static_cast<int>(t);
-        }
-        arena_.upload(&dd_.recXYA, recXY);
-        arena_.upload(&dd_.recXYB, recXY);
-        arena_.upload(&dd_.recPos, recPos);
     }
     {
         TileGrid& tg = dd_.tiles;
@@ -1007,11 +996,13 @@ void Engine<T>::build(const sp_hip_design& d) {
     const double P = d.num_pins, ts = sizeof(T), nd = dd_.nDens;
     const double stageBytes = static_cast<double>(dd_.tiles.tilesX) * dd_.tiles.tilesY * 2 *
                               kTileWin * kTileWin * 8;
-    // sort: pass 1 reads each record's node index and (x, y); pass 2 reads the
-    // whole record, writes it in the new order and the node's record position
-    bytesPer_[SP_HIP_K_SORT] = nd * ((4 + 2 * ts) + (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts + 4));
-    // scatter: reads the ordered records, writes them back and every tile window
-    bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts + 2 * ts) + stageBytes;
+    // sort: pass 1 reads each record's node index and gathers its (x, y); pass
+    // 2 reads the record (index, w/h) and (x, y) again and writes the record
+    // in the new order
+    bytesPer_[SP_HIP_K_SORT] = nd * ((4 + 2 * ts) + (4 + 2 * ts + 2 * ts + 4 + 2 * ts));
+    // scatter: reads the ordered records and their (x, y), writes the records
+    // back and every tile window
+    bytesPer_[SP_HIP_K_SCATTER] = nd * (4 + 2 * ts + 2 * ts + 4 + 2 * ts) + stageBytes;
     // finalize: atomic grids read and cleared, windows read, static terrain, rho
     bytesPer_[SP_HIP_K_FINALIZE] = nb_ * (8.0 * 2 + 8 * 2 + 8 * 2 + ts) + stageBytes;
     bytesPer_[SP_HIP_K_POISSON] = poisson_->bytes(false);
@@ -1032,8 +1023,8 @@ void Engine<T>::build(const sp_hip_design& d) {
     // (x and y); per pin entry its gradient
     bytesPer_[SP_HIP_K_NODE_GRAD] =
         n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 8 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
-    // update: index, w/h, A/G/B reads, v/u writes, (x, y) writes (node and record)
-    bytesPer_[SP_HIP_K_UPDATE] = n_ * (4 + 2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts + 4 + 2 * ts);
+    // update: w/h, A/G/B reads, v/u writes, the node's (x, y) write
+    bytesPer_[SP_HIP_K_UPDATE] = n_ * (2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts);
     // reduce: the block partials of the evaluation (1, 2 and 7 doubles)
     bytesPer_[SP_HIP_K_REDUCE] =
         8.0 * ((nb_ + kBlock - 1) / kBlock +

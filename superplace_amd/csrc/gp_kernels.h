@@ -40,18 +40,15 @@ struct DevDesign {
     int optAffine = 0, optSplit = 0, optShift = 0;
     // The density nodes in spatial order (by tile of their centre bin, tiles
     // in Morton order) as records: node index with the filler / macro flags
-    // in bits 30 / 29 (recId), (w, h) and (x, y) interleaved (recWH, recXY).
-    // launchTileSort reads the previous order (A) and writes the new one (B);
-    // the scatter copies B back to A.  recPos[i] is node i's record position
-    // (-1: no density record), through which the update keeps recXYA current,
-    // so the sort reads positions in record order instead of gathering them.
+    // in bits 30 / 29 (recId) and (w, h) interleaved (recWH).  The sort and
+    // the scatter gather a record's position from xy through its node index
+    // (a copy kept in record order cost the update more in scattered writes
+    // than the gathers cost).  launchTileSort reads the previous order (A)
+    // and writes the new one (B); the scatter copies B back to A.
     int* recIdA = nullptr;
     int* recIdB = nullptr;
     T* recWHA = nullptr;
     T* recWHB = nullptr;
-    T* recXYA = nullptr;
-    T* recXYB = nullptr;
-    int* recPos = nullptr;
     TileGrid tiles;
     const double* precArea = nullptr;  // area of opt node k (double)
     const float* precDeg = nullptr;    // pin count of opt node k
@@ -216,9 +213,6 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
 template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s);
 
-// Reloads recXYA from the node coordinates (after the host wrote them).
-template <typename T>
-void launchRecRefresh(const DevDesign<T>& d, hipStream_t s);
 
 // Sums the staged windows and the atomic grids into node / filler density,
 // then rho and the tau numerator; clears the atomic grids and tile counters.

@@ -120,7 +120,7 @@ __device__ inline unsigned blockTileHist(const DevDesign<T>& d, const GridGeom<T
     unsigned key = 0xFFFFFFFFu;
     if (t < d.nDens) {
         const int rec = d.recIdA[t];
-        const auto p = load2(d.recXYA, t);
+        const auto p = load2(d.xy, rec & kRecNode);
         *outRec = rec;
         *outXY = p;
         key = tileKeyOf<T>(p.x, p.y, g);
@@ -222,8 +222,6 @@ __device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g
             const auto wh = load2(d.recWHA, t);
             d.recIdB[pos] = rec;
             store2(d.recWHB, pos, wh.x, wh.y);
-            store2(d.recXYB, pos, p.x, p.y);
-            d.recPos[rec & kRecNode] = pos;
         }
         __syncthreads();
     }
@@ -316,14 +314,13 @@ __device__ inline void scatterBody(const DevDesign<T>& d, const GridGeom<T>& g, 
         if (sorted) {
             rec = d.recIdB[t];
             wh = load2(d.recWHB, t);
-            p = load2(d.recXYB, t);
+            p = load2(d.xy, rec & kRecNode);
             d.recIdA[t] = rec;
             store2(d.recWHA, t, wh.x, wh.y);
-            store2(d.recXYA, t, p.x, p.y);
         } else {
             rec = d.recIdA[t];
             wh = load2(d.recWHA, t);
-            p = load2(d.recXYA, t);
+            p = load2(d.xy, rec & kRecNode);
         }
         fl[j] = static_cast<uint8_t>(((rec & kRecFiller) ? F_FILLER : 0) |
                                      ((rec & kRecMacro) ? F_MACRO : 0));
@@ -876,8 +873,6 @@ __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G,
     }
     const T nx = uOut ? uu[0] : vv[0], ny = uOut ? uu[1] : vv[1];
     store2(d.xy, i, nx, ny);
-    const int rp = d.recPos[i];
-    if (rp >= 0) store2(d.recXYA, rp, nx, ny);
 }
 
 template <typename T>
@@ -887,13 +882,6 @@ __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, c
     updateBody<T>(d, A, G, B, alpha, m, vOut, uOut, ctl, bufs, blockIdx.x);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rec_refresh(DevDesign<T> d) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.nDens) return;
-    const auto p = load2(d.xy, d.recIdA[t] & kRecNode);
-    store2(d.recXYA, t, p.x, p.y);
-}
 
 long gridFor(long n) { return (n + kBlock - 1) / kBlock; }
 
@@ -981,12 +969,6 @@ void launchScatter(const DevDesign<T>& d, const GridGeom<T>& g, T qscale,
     SPHIP_CHECK(hipGetLastError());
 }
 
-template <typename T>
-void launchRecRefresh(const DevDesign<T>& d, hipStream_t s) {
-    if (d.nDens == 0) return;
-    launchKernel(k_rec_refresh<T>, gridFor(d.nDens), kBlock, 0, s, d);
-    SPHIP_CHECK(hipGetLastError());
-}
 
 template <typename T>
 void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl, hipStream_t s) {
@@ -1064,7 +1046,6 @@ void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, dou
     template void launchScatter<T>(const DevDesign<T>&, const GridGeom<T>&, T,                   \
                                    unsigned long long*, unsigned long long*, bool, ReduceWs&,    \
                                    const Ctl*, hipStream_t);                                                 \
-    template void launchRecRefresh<T>(const DevDesign<T>&, hipStream_t);                        \
     template void launchTileSort<T>(const DevDesign<T>&, const GridGeom<T>&, const Ctl*,         \
                                     hipStream_t);                                                \
     template void launchFinalize<T>(const GridGeom<T>&, const TileGrid&, unsigned long long*,     \
