@@ -26,6 +26,16 @@ struct HipError : std::runtime_error {
 
 constexpr int kBlock = 256;   // threads per block for the elementwise/gather kernels
 
+// MI355X deals a launch's workgroups to its 8 XCDs in turn (workgroup b to
+// XCD b % 8), and each XCD has its own L2.  Logical block
+// (b % 8) (n / 8) + b / 8 gives XCD x the logical blocks [x n/8, (x+1) n/8):
+// a contiguous share of whatever the blocks index.  Bijective on [0, n)
+// when 8 divides n; the identity otherwise.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcdBlock(int b, int n) {
+    return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
+}
+
 // Device bytes allocated by this thread through the library's allocation
 // helpers: an instance's footprint is the difference across its set-up.
 inline thread_local size_t g_devBytes = 0;

@@ -347,7 +347,7 @@ public:
         counts[BK_FINALIZE] = static_cast<int>(
             (std::max<long>(static_cast<long>(nb_), tg.nKeys) + kBlock - 1) / kBlock);
         counts[BK_WA] = dd_.waBlocks;
-        counts[BK_NODE] = (n_ + kBlock - 1) / kBlock;
+        counts[BK_NODE] = nodeBlocks(dd_.nCells, n_);
         counts[BK_REDUCE] = 1;
         counts[BK_UPDATE] = counts[BK_NODE];
         if (dd_.nDens == 0) counts[BK_SORT_COUNT] = counts[BK_SORT_PLACE] = 0;
@@ -641,6 +641,8 @@ void Engine<T>::build(const sp_hip_design& d) {
             if (stage == 2 && isMacro(i)) continue;
             optIdxHost_.push_back(i);
         }
+    dd_.nCells = static_cast<int>(optIdxHost_.size());
+    dd_.cellBlocks = nodePartBlocks(dd_.nCells);
     for (int i = d.num_nodes; i < total; ++i) optIdxHost_.push_back(i);
     n_ = static_cast<int>(optIdxHost_.size());
 
@@ -905,6 +907,38 @@ void Engine<T>::build(const sp_hip_design& d) {
             if (fused > 0) resident = std::min(resident, fused);
         }
         dd_.waBlocks = waBlocksFor(dd_.waWaveBase[kWaClasses + 1], resident);
+        // tasks by XCD: the XCD that runs the node-pass block of the lowest
+        // node of the task's first net (block b of a part runs logical block
+        // xcdBlock(b, part) on XCD b % kXcds), each XCD's tasks longest first
+        // (class order reversed); nodes outside the optimised set by index
+        const int tasks = dd_.waWaveBase[kWaClasses + 1];
+        const int cellShare = std::max(1, dd_.cellBlocks / kXcds);
+        const int fillShare = std::max(1, nodePartBlocks(n_ - dd_.nCells) / kXcds);
+        std::vector<int> kOfNode(total, -1);
+        for (int k = 0; k < n_; ++k) kOfNode[optIdxHost_[k]] = k;
+        auto xcdOfNode = [&](int i) {
+            const int k = kOfNode[i];
+            if (k < 0) return std::min(kXcds - 1, static_cast<int>(static_cast<long>(i) * kXcds / total));
+            if (k < dd_.nCells) return std::min(kXcds - 1, (k / kBlock) / cellShare);
+            return std::min(kXcds - 1, ((k - dd_.nCells) / kBlock) / fillShare);
+        };
+        std::vector<std::vector<int>> perXcd(kXcds);
+        for (int gw = tasks - 1; gw >= 0; --gw) {
+            int c = 0;
+            while (c <= kWaClasses && gw >= dd_.waWaveBase[c + 1]) ++c;
+            const int first = dd_.waNetLo[c] + (c < kWaClasses ? (gw - dd_.waWaveBase[c]) * waNetsPerTask(c)
+                                                               : gw - dd_.waWaveBase[c]);
+            const int node = netMinNode[netOrder[std::min(first, d.num_nets - 1)]];
+            perXcd[node >= 0 && node < total ? xcdOfNode(node) : 0].push_back(gw);
+        }
+        std::vector<int> order;
+        order.reserve(tasks);
+        dd_.waXcdStart[0] = 0;
+        for (int x = 0; x < kXcds; ++x) {
+            order.insert(order.end(), perXcd[x].begin(), perXcd[x].end());
+            dd_.waXcdStart[x + 1] = static_cast<int>(order.size());
+        }
+        arena_.upload(&dd_.waTaskOrder, order);
     }
     // node-ordered pin entries; slot -> entry for the gradient writes.  Entries
     // of pins that no net pass writes (no net, degree < 2 or above
@@ -968,7 +1002,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     // partial per region, see ReduceWs)
     const long maxBlocks = std::max<long>(
         {(static_cast<long>(nb_) + kBlock - 1) / kBlock, (dd_.waWaveBase[kWaClasses + 1] + 3L) / 4,
-         (n_ + kBlock - 1L) / kBlock,
+         static_cast<long>(nodeBlocks(dd_.nCells, n_)),
          static_cast<long>(dd_.tiles.tilesX) * dd_.tiles.tilesY +
              (dd_.nDens + dd_.tiles.chunk - 1) / dd_.tiles.chunk,
          1L});
@@ -1029,7 +1063,7 @@ void Engine<T>::build(const sp_hip_design& d) {
     bytesPer_[SP_HIP_K_REDUCE] =
         8.0 * ((nb_ + kBlock - 1) / kBlock +
                2.0 * dd_.waBlocks +
-               7.0 * ((n_ + kBlock - 1) / kBlock));
+               7.0 * nodeBlocks(dd_.nCells, n_));
 }
 
 template <typename T>

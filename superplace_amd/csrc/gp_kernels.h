@@ -38,6 +38,8 @@ struct DevDesign {
     // opt set: movable nodes, then the fillers past the fixed nodes),
     // optAffine is set and the kernels compute i instead of loading it.
     int optAffine = 0, optSplit = 0, optShift = 0;
+    int nCells = 0;        // optimised nodes that are not fillers (opt indices [0, nCells))
+    int cellBlocks = 0;    // blocks of the cell part of the node-pass and update grids
     // The density nodes in spatial order (by tile of their centre bin, tiles
     // in Morton order) as records: node index with the filler / macro flags
     // in bits 30 / 29 (recId) and (w, h) interleaved (recWH).  The sort and
@@ -76,6 +78,11 @@ struct DevDesign {
     // waWaveBase[c]; waWaveBase[kWaClasses + 1] is the total.
     int waNetLo[8];
     int waWaveBase[8];
+    // Wavefront tasks grouped by XCD: XCD x takes waTaskOrder[waXcdStart[x],
+    // waXcdStart[x + 1]) (tasks whose nets start in its share of the nodes,
+    // longest first), on its blocks b = x, x + kXcds, ... of the WA pass.
+    const int* waTaskOrder = nullptr;
+    int waXcdStart[kXcds + 1];
     int waBlocks = 1;     // blocks of every WA pass of the instance (fixes its partials' order)
     long waLaneBase[8];   // first lane-order entry of class c (c <= kWaClasses)
     // node -> pin entries; pinGrad holds the WA gradient (x, y) of each entry,
@@ -90,14 +97,25 @@ constexpr int kWaLaneClasses = 2;    // classes 0, 1 (degree 2, 3-4): one lane p
 __host__ __device__ inline int waNetsPerTask(int c) { return c < kWaLaneClasses ? 64 : 64 >> (c + 1); }
 __host__ __device__ inline int waEntriesPerTask(int c) { return c < kWaLaneClasses ? 64 * (2 << c) : 64; }
 constexpr int kWaTailRegs = 8;       // pins per lane per batch of a tail net's extrema pass
+// The node pass and the update run over the optimised nodes in two parts,
+// the non-filler nodes (opt indices [0, m)) and the fillers ([m, n)), each
+// part on a multiple of kXcds blocks; xcdBlock inside each part gives every
+// XCD a contiguous share of the cells and one of the fillers.  The WA pass
+// writes the pin entries of a share of the cells from the same XCD, so the
+// node pass reads them from that XCD's L2, and every XCD has cells.
+inline int nodePartBlocks(int count) { return roundUp(ceilDiv(count, kBlock), kXcds); }
+inline int nodeBlocks(int m, int n) {
+    return std::max(kXcds, nodePartBlocks(m) + nodePartBlocks(n - m));
+}
 constexpr int kWaMaxBlocks = 2048;   // the WA pass strides its wavefront tasks over at most these
 // Blocks of the WA pass for `waves` wavefront tasks (kBlock threads each),
 // at most `resident` (the blocks the GPU holds at once, so that no block
-// waits for another to end before it starts).
+// waits for another to end before it starts), a multiple of kXcds (every
+// XCD takes the tasks of its share of the nodes).
 inline int waBlocksFor(int waves, int resident = kWaMaxBlocks) {
-    const int b = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    const int cap = resident < kWaMaxBlocks ? resident : kWaMaxBlocks;
-    return b < 1 ? 1 : (b > cap ? (cap < 1 ? 1 : cap) : b);
+    const int b = roundUp(std::max(1, ceilDiv(waves, kBlock / 64)), kXcds);
+    const int cap = std::max(kXcds, std::min(resident, kWaMaxBlocks) / kXcds * kXcds);
+    return std::min(b, cap);
 }
 // Blocks of the WA pass's kernel (k_wa_pins) one GPU holds at once.
 template <typename T>

@@ -436,6 +436,29 @@ __device__ inline int optNode(const DevDesign<T>& d, int k) {
     return d.optIdx[k];
 }
 
+// Block b of a node-pass or update grid of nblocks blocks (nodeBlocks):
+// its first opt index, the end of its part, and its logical block (its
+// partial's slot).  Cells take blocks [0, cellBlocks), fillers the rest;
+// xcdBlock orders each part.
+struct NodeRange {
+    int k0, end, slot;
+};
+template <typename T>
+__device__ inline NodeRange nodeRange(const DevDesign<T>& d, int b, int nblocks) {
+    NodeRange r;
+    if (b < d.cellBlocks) {
+        r.slot = xcdBlock(b, d.cellBlocks);
+        r.k0 = r.slot * kBlock;
+        r.end = d.nCells;
+    } else {
+        const int f = xcdBlock(b - d.cellBlocks, nblocks - d.cellBlocks);
+        r.slot = d.cellBlocks + f;
+        r.k0 = d.nCells + f * kBlock;
+        r.end = d.nOpt;
+    }
+    return r;
+}
+
 // Loads are ordered by dependence: the node's own data and the Nesterov
 // vectors first (one memory round trip), then its footprint's field bins and
 // its pin entries (a second; pin entries eight at a time).  The sums run in
@@ -444,7 +467,8 @@ template <typename T>
 __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                                     T invG, float lambda, const T* u, const T* up, const T* gp,
                                     T* grad, T* dbgForce, T* dbgWl, double* partials,
-                                    const Ctl* ctl, const BufTable<T>& bufs, int vb) {
+                                    const Ctl* ctl, const BufTable<T>& bufs, int b,
+                                    int nblocks) {
     if (ctl) {
         if (ctl->done) return;
         lambda = ctl->lambda;
@@ -454,10 +478,11 @@ __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g,
         gp = pickBuf(bufs, ctl->nxt[B_GP]);
         grad = pickBuf(bufs, ctl->nxt[B_G]);
     }
-    const int k = vb * kBlock + threadIdx.x;
+    const NodeRange nr = nodeRange(d, b, nblocks);
+    const int k = nr.k0 + threadIdx.x;
     const int n = d.nOpt;
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (k < n) {
+    if (k < nr.end) {
         const int i = optNode(d, k);
         const double u0 = u[k], u1 = u[n + k];
         const double up0 = up ? static_cast<double>(up[k]) : 0.0;
@@ -556,7 +581,7 @@ __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g,
         v[5] = static_cast<double>(g0) * g0 + static_cast<double>(g1) * g1;
         v[6] = (isfinite(g0) ? 0.0 : 1.0) + (isfinite(g1) ? 0.0 : 1.0);
     }
-    blockPartial<7>(v, partials, vb);
+    blockPartial<7>(v, partials, nr.slot);
 }
 
 template <typename T>
@@ -566,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DevDesign<T> d, GridGeom<T
                                                       T* dbgForce, T* dbgWl, double* partials,
                                                       const Ctl* ctl, BufTable<T> bufs) {
     nodeGradBody<T>(d, g, exy, invG, lambda, u, up, gp, grad, dbgForce, dbgWl, partials, ctl,
-                    bufs, blockIdx.x);
+                    bufs, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------------ (d) decisions
@@ -826,15 +851,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceWs ws, double denom, in
 template <typename T>
 __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G, const T* B,
                                   T alpha, T m, T* vOut, T* uOut, const Ctl* ctl,
-                                  const BufTable<T>& bufs, int vb) {
-    const int k = vb * kBlock + threadIdx.x;
+                                  const BufTable<T>& bufs, int b, int nblocks) {
+    const NodeRange nr = nodeRange(d, b, nblocks);
+    const int k = nr.k0 + threadIdx.x;
     const int n = d.nOpt;
     if (ctl) {
         // After `done` the mode and indices stay fixed, so a repeated update
         // rewrites the same values (reads never alias writes).
         const int mode = ctl->mode;
         if (mode == kModeNone) return;
-        if (k < n && ctl->snapWrite >= 0) {   // tau-ladder snapshot of the evaluated point
+        if (k < nr.end && ctl->snapWrite >= 0) {   // tau-ladder snapshot of the evaluated point
             const T* src = pickBuf(bufs, ctl->snapSrc);
             T* dst = pickBuf(bufs, ctl->snapWrite);
             dst[k] = src[k];
@@ -851,7 +877,7 @@ __device__ inline void updateBody(const DevDesign<T>& d, const T* A, const T* G,
             vOut = pickBuf(bufs, c[B_V]); uOut = pickBuf(bufs, c[B_U]);
         }
     }
-    if (k >= n) return;
+    if (k >= nr.end) return;
     const int i = optNode(d, k);
     const T hw = T(0.5) * d.w[i], hh = T(0.5) * d.h[i];
     const T eps = static_cast<T>(1e-3f);
@@ -879,7 +905,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update(DevDesign<T> d, const T* A, const T* G,
                                                    const T* B, T alpha, T m, T* vOut, T* uOut,
                                                    const Ctl* ctl, BufTable<T> bufs) {
-    updateBody<T>(d, A, G, B, alpha, m, vOut, uOut, ctl, bufs, blockIdx.x);
+    updateBody<T>(d, A, G, B, alpha, m, vOut, uOut, ctl, bufs, blockIdx.x, gridDim.x);
 }
 
 
@@ -905,14 +931,14 @@ __global__ __launch_bounds__(kBlock) void k_slot_batch(const SlotArgs<T>* __rest
         waBlockTask<T>(a.wa, m.y, m.z);
     } else if constexpr (KIND == BK_NODE) {
         nodeGradBody<T>(a.d, a.g, a.exy, T(0), 0.f, nullptr, nullptr, nullptr, nullptr, nullptr,
-                        nullptr, a.ws.region(3), a.ctl, a.bufs, m.y);
+                        nullptr, a.ws.region(3), a.ctl, a.bufs, m.y, m.z);
     } else if constexpr (KIND == BK_REDUCE) {
         reduceBody<T>(a.ws, a.denom, 1, a.ctl, a.rows);
         __syncthreads();
         if (threadIdx.x == 0) a.doneOut[m.x] = a.ctl->done;
     } else if constexpr (KIND == BK_UPDATE) {
         updateBody<T>(a.d, nullptr, nullptr, nullptr, T(0), T(0), nullptr, nullptr, a.ctl, a.bufs,
-                      m.y);
+                      m.y, m.z);
     }
 }
 
@@ -1019,7 +1045,7 @@ void launchNodeGrad(const DevDesign<T>& d, const GridGeom<T>& g, const T* exy,
                     T* dbgForce, T* dbgWl, ReduceWs& ws, const Ctl* ctl, const BufTable<T>& bufs,
                     hipStream_t s) {
     const T invG = invGammaOf<T>(gamma);
-    ws.blocks[3] = static_cast<int>(gridFor(d.nOpt));
+    ws.blocks[3] = nodeBlocks(d.nCells, d.nOpt);
     launchKernel(k_node_grad<T>, ws.blocks[3], kBlock, 0, s, d, g, exy, invG, lambda, u, up, gp,
                  grad, dbgForce, dbgWl, ws.region(3), ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
@@ -1036,8 +1062,8 @@ template <typename T>
 void launchUpdate(const DevDesign<T>& d, const T* A, const T* G, const T* B, double alpha,
                   double m, T* vOut, T* uOut, const Ctl* ctl, const BufTable<T>& bufs,
                   hipStream_t s) {
-    launchKernel(k_update<T>, gridFor(d.nOpt), kBlock, 0, s, d, A, G, B, static_cast<T>(alpha),
-                                                   static_cast<T>(m), vOut, uOut, ctl, bufs);
+    launchKernel(k_update<T>, nodeBlocks(d.nCells, d.nOpt), kBlock, 0, s, d, A, G, B,
+                 static_cast<T>(alpha), static_cast<T>(m), vOut, uOut, ctl, bufs);
     SPHIP_CHECK(hipGetLastError());
 }
 

@@ -745,16 +745,9 @@ __device__ inline void dctColsBlock(const T* __restrict__ F1, T* __restrict__ Hc
     }
 }
 
-// MI355X deals a launch's workgroups to its 8 XCDs in turn (workgroup b to
-// XCD b % 8), and each XCD has its own L2.  Pass B's blocks own adjacent
-// column groups whose row segments share cache lines; logical block
-// (b % 8) (n / 8) + b / 8 gives each XCD n / 8 adjacent groups, so the
-// segments of one line meet in one L2 instead of eight.  Bijective on
-// [0, n) when 8 divides n; the identity otherwise.
-constexpr int kXcds = 8;
-__device__ inline int xcdBlock(int b, int n) {
-    return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
-}
+// Pass B's blocks own adjacent column groups whose row segments share cache
+// lines; xcdBlock (common.h) gives each XCD n / 8 adjacent groups, so the
+// segments of one line meet in one L2 instead of eight.
 
 // MAXT: the largest block the instance is launched with (G teams of
 // fftThreads(nx) threads); blocks of at most 256 threads take the 256-thread

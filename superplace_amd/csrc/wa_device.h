@@ -397,12 +397,16 @@ __device__ inline void waBlockTask(const WaTask<T>& w, int block, int nblocks) {
     double v[2] = {0.0, 0.0};
     // wavefront tasks are strided over the blocks (a bounded number of block
     // partials); each task belongs to one size class
-    // tasks are taken from the end (the tail nets and the largest classes
-    // first), so the longest tasks start in the first round
-    const int tasks = d.waWaveBase[kWaClasses + 1];
-    for (int q = block * (kBlock / 64) + (threadIdx.x >> 6); q < tasks;
-         q += nblocks * (kBlock / 64)) {
-        const int gw = tasks - 1 - q;
+    // Block b serves XCD b % kXcds (the XCD that runs it in a launch of its
+    // own or behind a multiple of kXcds other blocks): the tasks of the nets
+    // that start in that XCD's share of the nodes, longest first, so the
+    // pin entries of a share are written from one L2 and the node pass,
+    // which gives each XCD the same share, reads them there.
+    const int x = block % kXcds, nbx = nblocks / kXcds;
+    const int hi = d.waXcdStart[x + 1];
+    for (int q = d.waXcdStart[x] + (block / kXcds) * (kBlock / 64) + (threadIdx.x >> 6); q < hi;
+         q += nbx * (kBlock / 64)) {
+        const int gw = d.waTaskOrder[q];
         int c = 0;
         while (c <= kWaClasses && gw >= d.waWaveBase[c + 1]) ++c;
         static_assert(kWaLaneClasses == 2, "classes 0 and 1 take one lane per net");
