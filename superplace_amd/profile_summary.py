@@ -102,10 +102,14 @@ def trace(args):
     full = [s for s in solves if len(s) == 3 and (s[0][1] - s[0][0]) > 3000]
     if full:
         sums = [sum((t1 - t0) / 1e3 for t0, t1, _ in s) for s in full]
+        spans = [(s[-1][1] - s[0][0]) / 1e3 for s in full]
         out.append("")
         out.append(f"Poisson solve = the 3 FFT pass launches of one evaluation. Over {len(full)} "
                    f"solves (early exits after convergence excluded): summed kernel durations "
-                   f"mean {statistics.mean(sums):.2f} us, median {statistics.median(sums):.2f} us.")
+                   f"mean {statistics.mean(sums):.2f} us, median {statistics.median(sums):.2f} us; "
+                   f"first start to last end (the span a HIP event pair around the solve "
+                   f"times) mean {statistics.mean(spans):.2f} us, median "
+                   f"{statistics.median(spans):.2f} us.")
     print("\n".join(out))
 
 
