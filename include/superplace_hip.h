@@ -34,7 +34,14 @@ typedef enum sp_hip_status {
     SP_HIP_EINTERNAL = 6
 } sp_hip_status;
 
-enum { SP_HIP_FLAG_FP64 = 1 };   /* FP64 arithmetic everywhere (the plain FP64 path) */
+enum {
+    SP_HIP_FLAG_FP64 = 1,    /* FP64 arithmetic everywhere (the plain FP64 path) */
+    SP_HIP_FLAG_BF16X3 = 2,  /* Poisson handles and sp_hip_gemm: the dense transforms as
+                                split-BF16 products on the BF16 matrix cores (each FP32
+                                operand as hi + lo bfloat16, three products); FP32 only */
+    SP_HIP_FLAG_DENSE = 4    /* Poisson handles: the dense transforms on every grid size
+                                (power-of-two grids otherwise take the FFT passes) */
+};
 
 /* Thread-local message of the last failing call on this thread. */
 const char* sp_hip_last_error(void);
@@ -51,7 +58,8 @@ int sp_hip_device_count(int* count);
  * All buffers are allocated at create time; solve does not allocate. */
 typedef struct sp_hip_poisson sp_hip_poisson;
 
-/* BackendConfig{nx, ny, step_x, step_y}; flags: SP_HIP_FLAG_FP64.
+/* BackendConfig{nx, ny, step_x, step_y}; flags: SP_HIP_FLAG_FP64, or
+ * SP_HIP_FLAG_DENSE and / or SP_HIP_FLAG_BF16X3 (FP32) to choose the transforms.
  * nx, ny >= 2 (any size: the transforms are padded internally).
  * Returns SP_HIP_ENODEV when no GPU is present. */
 int sp_hip_poisson_create(int nx, int ny, float step_x, float step_y, int flags,
@@ -509,7 +517,8 @@ int sp_rng_uniform(uint64_t seed, int64_t count, float lo, float hi, float* out)
 /* Batched C = A * B^T of row-major matrices (A m x k, B n x k, C m x n; dims
  * multiples of 64) on the matrix cores, the primitive of the Poisson
  * transforms.  Device pointers, float (or double with SP_HIP_FLAG_FP64);
- * synchronous.  No counterpart in the reference (FFTW / cuFFT there). */
+ * SP_HIP_FLAG_BF16X3: float, as split-BF16 products.  Synchronous.  No
+ * counterpart in the reference (FFTW / cuFFT there). */
 int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b,
                 void* const* c, int m, int n, int k);
 

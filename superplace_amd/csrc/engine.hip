@@ -1694,13 +1694,13 @@ public:
 template <typename T>
 class PoissonHandle final : public PoissonHandleBase {
 public:
-    PoissonHandle(int nx, int ny, float sx, float sy) : nx_(nx), ny_(ny) {
+    PoissonHandle(int nx, int ny, float sx, float sy, unsigned opts = 0) : nx_(nx), ny_(ny) {
         requireDevice();
         if (nx < 2 || ny < 2) throw InvalidArg("Poisson: grid must be at least 2x2");
         if (!(sx > 0.f) || !(sy > 0.f)) throw InvalidArg("Poisson: bin step must be positive");
         SPHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
         try {
-            plan_ = std::make_unique<PoissonPlan<T>>(nx, ny, sx, sy, s_);
+            plan_ = std::make_unique<PoissonPlan<T>>(nx, ny, sx, sy, s_, false, nullptr, opts);
             const size_t n = static_cast<size_t>(nx) * ny;
             SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hIn_), n * sizeof(T), hipHostMallocDefault));
             SPHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hOut_), 3 * n * sizeof(T), hipHostMallocDefault));
@@ -1754,6 +1754,7 @@ public:
 
     const char* name() const override {
         if (plan_->usesFft()) return sizeof(T) == 4 ? "hip-fft" : "hip-fft-fp64";
+        if (plan_->usesBf16x3()) return "hip-mfma-bf16x3";
         return sizeof(T) == 4 ? "hip-mfma" : "hip-mfma-fp64";
     }
 
@@ -2105,10 +2106,14 @@ int sp_hip_poisson_create(int nx, int ny, float step_x, float step_y, int flags,
         if (!out) throw sphip::InvalidArg("null output handle");
         *out = nullptr;
         auto h = std::make_unique<sp_hip_poisson>();
+        if ((flags & SP_HIP_FLAG_FP64) && (flags & SP_HIP_FLAG_BF16X3))
+            throw sphip::InvalidArg("Poisson: split-BF16 products are FP32 only");
+        const unsigned opts = ((flags & SP_HIP_FLAG_DENSE) ? sphip::kPlanDense : 0u) |
+                              ((flags & SP_HIP_FLAG_BF16X3) ? sphip::kPlanBf16x3 : 0u);
         if (flags & SP_HIP_FLAG_FP64)
-            h->impl = std::make_unique<sphip::PoissonHandle<double>>(nx, ny, step_x, step_y);
+            h->impl = std::make_unique<sphip::PoissonHandle<double>>(nx, ny, step_x, step_y, opts);
         else
-            h->impl = std::make_unique<sphip::PoissonHandle<float>>(nx, ny, step_x, step_y);
+            h->impl = std::make_unique<sphip::PoissonHandle<float>>(nx, ny, step_x, step_y, opts);
         *out = h.release();
     });
 }
@@ -2281,6 +2286,8 @@ int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b
         if (!a || !b || !c || batch < 1 || batch > 4) throw sphip::InvalidArg("bad gemm batch");
         if (m <= 0 || n <= 0 || k <= 0 || m % 64 || n % 64 || k % 64)
             throw sphip::InvalidArg("gemm dims must be positive multiples of 64");
+        if ((flags & SP_HIP_FLAG_FP64) && (flags & SP_HIP_FLAG_BF16X3))
+            throw sphip::InvalidArg("gemm: split-BF16 products are FP32 only");
         if (flags & SP_HIP_FLAG_FP64)
             sphip::launchGemm<double>(batch, reinterpret_cast<const double* const*>(a),
                                       reinterpret_cast<const double* const*>(b),
@@ -2288,7 +2295,8 @@ int sp_hip_gemm(int flags, int batch, const void* const* a, const void* const* b
         else
             sphip::launchGemm<float>(batch, reinterpret_cast<const float* const*>(a),
                                      reinterpret_cast<const float* const*>(b),
-                                     reinterpret_cast<float* const*>(c), m, n, k, k, k, n, nullptr);
+                                     reinterpret_cast<float* const*>(c), m, n, k, k, k, n, nullptr,
+                                     (flags & SP_HIP_FLAG_BF16X3) != 0);
         SPHIP_CHECK(hipDeviceSynchronize());
     });
 }

@@ -46,6 +46,11 @@ struct PoissonBatchArgs {
     const int* skip;    // the instance's done flag
 };
 
+// PoissonPlan options: kPlanDense runs the dense transforms on every grid
+// (power-of-two grids too, which otherwise take the FFT passes); kPlanBf16x3
+// runs the dense products as split-BF16 products (FP32 plans only).
+enum : unsigned { kPlanDense = 1u, kPlanBf16x3 = 2u };
+
 template <typename T>
 class PoissonPlan {
 public:
@@ -53,9 +58,9 @@ public:
     // (element 2 (x * ld + y) + axis) instead of fieldX() / fieldY().
     // arena: the plan's arrays are reserved there (and the arena owns them);
     // they exist once the caller has placed the arena.  Without, the plan
-    // allocates them and is ready on return.
+    // allocates them and is ready on return.  opts: kPlan* bits.
     PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
-                bool interleaved = false, DevArena* arena = nullptr);
+                bool interleaved = false, DevArena* arena = nullptr, unsigned opts = 0);
     ~PoissonPlan();
     PoissonPlan(const PoissonPlan&) = delete;
     PoissonPlan& operator=(const PoissonPlan&) = delete;
@@ -100,6 +105,7 @@ public:
     double bytes(bool withPhi) const;
     // Power-of-two grids (4 .. 1024 per side) take the FFT path.
     bool usesFft() const { return fft_; }
+    bool usesBf16x3() const { return x3_; }
     // The plan's buffers for a batched solve (interleaved FFT plans only).
     PoissonBatchArgs<T> batchArgs(const int* skip) const;
     static bool fftShape(int n);
@@ -119,6 +125,7 @@ private:
     // FFT path: per axis the twiddle factor tables (complex, interleaved);
     // LDS bytes of the three passes
     bool fft_ = false;
+    bool x3_ = false;   // dense products as split-BF16 products
     double *tabX_ = nullptr, *tabY_ = nullptr;
     size_t ldsA_ = 0, ldsB_ = 0, ldsC_ = 0, ldsBW_ = 0;
     int colPairs_ = 1;   // column pairs per block of the column pass
@@ -161,9 +168,10 @@ inline int poissonBatchBlocks(int pass, int nx, int ny) {
 }
 
 // Batched C = A * B^T (A M x K, B N x K, row-major, dims multiples of 64) on
-// MFMA; the plan's primitive, exposed for the GEMM unit test.
+// MFMA; the plan's primitive, exposed for the GEMM unit test.  x3: as
+// split-BF16 products (T = float only).
 template <typename T>
 void launchGemm(int batch, const T* const* A, const T* const* B, T* const* C, int M, int N,
-                int K, int lda, int ldb, int ldc, hipStream_t stream);
+                int K, int lda, int ldb, int ldc, hipStream_t stream, bool x3 = false);
 
 }  // namespace sphip

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "poisson.h"
@@ -187,6 +188,129 @@ __global__ __launch_bounds__(256) void k_gemm(GemmBatch<T> batch, int M, int N, 
                                  : T(0);
             } else {
                 val = T(0);
+            }
+        }
+        if (TRANS)
+            C[(static_cast<long>(gc) * ldc + gr) * cstep] = val;
+        else
+            C[(static_cast<long>(gr) * ldc + gc) * cstep] = val;
+    }
+}
+
+// ------------------------------------------------------------ split-BF16 products
+// C = A * Bt^T as k_gemm<float>, on the BF16 matrix cores: every FP32
+// operand x is split into hi = bf16(x) and lo = bf16(x - hi) (round to
+// nearest even; |x - hi - lo| <= 2^-17 |x| roughly), and each product is
+// a_hi b_hi + a_hi b_lo + a_lo b_hi: three v_mfma_f32_16x16x32_bf16 per 32 k,
+// accumulated in FP32 over a panel of 256 k, then in FP64.  The dropped
+// a_lo b_lo and the split residues leave about 2^-16 relative error per
+// product.  Lane layout of the 16x16x32 instruction (measured on gfx950):
+// lane l supplies A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16],
+// j < 8, and holds C[4 (l / 16) + e][l % 16], e < 4, as the f32 16x16x4 form.
+// The panels are split while they are copied into LDS (hi and lo arrays,
+// 32 rows of 256 k, rows padded by 16 bytes).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+struct X3Geom {
+    static constexpr int KP = 256;        // panel k columns
+    static constexpr int LD = KP + 8;     // padded row, in bf16
+    static constexpr int ROWS = 32;
+    static constexpr unsigned kLds = 4u * ROWS * LD * sizeof(__bf16);   // A hi, A lo, B hi, B lo
+};
+
+__device__ inline void splitToLds(const float4& v, __bf16* hi, __bf16* lo) {
+    bf16x4 h, l;
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h[i] = static_cast<__bf16>(x[i]);
+        l[i] = static_cast<__bf16>(x[i] - static_cast<float>(h[i]));
+    }
+    *reinterpret_cast<bf16x4*>(hi) = h;
+    *reinterpret_cast<bf16x4*>(lo) = l;
+}
+
+template <int EPI, bool TRANS>
+__global__ __launch_bounds__(256) void k_gemm_x3(GemmBatch<float> batch, int M, int N, int K,
+                                                 int lda, int ldb, int ldc, int cstep,
+                                                 SpectralEpi<float> epi, const int* skip) {
+    if (skip && *skip) return;   // device loop finished: uniform early exit
+    using G = X3Geom;
+    constexpr int KP = G::KP, LD = G::LD, ROWS = G::ROWS;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __bf16* Ah = reinterpret_cast<__bf16*>(smem);
+    __bf16* Al = Ah + ROWS * LD;
+    __bf16* Bh = Al + ROWS * LD;
+    __bf16* Bl = Bh + ROWS * LD;
+
+    const float* __restrict__ A = batch.A[blockIdx.z];
+    const float* __restrict__ Bt = batch.B[blockIdx.z];
+    float* __restrict__ C = batch.C[blockIdx.z];
+    const int m0 = blockIdx.y * ROWS, n0 = blockIdx.x * ROWS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int aOff = (wm * 16 + r) * LD + 8 * q, bOff = (wn * 16 + r) * LD + 8 * q;
+
+    double acc64[4] = {0.0, 0.0, 0.0, 0.0};
+    // each thread copies 8 float4 of each operand per full panel
+    constexpr int kPer = ROWS * KP / 4 / 256;
+    for (int kc = 0; kc < K; kc += KP) {
+        const int kw = K - kc < KP ? K - kc : KP;   // multiple of 64
+        const int v4 = kw / 4;                      // float4 per panel row
+        if (kc > 0) __syncthreads();                // previous panel fully read
+        float4 av[kPer], bv[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = threadIdx.x + 256 * i, row = e / v4, c4 = e - row * v4;
+            if (row < ROWS) {
+                av[i] = *reinterpret_cast<const float4*>(A + static_cast<long>(m0 + row) * lda +
+                                                         kc + 4 * c4);
+                bv[i] = *reinterpret_cast<const float4*>(Bt + static_cast<long>(n0 + row) * ldb +
+                                                         kc + 4 * c4);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = threadIdx.x + 256 * i, row = e / v4, c4 = e - row * v4;
+            if (row < ROWS) {
+                splitToLds(av[i], Ah + row * LD + 4 * c4, Al + row * LD + 4 * c4);
+                splitToLds(bv[i], Bh + row * LD + 4 * c4, Bl + row * LD + 4 * c4);
+            }
+        }
+        __syncthreads();
+        vec4<float> acc = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < kw; k0 += 32) {
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(Ah + aOff + k0);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(Al + aOff + k0);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(Bh + bOff + k0);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(Bl + bOff + k0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc64[e] += static_cast<double>(acc[e]);
+    }
+
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int row = wm * 16 + 4 * q + e;
+        const int col = wn * 16 + r;
+        const int gr = m0 + row, gc = n0 + col;
+        const double full = acc64[e];
+        float val = static_cast<float>(full);
+        if (EPI == kEpiSpectral) {
+            // P = F / (4 nx ny) / (u_j^2 + v_k^2), P[0][0] = 0 (cpu_poisson.cpp:163-173)
+            if (gr < epi.nx && gc < epi.ny) {
+                const float uj = epi.u[gr], vk = epi.v[gc];
+                const float den = uj * uj + vk * vk;
+                val = den > 0.f ? static_cast<float>(full * static_cast<double>(epi.invNorm) /
+                                                     static_cast<double>(den))
+                                : 0.f;
+            } else {
+                val = 0.f;
             }
         }
         if (TRANS)
@@ -877,9 +1001,27 @@ __global__ __launch_bounds__(kBlock) void k_idct_rows(const T* __restrict__ Hc,
 template <typename T, int EPI, bool TRANS>
 void gemmDispatch(int batch, const GemmBatch<T>& gb, int M, int N, int K, int lda, int ldb,
                   int ldc, const SpectralEpi<T>& epi, hipStream_t s, const int* skip = nullptr,
-                  int cstep = 1) {
+                  int cstep = 1, bool x3 = false) {
     if (M % 64 || N % 64 || K % 64) throw std::runtime_error("gemm: dims must be multiples of 64");
     if (batch < 1 || batch > 4) throw std::runtime_error("gemm: batch must be 1..4");
+    if constexpr (std::is_same<T, float>::value) {
+        if (x3) {
+            constexpr unsigned shm = X3Geom::kLds;
+            static std::once_flag attrOnce;
+            std::call_once(attrOnce, [] {
+                SPHIP_CHECK(hipFuncSetAttribute(
+                    reinterpret_cast<const void*>(k_gemm_x3<EPI, TRANS>),
+                    hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+            });
+            dim3 grid(N / 32, M / 32, batch);
+            launchKernel(k_gemm_x3<EPI, TRANS>, grid, 256, shm, s, gb, M, N, K, lda, ldb, ldc,
+                         cstep, epi, skip);
+            SPHIP_CHECK(hipGetLastError());
+            return;
+        }
+    } else {
+        if (x3) throw std::runtime_error("gemm: split-BF16 products are FP32 only");
+    }
     // the panels need more than the default 64 KB of LDS; set once per
     // instantiation (several threads may solve at once)
     constexpr unsigned shm = 2 * PanelGeom<T>::ROWS * PanelGeom<T>::LD * sizeof(T);
@@ -915,23 +1057,27 @@ T* upload(const std::vector<double>& h, hipStream_t s) {
 
 template <typename T>
 void launchGemm(int batch, const T* const* A, const T* const* B, T* const* C, int M, int N,
-                int K, int lda, int ldb, int ldc, hipStream_t stream) {
+                int K, int lda, int ldb, int ldc, hipStream_t stream, bool x3) {
     GemmBatch<T> gb{};
     for (int i = 0; i < batch && i < 4; ++i) {
         gb.A[i] = A[i];
         gb.B[i] = B[i];
         gb.C[i] = C[i];
     }
-    gemmDispatch<T, kEpiNone, false>(batch, gb, M, N, K, lda, ldb, ldc, SpectralEpi<T>{}, stream);
+    gemmDispatch<T, kEpiNone, false>(batch, gb, M, N, K, lda, ldb, ldc, SpectralEpi<T>{}, stream,
+                                     nullptr, 1, x3);
 }
 
 template <typename T>
 PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStream_t stream,
-                            bool interleaved, DevArena* arena)
+                            bool interleaved, DevArena* arena, unsigned opts)
     : nx_(nx), ny_(ny), mx_(roundUp(nx, 64)), my_(roundUp(ny, 64)), stream_(stream),
       interleaved_(interleaved), ownsMem_(arena == nullptr), stepX_(stepX), stepY_(stepY) {
     if (nx < 2 || ny < 2) throw std::invalid_argument("Poisson grid must be at least 2x2");
     if (!(stepX > 0) || !(stepY > 0)) throw std::invalid_argument("bin step must be positive");
+    if ((opts & kPlanBf16x3) && sizeof(T) != 4)
+        throw std::invalid_argument("split-BF16 products are FP32 only");
+    x3_ = (opts & kPlanBf16x3) != 0;
     const double pi = 3.14159265358979323846;
     const size_t gx = static_cast<size_t>(mx_) * my_;
     // an array with host contents, and a zero-filled one
@@ -947,7 +1093,7 @@ PoissonPlan<T>::PoissonPlan(int nx, int ny, double stepX, double stepY, hipStrea
             SPHIP_CHECK(hipMemsetAsync(*dst, 0, count * sizeof(T), stream));
         }
     };
-    fft_ = fftShape(nx) && fftShape(ny);
+    fft_ = fftShape(nx) && fftShape(ny) && !(opts & kPlanDense);
     if (fft_) {
         // the factor tables of exp(-2 pi i t / 4L) (Twid): lo, then hi
         auto tables = [&](int L) {
@@ -1216,30 +1362,32 @@ void PoissonPlan<T>::solve(bool withPhi, const int* skip) {
     {   // T0^T = (rho * Cy^T)^T
         GemmBatch<T> gb{};
         gb.A[0] = rho_; gb.B[0] = cyT_; gb.C[0] = t0_;
-        gemmDispatch<T, kEpiNone, true>(1, gb, M, N, N, N, N, M, none, stream_, skip);
+        gemmDispatch<T, kEpiNone, true>(1, gb, M, N, N, N, N, M, none, stream_, skip, 1, x3_);
     }
     {   // P = spectral(Cx * T0)
         GemmBatch<T> gb{};
         gb.A[0] = cx_; gb.B[0] = t0_; gb.C[0] = p_;
         SpectralEpi<T> epi{u_, v_, nx_, ny_, invNorm_};
-        gemmDispatch<T, kEpiSpectral, false>(1, gb, M, N, M, M, M, N, epi, stream_, skip);
+        gemmDispatch<T, kEpiSpectral, false>(1, gb, M, N, M, M, M, N, epi, stream_, skip, 1,
+                                             x3_);
     }
     {   // T1^T = (P * Ay^T)^T, T2^T = (P * Bsv)^T
         GemmBatch<T> gb{};
         gb.A[0] = p_; gb.B[0] = ayT_; gb.C[0] = t1_;
         gb.A[1] = p_; gb.B[1] = bsv_; gb.C[1] = t2_;
-        gemmDispatch<T, kEpiNone, true>(2, gb, M, N, N, N, N, M, none, stream_, skip);
+        gemmDispatch<T, kEpiNone, true>(2, gb, M, N, N, N, N, M, none, stream_, skip, 1, x3_);
     }
     {   // xi_x = Sxu * T1, xi_y = Ax * T2 (interleaved or not), phi = Ax * T1
         GemmBatch<T> gb{};
         gb.A[0] = sxu_; gb.B[0] = t1_; gb.C[0] = interleaved_ ? exy_ : ex_;
         gb.A[1] = ax_;  gb.B[1] = t2_; gb.C[1] = interleaved_ ? exy_ + 1 : ey_;
         gemmDispatch<T, kEpiNone, false>(2, gb, M, N, M, M, M, N, none, stream_, skip,
-                                         interleaved_ ? 2 : 1);
+                                         interleaved_ ? 2 : 1, x3_);
         if (withPhi) {
             GemmBatch<T> gp{};
             gp.A[0] = ax_; gp.B[0] = t1_; gp.C[0] = phi_;
-            gemmDispatch<T, kEpiNone, false>(1, gp, M, N, M, M, M, N, none, stream_, skip);
+            gemmDispatch<T, kEpiNone, false>(1, gp, M, N, M, M, M, N, none, stream_, skip, 1,
+                                             x3_);
         }
     }
 }
@@ -1368,8 +1516,8 @@ template class PoissonPlan<double>;
 template void launchPoissonBatch<float>(int, const PoissonBatchArgs<float>*, const int4*, int, int,
                                         int, int, hipStream_t);
 template void launchGemm<float>(int, const float* const*, const float* const*, float* const*,
-                                int, int, int, int, int, int, hipStream_t);
+                                int, int, int, int, int, int, hipStream_t, bool);
 template void launchGemm<double>(int, const double* const*, const double* const*,
-                                 double* const*, int, int, int, int, int, int, hipStream_t);
+                                 double* const*, int, int, int, int, int, int, hipStream_t, bool);
 
 }  // namespace sphip

@@ -27,6 +27,8 @@ HEADER = HERE.parent / "include" / "superplace_hip.h"
 
 SP_HIP_OK, SP_HIP_ENODEV, SP_HIP_EINVAL, SP_HIP_EOOM, SP_HIP_EDEVICE, SP_HIP_ENONFINITE = range(6)
 SP_HIP_FLAG_FP64 = 1
+SP_HIP_FLAG_BF16X3 = 2
+SP_HIP_FLAG_DENSE = 4
 KERNEL_FAMILIES = ["scatter", "finalize", "poisson", "wa_nets", "node_grad", "update", "sort",
                    "reduce"]
 
@@ -267,6 +269,10 @@ class BackendConfig:
     prefer_gpu: bool = True
     fftw_threads: int = 4
     fp64: bool = False
+    # the transforms (no counterpart in BackendConfig): dense products on
+    # every grid size, and / or those products as split-BF16 products (FP32)
+    dense: bool = False
+    bf16x3: bool = False
 
 
 class HipPoissonBackend:
@@ -278,8 +284,10 @@ class HipPoissonBackend:
     def __init__(self, cfg: BackendConfig):
         self.nx, self.ny = int(cfg.nx), int(cfg.ny)
         h = _P()
+        flags = ((SP_HIP_FLAG_FP64 if cfg.fp64 else 0) | (SP_HIP_FLAG_DENSE if cfg.dense else 0) |
+                 (SP_HIP_FLAG_BF16X3 if cfg.bf16x3 else 0))
         check(lib().sp_hip_poisson_create(self.nx, self.ny, float(cfg.step_x), float(cfg.step_y),
-                                          SP_HIP_FLAG_FP64 if cfg.fp64 else 0, C.byref(h)))
+                                          flags, C.byref(h)))
         self._h = h
 
     def _span(self, a, writable):
@@ -328,13 +336,13 @@ def make_poisson_backend(cfg: BackendConfig) -> HipPoissonBackend:
     return HipPoissonBackend(cfg)
 
 
-def gemm(a_ptrs, b_ptrs, c_ptrs, m, n, k, fp64=False):
+def gemm(a_ptrs, b_ptrs, c_ptrs, m, n, k, fp64=False, bf16x3=False):
     """Batched C = A B^T on device pointers (A m x k, B n x k, row-major,
-    multiples of 64)."""
+    multiples of 64); bf16x3: FP32 as split-BF16 products."""
     batch = len(a_ptrs)
     arr = lambda ps: (_P * batch)(*[_P(p) for p in ps])
-    check(lib().sp_hip_gemm(SP_HIP_FLAG_FP64 if fp64 else 0, batch, arr(a_ptrs), arr(b_ptrs),
-                            arr(c_ptrs), m, n, k))
+    flags = (SP_HIP_FLAG_FP64 if fp64 else 0) | (SP_HIP_FLAG_BF16X3 if bf16x3 else 0)
+    check(lib().sp_hip_gemm(flags, batch, arr(a_ptrs), arr(b_ptrs), arr(c_ptrs), m, n, k))
 
 
 # ------------------------------------------------------------------------ GP engine

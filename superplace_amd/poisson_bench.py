@@ -7,6 +7,11 @@ reference's CUDA backend), and one JSON line with the tool's keys plus
 "backend".
 
     python -m superplace_amd.poisson_bench [n=512] [workers=4] [rounds=100] [omp_threads=1]
+                                           [transforms=auto|dense|bf16x3]
+
+transforms (no counterpart in the tool): auto takes the FFT passes on
+power-of-two grids and the dense MFMA products elsewhere; dense takes the
+products on every grid; bf16x3 the products as split-BF16 products.
 """
 from __future__ import annotations
 
@@ -27,13 +32,17 @@ def density(n: int) -> np.ndarray:
     return rho.astype(np.float32).reshape(n, n)
 
 
-def run(n=512, workers=4, rounds=100, omp_threads=1) -> dict:
+def run(n=512, workers=4, rounds=100, omp_threads=1, transforms="auto") -> dict:
     if n < 2 or rounds < 1 or omp_threads < 1:
         raise ValueError("invalid benchmark dimensions/iterations/threads")
+    if transforms not in ("auto", "dense", "bf16x3"):
+        raise ValueError("transforms must be auto, dense or bf16x3")
     rho = density(n)
     out = [np.zeros((n, n), np.float32) for _ in range(3)]
     t0 = time.perf_counter()
-    be = hip.make_poisson_backend(hip.BackendConfig(n, n, 1.25, 0.75, fftw_threads=workers))
+    be = hip.make_poisson_backend(hip.BackendConfig(n, n, 1.25, 0.75, fftw_threads=workers,
+                                                    dense=transforms != "auto",
+                                                    bf16x3=transforms == "bf16x3"))
     plan_ms = (time.perf_counter() - t0) * 1e3
     for _ in range(10):
         be.solve(rho, *out)
@@ -50,7 +59,7 @@ def run(n=512, workers=4, rounds=100, omp_threads=1) -> dict:
 
 def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else argv
-    args = [int(a) for a in argv[:4]]
+    args = [int(a) for a in argv[:4]] + list(argv[4:5])
     try:
         print(json.dumps(run(*args), separators=(",", ":")))
     except Exception as e:   # the tool prints the message and exits 1

@@ -200,3 +200,63 @@ def test_poisson_bench_tool():
     assert r["grid"] == 64 and r["iterations"] == 5 and r["fftw_plan_threads"] == 0
     phi, _, _ = oracle.poisson(poisson_bench.density(64).astype(np.float64), 1.25, 0.75)
     assert abs(r["checksum"] - phi.sum()) <= 1e-4 * np.abs(phi).sum()
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (512, 512), (1024, 1024)])
+def test_dense_transforms_on_power_of_two_grids(shape):
+    """SP_HIP_FLAG_DENSE: the MFMA products on grids the FFT passes would
+    take, to the same criterion."""
+    nx, ny = shape
+    rho = np.random.default_rng(nx * 7 + ny).uniform(0, 1, shape).astype(np.float32)
+    be = hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 22.65625, 21.0, dense=True))
+    assert be.name() == "hip-mfma"
+    out = [np.zeros(shape, np.float32) for _ in range(3)]
+    be.solve(rho, *out)
+    ref = oracle.poisson(rho.astype(np.float64), 22.65625, 21.0)
+    errs = [rel(g, r) for g, r in zip(out, ref)]
+    print(shape, errs, flush=True)
+    assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(17, 29), (500, 500), (512, 512)])
+def test_split_bf16_products_error(shape):
+    """SP_HIP_FLAG_BF16X3: the dense products as hi/lo bfloat16 splits
+    (three products).  Two bfloat16 pieces carry about 16 bits, so the
+    fields land at 0.5-2e-5 of max|ref|, above the 1e-5 criterion that the
+    FP32 paths meet; the bound here is the measured one with a margin."""
+    nx, ny = shape
+    rho = np.random.default_rng(nx * 7 + ny).uniform(0, 1, shape).astype(np.float32)
+    be = hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 22.65625, 21.0, dense=True,
+                                                 bf16x3=True))
+    assert be.name() == "hip-mfma-bf16x3"
+    out = [np.zeros(shape, np.float32) for _ in range(3)]
+    be.solve(rho, *out)
+    ref = oracle.poisson(rho.astype(np.float64), 22.65625, 21.0)
+    errs = [rel(g, r) for g, r in zip(out, ref)]
+    print(shape, errs, flush=True)
+    assert max(errs) < 4e-5
+    with pytest.raises(hip.HipError):
+        hip.HipPoissonBackend(hip.BackendConfig(nx, ny, 1.0, 1.0, fp64=True, bf16x3=True))
+
+
+@pytest.mark.parametrize("mnk", [(192, 128, 320), (1024, 1024, 512)])
+def test_gemm_bf16x3_matches_numpy(mnk):
+    """Split-BF16 products: each FP32 operand as hi + lo bfloat16 leaves a
+    residue of about 2^-16 of it, so C is within a few 1e-6 of max|C| for
+    standard normal operands."""
+    from tests.hiprt import DevArray
+    rng = np.random.default_rng(4)
+    m, n, k = mnk
+    A = [rng.standard_normal((m, k)).astype(np.float32) for _ in range(2)]
+    B = [rng.standard_normal((n, k)).astype(np.float32) for _ in range(2)]
+    dA, dB = [DevArray(a) for a in A], [DevArray(b) for b in B]
+    dC = [DevArray.zeros((m, n), np.float32) for _ in range(2)]
+    hip.gemm([a.ptr for a in dA], [b.ptr for b in dB], [c.ptr for c in dC], m, n, k, bf16x3=True)
+    for a, b, c in zip(A, B, dC):
+        ref = a.astype(np.float64) @ b.astype(np.float64).T
+        err = np.abs(c.get().astype(np.float64) - ref).max() / np.abs(ref).max()
+        print(mnk, err, flush=True)
+        assert err < 2e-5, err
+    with pytest.raises(hip.HipError):
+        hip.gemm([a.ptr for a in dA], [b.ptr for b in dB], [c.ptr for c in dC], m, n, k,
+                 fp64=True, bf16x3=True)
