@@ -824,15 +824,6 @@ void Engine<T>::build(const sp_hip_design& d) {
         const size_t stageLen = static_cast<size_t>(tg.tilesX) * tg.tilesY * 2 * kTileWin * kTileWin;
         arena_.zeros(&tg.stage, stageLen);
     }
-    std::vector<double> parea(n_);
-    std::vector<float> pdeg(n_);
-    for (int k = 0; k < n_; ++k) {
-        const int i = optIdxHost_[k];
-        parea[k] = static_cast<double>(d.node_w[i]) * static_cast<double>(d.node_h[i]);
-        pdeg[k] = static_cast<float>(d.node2pin_start[i + 1] - d.node2pin_start[i]);
-    }
-    arena_.upload(&dd_.precArea, parea);
-    arena_.upload(&dd_.precDeg, pdeg);
     // Nets are renumbered by WA size class (degree 2, 3-4, 5-8, ..., then
     // larger), so each class is a contiguous range of nets and of pin slots,
     // and inside a class by their lowest node index, so that neighbouring

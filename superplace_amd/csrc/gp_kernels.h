@@ -52,8 +52,6 @@ struct DevDesign {
     T* recWHA = nullptr;
     T* recWHB = nullptr;
     TileGrid tiles;
-    const double* precArea = nullptr;  // area of opt node k (double)
-    const float* precDeg = nullptr;    // pin count of opt node k
     // Nets are renumbered by degree (ascending), so each WA size class below
     // is a contiguous range of nets.  Classes below kWaClasses keep their
     // pins in lane order (waLaneBase; node -1: an empty slot), with their pin

@@ -489,7 +489,6 @@ __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g,
         const double up1 = up ? static_cast<double>(up[n + k]) : 0.0;
         const double gp0 = gp ? static_cast<double>(gp[k]) : 0.0;
         const double gp1 = gp ? static_cast<double>(gp[n + k]) : 0.0;
-        const double pdeg = d.precDeg[k], parea = d.precArea[k];
         const auto pi = load2(d.xy, i);
         const T wi = d.w[i], hi = d.h[i];
         const uint8_t f = d.flags[i];
@@ -563,6 +562,10 @@ __device__ inline void nodeGradBody(const DevDesign<T>& d, const GridGeom<T>& g,
                 }
         }
         const T lam = static_cast<T>(lambda);
+        // the preconditioner's pin count and area (eplace.cpp:178): the
+        // node's pin range and w h (exact in FP64)
+        const double pdeg = static_cast<double>(static_cast<float>(t1 - t0));
+        const double parea = static_cast<double>(wi) * static_cast<double>(hi);
         const T precond = static_cast<T>(1.0 / fmax(1.0, pdeg + static_cast<double>(lambda) * parea));
         const T g0 = precond * (gx - lam * fx);
         const T g1 = precond * (gy - lam * fy);
