@@ -1043,11 +1043,11 @@ void Engine<T>::build(const sp_hip_design& d) {
     bytesPer_[SP_HIP_K_WA_NETS] = static_cast<double>(dd_.waLaneBase[kWaClasses]) * (4 + 2 * ts + 4) +
                                   P * (2 * ts + 2 * ts) +
                                   (dd_.waNetLo[kWaClasses + 1] - dd_.waNetLo[kWaClasses]) * 4.0;
-    // node pass: per node its index, x/y, w/h, flags, pin range, the
-    // preconditioner terms, u/up/gp reads and the grad write, four field bins
-    // (x and y); per pin entry its gradient
+    // node pass: per node its index, x/y, w/h, flags, pin range (two ints),
+    // u/up/gp reads and the grad write, four field bins (x and y); per pin
+    // entry its gradient
     bytesPer_[SP_HIP_K_NODE_GRAD] =
-        n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 8 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
+        n_ * (4 + 2 * ts + 2 * ts + 1 + 4 + 4 + 4 * 2 * ts) + P * 2 * ts + n_ * 4.0 * 2 * ts;
     // update: w/h, A/G/B reads, v/u writes, the node's (x, y) write
     bytesPer_[SP_HIP_K_UPDATE] = n_ * (2 * ts + 3 * 2 * ts + 2 * 2 * ts + 2 * ts);
     // reduce: the block partials of the evaluation (1, 2 and 7 doubles)
