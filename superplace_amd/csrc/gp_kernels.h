@@ -213,8 +213,8 @@ struct Ctl {
 // with atomics, as do charges that fall outside a window (large nodes).
 // `sorted` (with ctl: and ctl->sortNext): launchTileSort ran for this
 // evaluation, so the scatter reads the new order (B) and copies it to A;
-// otherwise it reads A, the order of the last sort with positions kept
-// current by the update (charges of nodes that left their tile's window
+// otherwise it reads A, the order of the last sort, gathering the current
+// positions (charges of nodes that left their tile's window
 // since then go through the atomic grids).  The count of such nodes per
 // block goes to region 2 of ws; k_reduce re-sorts once it passes
 // ctl->spillLimit.
