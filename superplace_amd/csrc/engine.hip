@@ -342,7 +342,7 @@ public:
         const TileGrid& tg = dd_.tiles;
         counts[BK_SORT_COUNT] = static_cast<int>(
             std::min<long>((dd_.nDens + kBlock - 1) / kBlock, kSortBlocks));
-        counts[BK_SORT_PLACE] = counts[BK_SORT_COUNT];
+        counts[BK_SORT_PLACE] = counts[BK_SORT_COUNT] + 1;   // + the publishing block
         counts[BK_SCATTER] = tg.tilesX * tg.tilesY + (dd_.nDens + tg.chunk - 1) / tg.chunk;
         counts[BK_FINALIZE] = static_cast<int>(
             (std::max<long>(static_cast<long>(nb_), tg.nKeys) + kBlock - 1) / kBlock);

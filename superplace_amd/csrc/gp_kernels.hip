@@ -195,8 +195,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(DevDesign<T> d, GridGeom<
 }
 
 // Pass 2: every block scans the counts itself (a few KB from L2), so no
-// separate scan launch is needed.  Block 0 also publishes the key starts and
-// the extra scatter blocks of crowded tiles.
+// separate scan launch is needed.  The grid has one block more than the
+// counting pass: that last block publishes the key starts and the extra
+// scatter blocks of crowded tiles, and places no records.
 template <typename T>
 __device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl,
                                      int vb, int nvb) {
@@ -206,8 +207,27 @@ __device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g
     __shared__ int waveTot[kBlock / 64];
     const TileGrid& tg = d.tiles;
     blockExclusiveScan(tg.count, tg.nKeys, prefix, waveTot);
+    // the last block publishes the key starts and the extra scatter blocks
+    // of crowded tiles while the others place the records
+    const int placers = nvb - 1;
+    if (vb == placers) {
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
+            tg.start[k] = prefix[k];
+            tg.size[k] = tg.count[k];
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
+            const int c = tg.count[k];
+            prefix[k] = c > tg.chunk ? (c - 1) / tg.chunk : 0;
+        }
+        __syncthreads();
+        const int total = blockExclusiveScan(prefix, tg.nKeys, prefix, waveTot);
+        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) tg.extraStart[k] = prefix[k];
+        if (threadIdx.x == 0) tg.extraStart[tg.nKeys] = total;
+        return;
+    }
     const int chunks = (d.nDens + kBlock - 1) / kBlock;
-    for (int chunk = vb; chunk < chunks; chunk += nvb) {
+    for (int chunk = vb; chunk < chunks; chunk += placers) {
         int rec = 0;
         typename Vec2<T>::type p;
         const unsigned key = blockTileHist<T>(d, g, h, chunk, &rec, &p);
@@ -224,21 +244,6 @@ __device__ inline void tilePlaceBody(const DevDesign<T>& d, const GridGeom<T>& g
             store2(d.recWHB, pos, wh.x, wh.y);
         }
         __syncthreads();
-    }
-    if (vb == 0) {
-        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
-            tg.start[k] = prefix[k];
-            tg.size[k] = tg.count[k];
-        }
-        __syncthreads();
-        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) {
-            const int c = tg.count[k];
-            prefix[k] = c > tg.chunk ? (c - 1) / tg.chunk : 0;
-        }
-        __syncthreads();
-        const int total = blockExclusiveScan(prefix, tg.nKeys, prefix, waveTot);
-        for (int k = threadIdx.x; k < tg.nKeys; k += kBlock) tg.extraStart[k] = prefix[k];
-        if (threadIdx.x == 0) tg.extraStart[tg.nKeys] = total;
     }
 }
 
@@ -1006,7 +1011,7 @@ void launchTileSort(const DevDesign<T>& d, const GridGeom<T>& g, const Ctl* ctl,
     const int blocks = static_cast<int>(std::min<long>(gridFor(d.nDens), kSortBlocks));
     launchKernel(k_tile_count<T>, blocks, kBlock, 0, s, d, g, ctl);
     SPHIP_CHECK(hipGetLastError());
-    launchKernel(k_tile_place<T>, blocks, kBlock, 0, s, d, g, ctl);
+    launchKernel(k_tile_place<T>, blocks + 1, kBlock, 0, s, d, g, ctl);   // + the publishing block
     SPHIP_CHECK(hipGetLastError());
 }
 
