@@ -165,7 +165,7 @@ public:
         s_ = takeStream(device_);
         const size_t before = g_devBytes;
         try {
-            build(fillerOrder(d));
+            build(nodeOrder(d));
         } catch (...) {
             release();
             throw;
@@ -413,18 +413,26 @@ private:
         SPHIP_CHECK(hipStreamSynchronize(s_));
     }
     std::vector<T> xyStage_;
-    // Fillers carry no pins, so their order is free: the engine keeps them in
-    // Morton order of their start positions (a 1024 x 1024 lattice over the
-    // core), so that neighbouring threads of the node pass and the update
-    // read neighbouring field bins and write neighbouring density records,
-    // and the first spatial sort starts from a nearly sorted order.  ext_
-    // maps the engine's node index to the caller's (empty: the identity);
-    // positions and node indices cross the C ABI in the caller's order.
+    // The engine's own node order (nodeOrder), which the caller never sees:
+    // ext_ maps the engine's node index to the caller's (empty: the
+    // identity); positions and node indices cross the C ABI in the caller's
+    // order.
+    //  - Fillers carry no pins, so their order is free: they are kept in
+    //    Morton order of their start positions (a 1024 x 1024 lattice over
+    //    the core), so that neighbouring threads of the node pass and the
+    //    update read neighbouring field bins and write neighbouring density
+    //    records, and the first spatial sort starts from a nearly sorted order.
+    //  - The WA pass gathers a net's pins and the node pass reads their
+    //    gradient entries through node indices, so the cells are kept in the
+    //    caller's order where its small nets join nearby indices, and
+    //    renumbered by breadth-first search over the nets where they do not
+    //    (cellsLocal).
     std::vector<int> ext_;
     std::vector<float> ordX_, ordY_, ordW_, ordH_;
     std::vector<uint8_t> ordF_;
+    std::vector<int32_t> ordP2N_, ordN2PStart_, ordN2P_;
     int extOf(int i) const { return ext_.empty() ? i : ext_[i]; }
-    sp_hip_design fillerOrder(const sp_hip_design& d);
+    sp_hip_design nodeOrder(const sp_hip_design& d);
     void build(const sp_hip_design& d);
     void release();
     EvalScalars evaluate(const T* u, const T* up, const T* gp, T* grad, float lambda, float gamma,
@@ -588,29 +596,104 @@ void Engine<T>::release() {
     s_ = nullptr;
 }
 
+namespace {
+
+// Nets of 2 to 4 pins whose movable pins span more than this many node
+// indices gather from separate cache lines in the WA pass.
+constexpr int kLocalSpan = 256;
+
+// True when at most a tenth of the design's 2-4 pin nets span more than
+// kLocalSpan movable node indices (nets built from a hierarchy, or numbered
+// by connectivity, as the benchmark generator's are); false for an order
+// that scatters nets over the index range.
+bool cellsLocal(const sp_hip_design& d) {
+    const int nm = d.num_movable;
+    long small = 0, far = 0;
+    for (int e = 0; e < d.num_nets; ++e) {
+        const int p0 = d.net2pin_start[e], p1 = d.net2pin_start[e + 1];
+        if (p1 - p0 < 2 || p1 - p0 > 4) continue;
+        int lo = nm, hi = -1;
+        for (int p = p0; p < p1; ++p) {
+            const int i = d.pin2node[d.flat_net2pin[p]];
+            if (i < nm) {
+                lo = std::min(lo, i);
+                hi = std::max(hi, i);
+            }
+        }
+        if (hi < lo) continue;
+        ++small;
+        if (hi - lo > kLocalSpan) ++far;
+    }
+    return 10 * far <= small;
+}
+
+// Movable nodes in breadth-first order over the nets (nets above 16 pins
+// carry no locality worth following and are skipped), from the lowest
+// unvisited index each time the queue empties: the nodes of a net are
+// numbered together when its first node is reached.  order[new] = old.
+std::vector<int> bfsCells(const sp_hip_design& d) {
+    constexpr int kMaxDeg = 16;
+    const int nm = d.num_movable;
+    std::vector<int> order;
+    order.reserve(nm);
+    std::vector<char> seen(nm, 0);
+    for (int s = 0; s < nm; ++s) {
+        if (seen[s]) continue;
+        seen[s] = 1;
+        size_t head = order.size();
+        order.push_back(s);
+        while (head < order.size()) {
+            const int u = order[head++];
+            for (int q = d.node2pin_start[u]; q < d.node2pin_start[u + 1]; ++q) {
+                const int e = d.pin2net[d.flat_node2pin[q]];
+                const int p0 = d.net2pin_start[e], p1 = d.net2pin_start[e + 1];
+                if (p1 - p0 > kMaxDeg) continue;
+                for (int p = p0; p < p1; ++p) {
+                    const int v = d.pin2node[d.flat_net2pin[p]];
+                    if (v < nm && !seen[v]) {
+                        seen[v] = 1;
+                        order.push_back(v);
+                    }
+                }
+            }
+        }
+    }
+    return order;
+}
+
+}  // namespace
+
 template <typename T>
-sp_hip_design Engine<T>::fillerOrder(const sp_hip_design& d) {
-    const int nn = d.num_nodes, total = d.num_nodes + d.num_fillers;
-    if (d.num_fillers < 2) return d;
-    const double lx = d.core_lx, ly = d.core_ly;
-    const double w = std::max(1e-30, static_cast<double>(d.core_hx) - lx);
-    const double h = std::max(1e-30, static_cast<double>(d.core_hy) - ly);
-    auto lattice = [](double v) {
-        return static_cast<uint32_t>(std::clamp(static_cast<int>(v * 1024.0), 0, 1023));
-    };
-    auto morton = [](uint32_t ix, uint32_t iy) {
-        uint64_t k = 0;
-        for (int b = 0; b < 10; ++b)
-            k |= static_cast<uint64_t>(((ix >> b) & 1u) | (((iy >> b) & 1u) << 1)) << (2 * b);
-        return k;
-    };
-    std::vector<uint64_t> key(d.num_fillers);
-    for (int f = 0; f < d.num_fillers; ++f)
-        key[f] = morton(lattice((d.node_x[nn + f] - lx) / w), lattice((d.node_y[nn + f] - ly) / h));
+sp_hip_design Engine<T>::nodeOrder(const sp_hip_design& d) {
+    const int nn = d.num_nodes, nm = d.num_movable, total = d.num_nodes + d.num_fillers;
+    const bool renumber = nm > 1 && !cellsLocal(d);
+    if (d.num_fillers < 2 && !renumber) return d;
     ext_.resize(total);
     std::iota(ext_.begin(), ext_.end(), 0);
-    std::stable_sort(ext_.begin() + nn, ext_.end(),
-                     [&](int a, int b) { return key[a - nn] < key[b - nn]; });
+    if (renumber) {
+        const std::vector<int> order = bfsCells(d);
+        std::copy(order.begin(), order.end(), ext_.begin());
+    }
+    if (d.num_fillers >= 2) {
+        const double lx = d.core_lx, ly = d.core_ly;
+        const double w = std::max(1e-30, static_cast<double>(d.core_hx) - lx);
+        const double h = std::max(1e-30, static_cast<double>(d.core_hy) - ly);
+        auto lattice = [](double v) {
+            return static_cast<uint32_t>(std::clamp(static_cast<int>(v * 1024.0), 0, 1023));
+        };
+        auto morton = [](uint32_t ix, uint32_t iy) {
+            uint64_t k = 0;
+            for (int b = 0; b < 10; ++b)
+                k |= static_cast<uint64_t>(((ix >> b) & 1u) | (((iy >> b) & 1u) << 1)) << (2 * b);
+            return k;
+        };
+        std::vector<uint64_t> key(d.num_fillers);
+        for (int f = 0; f < d.num_fillers; ++f)
+            key[f] = morton(lattice((d.node_x[nn + f] - lx) / w),
+                            lattice((d.node_y[nn + f] - ly) / h));
+        std::stable_sort(ext_.begin() + nn, ext_.end(),
+                         [&](int a, int b) { return key[a - nn] < key[b - nn]; });
+    }
     auto gather = [&](auto& dst, const auto* src) {
         dst.resize(total);
         for (int i = 0; i < total; ++i) dst[i] = src[ext_[i]];
@@ -622,6 +705,25 @@ sp_hip_design Engine<T>::fillerOrder(const sp_hip_design& d) {
     p.node_w = gather(ordW_, d.node_w);
     p.node_h = gather(ordH_, d.node_h);
     p.node_flags = gather(ordF_, d.node_flags);
+    if (renumber) {
+        // pins keep their ids; each node keeps its pins in the caller's order
+        std::vector<int> inv(total);
+        for (int i = 0; i < total; ++i) inv[ext_[i]] = i;
+        ordP2N_.resize(d.num_pins);
+        for (int q = 0; q < d.num_pins; ++q) ordP2N_[q] = inv[d.pin2node[q]];
+        ordN2PStart_.assign(total + 1, 0);
+        ordN2P_.resize(d.node2pin_start[total]);
+        for (int i = 0; i < total; ++i) {
+            const int o = ext_[i];
+            const int c = d.node2pin_start[o + 1] - d.node2pin_start[o];
+            ordN2PStart_[i + 1] = ordN2PStart_[i] + c;
+            std::copy(d.flat_node2pin + d.node2pin_start[o], d.flat_node2pin + d.node2pin_start[o + 1],
+                      ordN2P_.begin() + ordN2PStart_[i]);
+        }
+        p.pin2node = ordP2N_.data();
+        p.node2pin_start = ordN2PStart_.data();
+        p.flat_node2pin = ordN2P_.data();
+    }
     return p;
 }
 
