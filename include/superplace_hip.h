@@ -251,10 +251,15 @@ enum { SP_HIP_K_SCATTER = 0, SP_HIP_K_FINALIZE, SP_HIP_K_POISSON, SP_HIP_K_WA_NE
        SP_HIP_K_NODE_GRAD, SP_HIP_K_UPDATE, SP_HIP_K_SORT, SP_HIP_K_REDUCE, SP_HIP_K_COUNT };
 
 /* Builds the device copy of the design and the bin grid for one stage
- * (EPlace::EPlace + BinGrid::initialize, eplace.cpp:205-221). */
+ * (EPlace::EPlace + BinGrid::initialize, eplace.cpp:205-221).  The copy is
+ * kept in an order of the engine's own (fillers by position; cells by
+ * connectivity when the design's small nets join far-apart indices); every
+ * entry point below takes and returns nodes in the caller's numbering. */
 int sp_hip_gp_create(const sp_hip_design* d, const sp_hip_gp_config* cfg, sp_hip_gp** out);
 int sp_hip_gp_info_get(const sp_hip_gp* gp, sp_hip_gp_info* info);
-/* The optimised node indices, in pos order (buildOptSet, eplace.cpp:82-94). */
+/* The optimised node indices (caller numbering) in the engine's pos order,
+ * the order of the per-node vectors of sp_hip_eval_debug: buildOptSet's set
+ * (eplace.cpp:82-94), its order the engine's. */
 int sp_hip_gp_opt_index(const sp_hip_gp* gp, int32_t* idx);
 /* One objAndGrad at host positions x, y of ALL nodes (written as given,
  * like EPlace's scatter), with lambda and gamma.  dbg may be NULL. */
