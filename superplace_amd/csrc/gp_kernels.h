@@ -179,6 +179,8 @@ struct DevRow {   // one metrics row (src/util/metrics.h:12-28)
     unsigned long long ticks;   // device real-time clock when the row was written
 };
 
+// (The decisions keep every field but the tau-ladder arrays in registers:
+// a field added here is added to SPHIP_CTL_SCALARS in gp_kernels.hip too.)
 struct Ctl {
     // cur: table indices the update of this slot uses; nxt: the indices the
     // next evaluation uses (after this slot's step rotation).

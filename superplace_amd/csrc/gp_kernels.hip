@@ -633,87 +633,105 @@ __device__ inline float lambdaMul(double dH, double ref) {
 // the iteration limit.  The update kernel executes the chosen mode with the
 // `cur` indices; `nxt` holds the indices after the step's rotation
 // (nesterov.cpp:109-118).
+// The loop state's scalars and buffer roles in registers for the decisions
+// (thread 0 of the reduction otherwise makes a chain of dependent LDS round
+// trips through Ctl); the tau-ladder arrays, indexed by counts, stay in Ctl.
+#define SPHIP_CTL_SCALARS(X)                                                              \
+    X(mode) X(done) X(status) X(iter) X(tries) X(maxIter) X(maxBt) X(backtrack) X(useBB)   \
+    X(stage) X(window) X(evals) X(finalBuf) X(rollbackIter) X(snapWrite) X(snapSrc)        \
+    X(nSnap) X(nFree) X(a) X(alpha) X(alpha0) X(m) X(momPrev) X(lambda) X(gamma)           \
+    X(targetTau) X(stepX) X(minGain) X(bestTau) X(refTau) X(sinceImprove) X(havePrev)      \
+    X(sortNext) X(spillLimit) X(prevHpwl) X(effRef)
+struct CtlRegs {
+    int cur[B_ROLES], nxt[B_ROLES];
+#define SPHIP_DECL(f) decltype(Ctl::f) f;
+    SPHIP_CTL_SCALARS(SPHIP_DECL)
+#undef SPHIP_DECL
+};
+
 template <typename T>
-__device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
+__device__ __attribute__((always_inline)) inline void decideRegs(CtlRegs& st, Ctl& c,
+                                                                 const double* sc, DevRow* rows) {
 #pragma clang fp contract(off)
-    for (int r = 0; r < B_ROLES; ++r) c.cur[r] = c.nxt[r];
-    c.snapWrite = -1;
+#pragma unroll
+    for (int r = 0; r < B_ROLES; ++r) st.cur[r] = st.nxt[r];
+    st.snapWrite = -1;
     const double tau = sc[0], wl = sc[1], hpwl = sc[2], sumWl = sc[3], sumDen = sc[4];
     const double dx = sqrt(sc[5]), dg = sqrt(sc[6]), sy = sc[7], nonfin = sc[9];
     if (!isfinite(tau) || !isfinite(hpwl) || !isfinite(wl) || !isfinite(sumWl) ||
         !isfinite(sumDen) || nonfin > 0.0) {
-        c.done = 1;
-        c.status = kNonFinite;
-        c.mode = kModeNone;
+        st.done = 1;
+        st.status = kNonFinite;
+        st.mode = kModeNone;
         return;
     }
-    ++c.evals;
-    if (c.backtrack && c.iter > 0 && c.tries < c.maxBt - 1 && dx > 0.0 && dg > 0.0 &&
+    ++st.evals;
+    if (st.backtrack && st.iter > 0 && st.tries < st.maxBt - 1 && dx > 0.0 && dg > 0.0 &&
         isfinite(dg)) {
         const double lip = dx / dg;
-        if (!(lip > 0.95 * c.alpha)) {
-            c.alpha = static_cast<double>(static_cast<T>(lip));
-            c.m = c.momPrev;
-            c.mode = kModeRedo;   // v, u rewritten in place: nxt = cur
-            ++c.tries;
+        if (!(lip > 0.95 * st.alpha)) {
+            st.alpha = static_cast<double>(static_cast<T>(lip));
+            st.m = st.momPrev;
+            st.mode = kModeRedo;   // v, u rewritten in place: nxt = cur
+            ++st.tries;
             return;
         }
     }
-    c.tries = 0;
-    if (c.iter == 0) {
-        c.alpha = c.alpha0;
+    st.tries = 0;
+    if (st.iter == 0) {
+        st.alpha = st.alpha0;
     } else if (dx > 0.0 && dg > 0.0) {
         double next = dx / dg;
-        if (c.useBB) {
+        if (st.useBB) {
             const double yy = dg * dg;
             if (yy > 0.0) {
                 const double bb = sy / yy;
                 if (bb > 0.0 && isfinite(bb)) next = bb;
             }
         }
-        c.alpha = static_cast<double>(static_cast<T>(next));
+        st.alpha = static_cast<double>(static_cast<T>(next));
     }
-    const T aT = static_cast<T>(c.a);
+    const T aT = static_cast<T>(st.a);
     const T aNext = T(0.5) * (T(1) + sqrt(T(4) * aT * aT + T(1)));
     const T mom = (aT - T(1)) / aNext;
-    c.m = static_cast<double>(mom);
-    c.momPrev = c.m;
-    c.a = static_cast<double>(aNext);
-    c.mode = kModeStep;
-    c.nxt[B_U] = c.cur[B_UN];
-    c.nxt[B_V] = c.cur[B_VN];
-    c.nxt[B_UP] = c.cur[B_U];
-    c.nxt[B_VP] = c.cur[B_V];
-    c.nxt[B_UN] = c.cur[B_UP];
-    c.nxt[B_VN] = c.cur[B_VP];
-    c.nxt[B_G] = c.cur[B_GP];
-    c.nxt[B_GP] = c.cur[B_G];
-    const int lastEval = c.cur[B_U];   // the point this evaluation used
+    st.m = static_cast<double>(mom);
+    st.momPrev = st.m;
+    st.a = static_cast<double>(aNext);
+    st.mode = kModeStep;
+    st.nxt[B_U] = st.cur[B_UN];
+    st.nxt[B_V] = st.cur[B_VN];
+    st.nxt[B_UP] = st.cur[B_U];
+    st.nxt[B_VP] = st.cur[B_V];
+    st.nxt[B_UN] = st.cur[B_UP];
+    st.nxt[B_VN] = st.cur[B_VP];
+    st.nxt[B_G] = st.cur[B_GP];
+    st.nxt[B_GP] = st.cur[B_G];
+    const int lastEval = st.cur[B_U];   // the point this evaluation used
 
-    const float evalLambda = c.lambda, evalGamma = c.gamma;
+    const float evalLambda = st.lambda, evalGamma = st.gamma;
     const float ftau = static_cast<float>(tau);
-    const double dH = c.havePrev ? hpwl - c.prevHpwl : 0.0;
-    const float mu = c.havePrev ? lambdaMul(dH, c.effRef) : 1.f;
-    if (c.havePrev) c.lambda *= mu;
-    c.prevHpwl = hpwl;
-    c.havePrev = 1;
-    c.gamma = gammaOf(ftau, c.stepX);
-    if (!isfinite(c.lambda) || !isfinite(c.gamma)) {
-        c.done = 1;
-        c.status = kNonFinite;
-        c.mode = kModeNone;
+    const double dH = st.havePrev ? hpwl - st.prevHpwl : 0.0;
+    const float mu = st.havePrev ? lambdaMul(dH, st.effRef) : 1.f;
+    if (st.havePrev) st.lambda *= mu;
+    st.prevHpwl = hpwl;
+    st.havePrev = 1;
+    st.gamma = gammaOf(ftau, st.stepX);
+    if (!isfinite(st.lambda) || !isfinite(st.gamma)) {
+        st.done = 1;
+        st.status = kNonFinite;
+        st.mode = kModeNone;
         return;
     }
-    const int it = c.iter;
+    const int it = st.iter;
     if (rows) {
         DevRow& r = rows[it];
         r.iter = it;
-        r.evaluations = static_cast<int>(c.evals);
+        r.evaluations = static_cast<int>(st.evals);
         r.hpwl = hpwl;
         r.overflow = ftau;
-        r.lambda = c.lambda;
-        r.gamma = c.gamma;
-        r.step = c.alpha;
+        r.lambda = st.lambda;
+        r.gamma = st.gamma;
+        r.step = st.alpha;
         r.gradWl = static_cast<float>(sumWl);
         r.gradDen = static_cast<float>(sumDen);
         r.evalLambda = evalLambda;
@@ -722,62 +740,84 @@ __device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
         r.mu = mu;
         r.ticks = __builtin_amdgcn_s_memrealtime();
     }
-    c.iter = it + 1;
-    if (c.stage != 1 && ftau < c.targetTau) {   // deliver the look-ahead point
-        c.done = 1;
-        c.status = kConverged;
-        c.finalBuf = lastEval;
-        c.mode = kModeNone;
+    st.iter = it + 1;
+    if (st.stage != 1 && ftau < st.targetTau) {   // deliver the look-ahead point
+        st.done = 1;
+        st.status = kConverged;
+        st.finalBuf = lastEval;
+        st.mode = kModeNone;
         return;
     }
-    if (c.stage != 1 && c.window > 0) {
-        if (ftau < c.refTau * (1.f - c.minGain)) {
-            c.refTau = ftau;
-            c.sinceImprove = 0;
+    if (st.stage != 1 && st.window > 0) {
+        if (ftau < st.refTau * (1.f - st.minGain)) {
+            st.refTau = ftau;
+            st.sinceImprove = 0;
         } else {
-            ++c.sinceImprove;
+            ++st.sinceImprove;
         }
-        c.bestTau = fminf(c.bestTau, ftau);
-        const float tol = c.bestTau * (1.f + c.minGain);
+        st.bestTau = fminf(st.bestTau, ftau);
+        const float tol = st.bestTau * (1.f + st.minGain);
         auto dropFront = [&]() {
-            c.freeSnap[c.nFree++] = c.snapBuf[0];
-            for (int s = 1; s < c.nSnap; ++s) {
+            c.freeSnap[st.nFree++] = c.snapBuf[0];
+            for (int s = 1; s < st.nSnap; ++s) {
                 c.snapBuf[s - 1] = c.snapBuf[s];
                 c.snapTau[s - 1] = c.snapTau[s];
                 c.snapHpwl[s - 1] = c.snapHpwl[s];
                 c.snapIter[s - 1] = c.snapIter[s];
             }
-            --c.nSnap;
+            --st.nSnap;
         };
-        while (c.nSnap > 0 && c.snapTau[0] > tol) dropFront();
-        if (c.nSnap == 0 || ftau * (1.f + 0.5f * c.minGain) <= c.snapTau[c.nSnap - 1]) {
-            if (c.nSnap == kMaxSnaps) dropFront();
-            const int b = c.freeSnap[--c.nFree];
-            c.snapBuf[c.nSnap] = b;
-            c.snapTau[c.nSnap] = ftau;
-            c.snapHpwl[c.nSnap] = hpwl;
-            c.snapIter[c.nSnap] = it;
-            ++c.nSnap;
-            c.snapWrite = b;
-            c.snapSrc = lastEval;
+        while (st.nSnap > 0 && c.snapTau[0] > tol) dropFront();
+        if (st.nSnap == 0 || ftau * (1.f + 0.5f * st.minGain) <= c.snapTau[st.nSnap - 1]) {
+            if (st.nSnap == kMaxSnaps) dropFront();
+            const int b = c.freeSnap[--st.nFree];
+            c.snapBuf[st.nSnap] = b;
+            c.snapTau[st.nSnap] = ftau;
+            c.snapHpwl[st.nSnap] = hpwl;
+            c.snapIter[st.nSnap] = it;
+            ++st.nSnap;
+            st.snapWrite = b;
+            st.snapSrc = lastEval;
         }
-        if (c.sinceImprove >= c.window) {
+        if (st.sinceImprove >= st.window) {
             int pick = 0;
-            for (int s = 1; s < c.nSnap; ++s)
+            for (int s = 1; s < st.nSnap; ++s)
                 if (c.snapHpwl[s] < c.snapHpwl[pick]) pick = s;
-            c.done = 1;
-            c.status = kStagnated;
-            c.finalBuf = c.snapBuf[pick];
-            c.rollbackIter = c.snapIter[pick];
+            st.done = 1;
+            st.status = kStagnated;
+            st.finalBuf = c.snapBuf[pick];
+            st.rollbackIter = c.snapIter[pick];
             // the update still runs this slot so that a snapshot taken now is copied
             return;
         }
     }
-    if (c.iter >= c.maxIter) {
-        c.done = 1;
-        c.status = kIterLimit;
-        c.finalBuf = c.cur[B_VN];   // v after this step (pos = v, nesterov.cpp:121)
+    if (st.iter >= st.maxIter) {
+        st.done = 1;
+        st.status = kIterLimit;
+        st.finalBuf = st.cur[B_VN];   // v after this step (pos = v, nesterov.cpp:121)
     }
+}
+
+template <typename T>
+__device__ void decideStep(Ctl& c, const double* sc, DevRow* rows) {
+    CtlRegs st;
+#pragma unroll
+    for (int r = 0; r < B_ROLES; ++r) {
+        st.cur[r] = c.cur[r];
+        st.nxt[r] = c.nxt[r];
+    }
+#define SPHIP_LOAD(f) st.f = c.f;
+    SPHIP_CTL_SCALARS(SPHIP_LOAD)
+#undef SPHIP_LOAD
+    decideRegs<T>(st, c, sc, rows);
+#pragma unroll
+    for (int r = 0; r < B_ROLES; ++r) {
+        c.cur[r] = st.cur[r];
+        c.nxt[r] = st.nxt[r];
+    }
+#define SPHIP_STORE(f) c.f = st.f;
+    SPHIP_CTL_SCALARS(SPHIP_STORE)
+#undef SPHIP_STORE
 }
 
 // Single block: sums the regions this evaluation wrote into EvalScalars
